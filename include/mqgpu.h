@@ -1,0 +1,187 @@
+/*
+ * mqgpu.h -- C ABI of the MI355X DMRG sweep hot path.
+ *
+ * The library replaces, for one site problem at a time, the three C++
+ * template entry points of the reference's contraction engine and local
+ * eigensolver:
+ *
+ *   contraction::Engine<M,M,S>::site_hamil2(ket, L, R, W)
+ *       dmrg/framework/dmrg/mp_tensors/contractions/non-abelian/engine.hpp:198
+ *       dmrg/framework/dmrg/mp_tensors/contractions/abelian/engine.hpp:197
+ *       called from ietl::mult, dmrg/framework/dmrg/optimize/ietl_lanczos_solver.h:109-115
+ *     -> mq_sigma
+ *   contraction::Engine<M,M,S>::overlap_mpo_left_step(bra, ket, L, W)
+ *       non-abelian/engine.hpp:105, common/move_boundary.hpp:130-187
+ *     -> mq_boundary_left_step
+ *   contraction::Engine<M,M,S>::overlap_mpo_right_step(bra, ket, R, W)
+ *       non-abelian/engine.hpp:116, common/move_boundary.hpp:191-229
+ *     -> mq_boundary_right_step
+ *   solve_ietl_jcd(SiteProblem&, MPSTensor const&, BaseParameters&)
+ *       dmrg/framework/dmrg/optimize/ietl_jacobi_davidson.h:18-77 with
+ *       ietl::jacobi_davidson::calculate_eigenvalue, dmrg/framework/ietl/jacobi.h:364-451
+ *     -> mq_jacobi_davidson
+ *
+ * All data crosses the boundary as flat integer metadata plus FP64 arrays.
+ * Every function returns MQ_OK (0) or a negative error code; a message for
+ * the last error of a context is available from mq_last_error().  No C++
+ * exception crosses the ABI.  A context owns one HIP stream; calls on one
+ * context must not overlap.
+ *
+ * Data conventions (shared by every entry point):
+ *  - A charge is two int32: SU2U1 (N, 2S), TwoU1 (N_alpha, N_beta).
+ *  - An mq_index lists (charge, size) in descending lexicographic charge
+ *    order (dmrg/framework/dmrg/block_matrix/indexing_stable.hpp:82-130).
+ *  - Dense blocks are column-major.
+ *  - An MPS site tensor (ket psi, sigma, Davidson vectors) is stored
+ *    LEFT-PAIRED and CANONICAL: one block per charge c of the right index
+ *    that also occurs in phys*left, in descending charge order, with
+ *    rows = sum over (s, l) with s+l=c of size(s)*size(l) (the fused
+ *    row offsets follow ProductBasis(phys, left), indexing_stable.hpp:305-343)
+ *    and cols = size(c); blocks are contiguous in that order.
+ *  - A boundary is a vector over the MPO bond index b of block matrices;
+ *    block (ket charge, bra charge) holds rows = ket size, cols = bra size
+ *    (common/move_boundary.hpp:165, non-abelian/gemm.hpp:193-201).  The
+ *    boundary descriptor gives, per bond b, a range of blocks with explicit
+ *    element offsets into one flat FP64 arena.  Bonds whose hermitian
+ *    partner is stored instead (MPO herm_info left_skip/right_skip) carry no
+ *    blocks (common/boundary_times_mps.hpp:196-207).
+ */
+#ifndef MQGPU_H
+#define MQGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MQ_OK 0
+#define MQ_ERR_ARG (-1)
+#define MQ_ERR_HIP (-2)
+#define MQ_ERR_PLAN (-3)
+#define MQ_ERR_ALLOC (-4)
+#define MQ_ERR_STATE (-5)
+
+enum mq_symm { MQ_SYMM_TWOU1 = 0, MQ_SYMM_SU2U1 = 1 };
+enum mq_prec { MQ_PREC_FP64 = 0, MQ_PREC_FP32 = 1, MQ_PREC_BF16X3 = 2 };
+
+typedef struct mq_index {
+    int32_t n;
+    const int32_t *charges; /* [2n] */
+    const int32_t *sizes;   /* [n]  */
+} mq_index;
+
+/* Block list: block i = (lc[2i..], rc[2i..]) of rows[i] x cols[i] at data + offset[i]. */
+typedef struct mq_blocks {
+    int32_t n;
+    const int32_t *lc;      /* [2n] */
+    const int32_t *rc;      /* [2n] */
+    const int32_t *rows;    /* [n]  */
+    const int32_t *cols;    /* [n]  */
+    const int64_t *offset;  /* [n]  element offset into the arena */
+} mq_blocks;
+
+/* Boundary structure: blocks of bond b are blocks[bptr[b] .. bptr[b+1]). */
+typedef struct mq_boundary_desc {
+    int32_t aux_dim;
+    const int32_t *bptr;    /* [aux_dim+1] */
+    mq_blocks blocks;
+    int64_t total;          /* arena length in elements */
+} mq_boundary_desc;
+
+/*
+ * MPO tensor (dmrg/framework/dmrg/mp_tensors/mpotensor.h:24-107):
+ * column-compressed (b1,b2) -> list of (operator, scale) terms, operator
+ * table with SU(2) sparse entries (block_matrix/sparse_operator.h:105-168),
+ * bond spins and hermitian partner info (mpotensor_detail.h:118-168).
+ */
+typedef struct mq_mpo_desc {
+    int32_t row_dim, col_dim;
+    const int32_t *col_ptr;     /* [col_dim+1] */
+    const int32_t *row_idx;     /* [nnz], ascending b1 within a column */
+    const int32_t *term_ptr;    /* [nnz+1] */
+    const int32_t *term_op;     /* [nterms] */
+    const double  *term_scale;  /* [nterms] */
+    int32_t n_ops;
+    const int32_t *op_spin;     /* [n_ops] 2S of the reduced operator */
+    const int32_t *op_bptr;     /* [n_ops+1] operator blocks */
+    const int32_t *opb_in;      /* [2*nb] physical input (ket) charge */
+    const int32_t *opb_out;     /* [2*nb] physical output (bra) charge */
+    const int32_t *opb_eptr;    /* [nb+1] sparse entries of each block */
+    const int32_t *ent_row;     /* [ne] row inside the physical in-block */
+    const int32_t *ent_col;     /* [ne] col inside the physical out-block */
+    const int32_t *ent_rspin;   /* [ne] */
+    const int32_t *ent_cspin;   /* [ne] */
+    const double  *ent_coeff;   /* [ne] */
+    const int32_t *left_spin;   /* [row_dim] 2S of MPO bond b1 */
+    const int32_t *right_spin;  /* [col_dim] */
+    const int32_t *left_herm;   /* [row_dim] hermitian partner of b1 */
+    const int32_t *right_herm;  /* [col_dim] */
+    const int32_t *left_phase;  /* [row_dim] */
+    const int32_t *right_phase; /* [col_dim] */
+} mq_mpo_desc;
+
+/* One site problem: ket tensor indices, MPO tensor, boundary structures. */
+typedef struct mq_site_desc {
+    int32_t symm;               /* enum mq_symm */
+    mq_index phys, left, right; /* ket (= bra) tensor indices */
+    mq_mpo_desc mpo;
+    mq_boundary_desc L;         /* aux_dim = mpo.row_dim (unused by right steps) */
+    mq_boundary_desc R;         /* aux_dim = mpo.col_dim (unused by left steps)  */
+} mq_site_desc;
+
+typedef struct mq_ctx mq_ctx;
+typedef struct mq_plan mq_plan;
+
+/* -- context ---------------------------------------------------------- */
+int mq_ctx_create(int device, mq_ctx **out);
+int mq_ctx_destroy(mq_ctx *ctx);
+const char *mq_last_error(const mq_ctx *ctx);
+/* HIP stream (hipStream_t) the context launches on. */
+void *mq_ctx_stream(mq_ctx *ctx);
+int mq_ctx_synchronize(mq_ctx *ctx);
+
+/* -- sigma vector ------------------------------------------------------- */
+/* Plan for site_hamil2 on one site problem; keeps device task lists. */
+int mq_sigma_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out);
+int mq_plan_destroy(mq_plan *plan);
+/* Elements of one MPS vector (canonical left-paired layout). */
+int64_t mq_plan_vec_len(const mq_plan *plan);
+/* Algorithmic FLOPs of the reference schedule (2mnk per block GEMM + 2 len
+ * per axpy, lbtm/rbtm chosen as site_hamil.hpp:35) and minimum HBM bytes. */
+int mq_plan_flops(const mq_plan *plan, double *flops, double *bytes);
+/* Device scratch bytes the plan holds between launches. */
+int64_t mq_plan_workspace_bytes(const mq_plan *plan);
+/* y = H_eff x.  L, R: device boundary arenas; x, y: device vectors. */
+int mq_sigma(mq_plan *plan, const double *L, const double *R,
+             const double *x, double *y, int prec);
+
+/* -- boundary steps ----------------------------------------------------- */
+/* Plan for overlap_mpo_left_step (desc.L = input) or right step (desc.R). */
+int mq_lstep_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out);
+int mq_rstep_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out);
+/* Output boundary structure: aux_dim, per-bond block ranges and offsets.
+ * Arrays stay owned by the plan. */
+int mq_step_out_desc(const mq_plan *plan, mq_boundary_desc *out);
+/* out = step(in, A).  A: device MPS site tensor (canonical left-paired). */
+int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *out);
+
+/* -- local eigensolver -------------------------------------------------- */
+/* Jacobi-Davidson with the reference semantics: modified Gram-Schmidt with
+ * one refinement, Rayleigh matrix, lowest Ritz pair, residual test
+ * |r| <= tol*|theta| or |r| < tol, correction t = -r + (r.u)/(u.u) u,
+ * at most max_iter sigma vectors, no restart.  x0 (device, read) is the
+ * initial guess; u (device, written) the Ritz vector; *theta the eigenvalue.
+ * n_ortho vectors (device, each vec_len) are projected out as
+ * SingleSiteVS::project does (ietl_lanczos_solver.h:90-95). */
+int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R,
+                       const double *x0, double *u, double *theta,
+                       int max_iter, double tol, int prec,
+                       const double *const *ortho, int n_ortho,
+                       int *iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MQGPU_H */

@@ -1,0 +1,140 @@
+"""Two-site DMRG sweep driver (the caller of the hot path).
+
+Restates the microiteration loop of
+SweepBasedAlgorithms/GenericSweepSimulation.h:73-152 with the two-site
+trait of SweepOptimizationTypeTrait.h:87-144, the local solve of
+SweepBasedEnergyMinimization.h:75-100, the split/merge of
+SweepMPSUpdater.h:180-254 and the boundary propagation of
+BoundaryPropagator.h:45-142.  The engine argument provides the hot path
+(sigma vector / Jacobi-Davidson / boundary steps): `GpuEngine` in this
+package, or the CPU oracle in tests.
+"""
+from __future__ import annotations
+
+import time
+
+from .blocks import Index
+from .desc import MQ_SYMM_SU2U1, MQ_SYMM_TWOU1
+from .integrals import to_aligned
+from .mpo import TaggedMPOMaker, make_twosite_mpo
+from .mps import (MPSTensor, left_normalize_and_return, make_twosite_mps, multiply_from_left,
+                  multiply_from_right, normalize_right, random_mps, right_normalize_and_return, split_twosite)
+from .operators import OpTable, SU2Ops, su2_phys_index
+from .terms import su2_terms
+
+
+class Hamiltonian:
+    """Quantum-chemistry Hamiltonian as an MPO (harness input)."""
+
+    def __init__(self, L, nelec, spin, integrals, symm="su2u1"):
+        if symm != "su2u1":
+            raise NotImplementedError("only the SU2U1 model is built in this harness")
+        self.L, self.nelec, self.spin = L, nelec, spin
+        self.symm = MQ_SYMM_SU2U1
+        self.su2 = True
+        ints = to_aligned(integrals)
+        table = OpTable()
+        ops = SU2Ops(table)
+        self.terms = su2_terms(ints, L, nelec, ops)
+        mk = TaggedMPOMaker(L, table, ops.ident, ops.fill, ops.ident_full)
+        for t in self.terms:
+            mk.add_term(t)
+        self.mpo, self.ecore = mk.create_mpo()
+        self.phys = su2_phys_index()
+        self.right_end = (nelec, spin)
+        self._ts = {}
+
+    def ts_mpo(self, p):
+        w = self._ts.get(p)
+        if w is None:
+            w = make_twosite_mpo(self.mpo[p], self.mpo[p + 1], self.phys, self.phys, su2=self.su2)
+            self._ts[p] = w
+        return w
+
+
+def n_microiterations(L):
+    return 1 if L == 2 else 2 * (L - 2)
+
+
+def micro_site(L, i):
+    return i if i < L - 1 else 2 * L - 4 - i
+
+
+class TwoSiteDMRG:
+    def __init__(self, ham: Hamiltonian, engine, max_bond_dimension, init_bond_dimension=5, seed=42,
+                 cutoff=1e-16, jd_tol=1e-8, jd_maxiter=10, mps=None, log=None):
+        self.ham = ham
+        self.engine = engine
+        self.m = max_bond_dimension
+        self.cutoff = cutoff
+        self.tol = jd_tol
+        self.maxiter = jd_maxiter
+        self.log = log
+        L = ham.L
+        if mps is None:
+            mps = random_mps(L, ham.phys, ham.right_end, init_bond_dimension, seed=seed, su2=ham.su2)
+        self.mps = mps
+        normalize_right(self.mps)
+        self.left = [None] * (L + 1)
+        self.right = [None] * (L + 1)
+        self.left[0] = engine.seed_boundary((0, 0))
+        self.right[L] = engine.seed_boundary(ham.right_end)
+        for i in range(L - 1, 0, -1):
+            self.right[i] = engine.right_step(ham.symm, self.mps[i], self.right[i + 1], ham.mpo[i])
+        self.energies = []
+        self.records = []
+
+    def site_problem(self, site):
+        """Two-site psi0, boundaries and MPO of the microiteration at `site`."""
+        psi = make_twosite_mps(self.mps[site], self.mps[site + 1], su2=self.ham.su2)
+        return psi, self.left[site], self.right[site + 2], self.ham.ts_mpo(site)
+
+    def microiteration(self, sweep, i):
+        ham, L = self.ham, self.ham.L
+        site = micro_site(L, i)
+        forward = i < L - 1
+        change = (i == L - 2) or (i == n_microiterations(L) - 1)
+        t0 = time.perf_counter()
+        psi, Lb, Rb, W = self.site_problem(site)
+        theta, u, its = self.engine.jd(ham.symm, psi, Lb, Rb, W, self.maxiter, self.tol)
+        t1 = time.perf_counter()
+        E = theta + ham.ecore
+        ut = MPSTensor.from_flat(psi.phys, psi.left, psi.right, u)
+        l2r = forward and not change
+        A, B, info = split_twosite(ut, ham.phys, ham.phys, self.mps[site].left, self.mps[site + 1].right,
+                                   self.m, self.cutoff, l2r=l2r, su2=ham.su2)
+        self.mps[site], self.mps[site + 1] = A, B
+        if l2r:
+            self.left[site + 1] = self.engine.left_step(ham.symm, self.mps[site], self.left[site], ham.mpo[site])
+            if site + 2 < L:
+                R = left_normalize_and_return(self.mps[site + 1])
+                multiply_from_left(self.mps[site + 2], R)
+            else:
+                left_normalize_and_return(self.mps[site + 1])
+        else:
+            self.right[site + 1] = self.engine.right_step(ham.symm, self.mps[site + 1], self.right[site + 2],
+                                                          ham.mpo[site + 1])
+            if site > 0:
+                Lf = right_normalize_and_return(self.mps[site])
+                multiply_from_right(self.mps[site - 1], Lf)
+            else:
+                right_normalize_and_return(self.mps[site])
+        t2 = time.perf_counter()
+        rec = dict(sweep=sweep, micro=i, site=site, energy=E, jd_iters=its, bond=info["bond_dimension"],
+                   truncated_weight=info["truncated_weight"], t_jd=t1 - t0, t_rest=t2 - t1,
+                   dim=len(u))
+        self.records.append(rec)
+        self.energies.append(E)
+        if self.log:
+            self.log(rec)
+        return E
+
+    def sweep(self, s):
+        for i in range(n_microiterations(self.ham.L)):
+            self.microiteration(s, i)
+        return self.energies[-1]
+
+    def run(self, nsweeps):
+        for s in range(nsweeps):
+            self.sweep(s)
+        return self.energies[-1]
