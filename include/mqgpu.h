@@ -153,9 +153,16 @@ int64_t mq_plan_vec_len(const mq_plan *plan);
 int mq_plan_flops(const mq_plan *plan, double *flops, double *bytes);
 /* Device scratch bytes the plan holds between launches. */
 int64_t mq_plan_workspace_bytes(const mq_plan *plan);
-/* y = H_eff x.  L, R: device boundary arenas; x, y: device vectors. */
+/* FLOPs this implementation executes per sigma (differs from the
+ * algorithmic count: no lazy products, no hermitian-skip savings). */
+double mq_plan_exec_flops(const mq_plan *plan);
+/* y = H_eff x.  L, R: device boundary arenas; x, y: device vectors.
+ * The boundaries are rearranged into GEMM panels at the first call with a
+ * given (L, R) pointer pair and reused while the pair stays the same. */
 int mq_sigma(mq_plan *plan, const double *L, const double *R,
              const double *x, double *y, int prec);
+/* Drop the rearranged boundaries (call after changing L or R in place). */
+int mq_plan_rebind(mq_plan *plan);
 
 /* -- boundary steps ----------------------------------------------------- */
 /* Plan for overlap_mpo_left_step (desc.L = input) or right step (desc.R). */
@@ -180,6 +187,26 @@ int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R,
                        int max_iter, double tol, int prec,
                        const double *const *ortho, int n_ortho,
                        int *iterations);
+
+/* -- host-side plan introspection (no GPU needed) ------------------------ */
+/* Builds the same task lists the device plans execute, on the host only, so
+ * that schedules, FLOP counts and workspace sizes can be inspected (and the
+ * task lists checked) on machines without a GPU.  kind: 0 sigma, 1 left
+ * step, 2 right step.  Arrays stay owned by the handle. */
+typedef struct mq_host_plan mq_host_plan;
+enum mq_plan_part {
+    MQ_PART_ARR_L = 0, MQ_PART_ARR_R = 1, MQ_PART_RESHAPE = 2, MQ_PART_G1_GROUPS = 3,
+    MQ_PART_G1_TILES = 4, MQ_PART_STRIPS = 5, MQ_PART_SRCS = 6, MQ_PART_G2_GROUPS = 7,
+    MQ_PART_G2_TILES = 8, MQ_PART_SCATTER = 9
+};
+int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out);
+int mq_host_plan_part(const mq_host_plan *hp, int part, const void **ptr, int64_t *count, int64_t *elem_bytes);
+/* arena sizes in elements: [vec, LA, RB, PRP, T, C, TMP, OUT, WS1, WS2] */
+int mq_host_plan_sizes(const mq_host_plan *hp, int64_t sizes[10]);
+int mq_host_plan_flops(const mq_host_plan *hp, double *ref_flops, double *exec_flops, double *min_bytes);
+int mq_host_plan_out_desc(const mq_host_plan *hp, mq_boundary_desc *out);
+int mq_host_plan_destroy(mq_host_plan *hp);
+const char *mq_host_last_error(void);
 
 #ifdef __cplusplus
 }

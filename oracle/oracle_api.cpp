@@ -42,6 +42,7 @@ Result *pack(const Boundary &B) {
 }
 
 Problem make_problem(const mq_site_desc *d) {
+    blas_name();  // resolve BLAS (and its environment) before any OpenMP region
     Problem p;
     p.symm = d->symm;
     p.phys = to_index(d->phys);

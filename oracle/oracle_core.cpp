@@ -18,7 +18,13 @@ static dgemm_fn g_dgemm = nullptr;
 static const char *g_blas = "in-repo blocked dgemm";
 static bool g_blas_init = false;
 
+static void blas_init_once();
 static void blas_init() {
+    // std::call_once-style: the first caller (serial API entry) does the work
+    static const bool done = (blas_init_once(), true);
+    (void)done;
+}
+static void blas_init_once() {
     if (g_blas_init) return;
     g_blas_init = true;
     const char *no = getenv("ORACLE_NO_BLAS");
@@ -103,12 +109,13 @@ BlockMatrix transpose(const BlockMatrix &x) {
 // ------------------------------------------------------------------ Wigner
 // Racah single-sum 6j; 9j as a sum over three 6j (Edmonds 6.4.3).
 static long double lfact(int n) {
-    static std::vector<long double> t;
-    if (t.empty()) {
-        t.resize(171);
-        t[0] = 1.0L;
-        for (int i = 1; i < 171; ++i) t[i] = t[i - 1] * i;
-    }
+    // function-local static: initialised once, thread-safe (called inside OpenMP loops)
+    static const std::vector<long double> t = [] {
+        std::vector<long double> v(171);
+        v[0] = 1.0L;
+        for (int i = 1; i < 171; ++i) v[i] = v[i - 1] * i;
+        return v;
+    }();
     if (n < 0 || n > 170) throw std::runtime_error("factorial out of range");
     return t[n];
 }
@@ -427,6 +434,9 @@ static std::vector<double> conjugate_phases(const BlockMatrix &bm, const MPO &mp
     return ret;
 }
 
+template <class F>
+static void omp_loop_counted(int n, F &&f);
+
 // ------------------------------------------------------------------ BoundaryMPSProduct
 // common/boundary_times_mps.hpp:84-218 (SU2 gemms).
 struct BMP {
@@ -449,27 +459,10 @@ struct BMP {
         }
     }
     void populate() {
-        int n = (int)L.size();
-        Counter *parent = g_counter;
-        std::vector<double> fl(omp_get_max_threads(), 0.0);
-#pragma omp parallel
-        {
-            Counter local;
-            if (parent) g_counter = &local;
-#pragma omp for schedule(dynamic, 1)
-            for (int b1 = 0; b1 < n; ++b1) {
-                if (mpo.row_nnz[b1] == 1) continue;
-                one(b1, data[b1]);
-            }
-            if (parent) {
-                fl[omp_get_thread_num()] += local.flops;
-                g_counter = nullptr;
-            }
-        }
-        if (parent) {
-            g_counter = parent;
-            for (double f : fl) parent->flops += f;
-        }
+        omp_loop_counted((int)L.size(), [&](int b1) {
+            if (mpo.row_nnz[b1] == 1) return;
+            one(b1, data[b1]);
+        });
     }
     const BlockMatrix &at(int b1, BlockMatrix &storage) const {
         if (mpo.row_nnz[b1] == 1) {
@@ -561,16 +554,26 @@ static void lbtm_kernel_abelian(int b2, BlockMatrix &ret, const BMP &t, const MP
     }
 }
 
+// omp_for over [0, n) with schedule(dynamic,1) (utils/parallel/loops.hpp:11-33);
+// exceptions are carried out of the parallel region and rethrown.
 template <class F>
 static void omp_loop_counted(int n, F &&f) {
     Counter *parent = g_counter;
     std::vector<double> fl(omp_get_max_threads(), 0.0);
+    std::string err;
 #pragma omp parallel
     {
         Counter local;
         if (parent) g_counter = &local;
 #pragma omp for schedule(dynamic, 1)
-        for (int i = 0; i < n; ++i) f(i);
+        for (int i = 0; i < n; ++i) {
+            try {
+                f(i);
+            } catch (std::exception &e) {
+#pragma omp critical(orc_err)
+                if (err.empty()) err = e.what();
+            }
+        }
         if (parent) {
             fl[omp_get_thread_num()] += local.flops;
             g_counter = nullptr;
@@ -580,6 +583,7 @@ static void omp_loop_counted(int n, F &&f) {
         g_counter = parent;
         for (double v : fl) parent->flops += v;
     }
+    if (!err.empty()) throw std::runtime_error(err);
 }
 
 // non-abelian/site_hamil.hpp:57-147 (and abelian/site_hamil.hpp:15-87)

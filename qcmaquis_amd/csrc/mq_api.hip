@@ -1,0 +1,568 @@
+// C ABI of the MI355X hot path (declared in include/mqgpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "mq_kernels.hpp"
+
+using namespace mq;
+
+struct mq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+namespace {
+
+template <class T>
+struct DevArray {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        free_();
+        n = count;
+        if (count) {
+            if (hipMalloc((void **)&p, sizeof(T) * count) != hipSuccess) {
+                p = nullptr;
+                throw std::runtime_error("hipMalloc failed");
+            }
+        }
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) {
+        alloc(v.size());
+        if (!v.empty() && hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+            throw std::runtime_error("hipMemcpyAsync failed");
+    }
+    void free_() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevArray() { free_(); }
+};
+
+struct DevGemm {
+    DevArray<GemmGroup> groups;
+    DevArray<GemmTile> tiles;
+    DevArray<int> split;
+    int ntiles = 0, nsplit = 0;
+    void upload(const GemmList &g, hipStream_t s) {
+        groups.upload(g.groups, s);
+        tiles.upload(g.tiles, s);
+        std::vector<int> sp;
+        for (size_t i = 0; i < g.groups.size(); ++i)
+            if (g.groups[i].split > 1) sp.push_back((int)i);
+        split.upload(sp, s);
+        ntiles = (int)g.tiles.size();
+        nsplit = (int)sp.size();
+    }
+};
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+struct mq_plan {
+    mq_ctx *ctx = nullptr;
+    HostPlan hp;
+    DevArray<Copy2D> arrL, arrR, reshape, scatter;
+    DevArray<Strip> strips;
+    DevArray<Src> srcs;
+    DevGemm g1, g2;
+    DevArray<double> la, rb, prp, t, c, tmp, ws;
+    const double *bound_L = nullptr, *bound_R = nullptr;
+    // Davidson workspace
+    DevArray<double> jd_vecs, jd_partial, jd_coef;
+    DevArray<const double *> jd_ptrs;
+    int jd_cap = 0;
+};
+
+static int fail(mq_ctx *ctx, const std::exception &e, int code) {
+    if (ctx) ctx->err = e.what();
+    return code;
+}
+
+extern "C" {
+
+int mq_ctx_create(int device, mq_ctx **out) {
+    if (!out) return MQ_ERR_ARG;
+    try {
+        std::unique_ptr<mq_ctx> c(new mq_ctx());
+        c->device = device;
+        check(hipSetDevice(device), "hipSetDevice");
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        *out = c.release();
+        return MQ_OK;
+    } catch (std::exception &e) {
+        fprintf(stderr, "mq_ctx_create: %s\n", e.what());
+        return MQ_ERR_HIP;
+    }
+}
+
+int mq_ctx_destroy(mq_ctx *ctx) {
+    if (!ctx) return MQ_ERR_ARG;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MQ_OK;
+}
+
+const char *mq_last_error(const mq_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void *mq_ctx_stream(mq_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int mq_ctx_synchronize(mq_ctx *ctx) {
+    if (!ctx) return MQ_ERR_ARG;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->err = hipGetErrorString(e);
+        return MQ_ERR_HIP;
+    }
+    return MQ_OK;
+}
+
+static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int kind) {
+    if (!ctx || !desc || !out) return MQ_ERR_ARG;
+    try {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        std::unique_ptr<mq_plan> p(new mq_plan());
+        p->ctx = ctx;
+        const int tm = 64, tn = 64, kc = 2048;
+        if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc);
+        else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc);
+        else p->hp = build_rstep_plan(*desc, tm, tn, kc);
+        hipStream_t s = ctx->stream;
+        HostPlan &h = p->hp;
+        p->arrL.upload(h.arrL, s);
+        p->arrR.upload(h.arrR, s);
+        p->reshape.upload(h.reshape, s);
+        p->scatter.upload(h.scatter, s);
+        p->strips.upload(h.wapply.strips, s);
+        p->srcs.upload(h.wapply.srcs, s);
+        p->g1.upload(h.g1, s);
+        p->g2.upload(h.g2, s);
+        p->la.alloc(h.n_la);
+        p->rb.alloc(h.n_rb);
+        p->prp.alloc(h.n_prp);
+        p->t.alloc(h.n_t);
+        p->c.alloc(h.n_c);
+        p->tmp.alloc(h.n_tmp);
+        p->ws.alloc(std::max(h.g1.ws, h.g2.ws));
+        check(hipStreamSynchronize(s), "plan upload");
+        *out = p.release();
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(ctx, e, MQ_ERR_PLAN);
+    }
+}
+
+int mq_sigma_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out) {
+    return make_plan(ctx, desc, out, PLAN_SIGMA);
+}
+int mq_lstep_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out) {
+    return make_plan(ctx, desc, out, PLAN_LSTEP);
+}
+int mq_rstep_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out) {
+    return make_plan(ctx, desc, out, PLAN_RSTEP);
+}
+
+int mq_plan_destroy(mq_plan *plan) {
+    if (!plan) return MQ_ERR_ARG;
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    delete plan;
+    return MQ_OK;
+}
+
+int64_t mq_plan_vec_len(const mq_plan *plan) { return plan ? plan->hp.vec_len : -1; }
+
+int mq_plan_flops(const mq_plan *plan, double *flops, double *bytes) {
+    if (!plan) return MQ_ERR_ARG;
+    if (flops) *flops = plan->hp.ref_flops;
+    if (bytes) *bytes = plan->hp.min_bytes;
+    return MQ_OK;
+}
+
+int64_t mq_plan_workspace_bytes(const mq_plan *plan) {
+    if (!plan) return -1;
+    const HostPlan &h = plan->hp;
+    return 8 * (h.n_la + h.n_rb + h.n_prp + h.n_t + h.n_c + h.n_tmp + std::max(h.g1.ws, h.g2.ws));
+}
+
+// Executed (not algorithmic) FLOPs of this implementation.
+double mq_plan_exec_flops(const mq_plan *plan) { return plan ? plan->hp.exec_flops : -1; }
+
+static void sigma_launch(mq_plan *p, const double *L, const double *R, const double *x, double *y) {
+    hipStream_t s = p->ctx->stream;
+    HostPlan &h = p->hp;
+    if (L != p->bound_L || R != p->bound_R) {
+        check(launch_copy2d(p->arrL.p, (int)h.arrL.size(), L, p->la.p, s), "arrange L");
+        check(launch_copy2d(p->arrR.p, (int)h.arrR.size(), R, p->rb.p, s), "arrange R");
+        p->bound_L = L;
+        p->bound_R = R;
+    }
+    check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s), "reshape");
+    check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la.p, p->prp.p,
+                      p->t.p, p->ws.p, s),
+          "gemm T");
+    check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t.p, p->c.p, s), "W apply");
+    check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c.p, p->rb.p, y,
+                      p->ws.p, s),
+          "gemm R");
+}
+
+int mq_sigma(mq_plan *plan, const double *L, const double *R, const double *x, double *y, int prec) {
+    if (!plan || !x || !y) return MQ_ERR_ARG;
+    if (plan->hp.kind != PLAN_SIGMA) {
+        plan->ctx->err = "mq_sigma: plan is not a sigma plan";
+        return MQ_ERR_ARG;
+    }
+    if (prec != MQ_PREC_FP64) {
+        plan->ctx->err = "mq_sigma: only MQ_PREC_FP64 is implemented";
+        return MQ_ERR_ARG;
+    }
+    try {
+        sigma_launch(plan, L, R, x, y);
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+// Forget the arranged boundaries (call after modifying L or R in place).
+int mq_plan_rebind(mq_plan *plan) {
+    if (!plan) return MQ_ERR_ARG;
+    plan->bound_L = plan->bound_R = nullptr;
+    return MQ_OK;
+}
+
+int mq_step_out_desc(const mq_plan *plan, mq_boundary_desc *out) {
+    if (!plan || !out || plan->hp.kind == PLAN_SIGMA) return MQ_ERR_ARG;
+    const HostPlan &h = plan->hp;
+    out->aux_dim = h.out_aux;
+    out->bptr = h.out_bptr.data();
+    out->blocks.n = (int32_t)h.out_rows.size();
+    out->blocks.lc = h.out_lc.data();
+    out->blocks.rc = h.out_rc.data();
+    out->blocks.rows = h.out_rows.data();
+    out->blocks.cols = h.out_cols.data();
+    out->blocks.offset = h.out_off.data();
+    out->total = h.n_out;
+    return MQ_OK;
+}
+
+int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *out) {
+    if (!plan || !in || !A || (!out && plan->hp.n_out > 0)) return MQ_ERR_ARG;
+    HostPlan &h = plan->hp;
+    if (h.kind == PLAN_SIGMA) return MQ_ERR_ARG;
+    hipStream_t s = plan->ctx->stream;
+    try {
+        check(launch_copy2d(plan->reshape.p, (int)h.reshape.size(), A, plan->prp.p, s), "reshape");
+        if (h.kind == PLAN_LSTEP) {
+            check(launch_copy2d(plan->arrL.p, (int)h.arrL.size(), in, plan->la.p, s), "arrange L");
+            check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
+                              plan->g1.nsplit, plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, s),
+                  "gemm T");
+            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p, s),
+                  "W apply");
+            check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
+                              plan->g2.nsplit, plan->c.p, A, plan->tmp.p, plan->ws.p, s),
+                  "gemm out");
+        } else {
+            check(launch_copy2d(plan->arrR.p, (int)h.arrR.size(), in, plan->rb.p, s), "arrange R");
+            check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
+                              plan->g1.nsplit, A, plan->rb.p, plan->t.p, plan->ws.p, s),
+                  "gemm t");
+            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p, s),
+                  "W apply");
+            check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
+                              plan->g2.nsplit, plan->c.p, plan->prp.p, plan->tmp.p, plan->ws.p, s),
+                  "gemm out");
+        }
+        check(launch_copy2d(plan->scatter.p, (int)h.scatter.size(), plan->tmp.p, out, s), "scatter");
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ Jacobi-Davidson
+namespace {
+
+struct JD {
+    mq_plan *p;
+    hipStream_t s;
+    int64_t n;
+    int nblk;
+    std::vector<double> host_partial;
+
+    double *vec(int i) { return p->jd_vecs.p + (int64_t)i * n; }
+
+    // dots of pairs (x_k, y_k), summed on the host in a fixed order
+    std::vector<double> dots(const std::vector<const double *> &xs, const std::vector<const double *> &ys) {
+        int np = (int)xs.size();
+        std::vector<const double *> ptrs(xs);
+        ptrs.insert(ptrs.end(), ys.begin(), ys.end());
+        check(hipMemcpyAsync(p->jd_ptrs.p, ptrs.data(), sizeof(double *) * ptrs.size(), hipMemcpyHostToDevice, s),
+              "dot ptrs");
+        check(launch_multidot(p->jd_ptrs.p, p->jd_ptrs.p + np, np, n, nblk, p->jd_partial.p, s), "multidot");
+        host_partial.resize((size_t)np * nblk);
+        check(hipMemcpyAsync(host_partial.data(), p->jd_partial.p, sizeof(double) * np * nblk, hipMemcpyDeviceToHost,
+                             s),
+              "dot partials");
+        check(hipStreamSynchronize(s), "dot sync");
+        std::vector<double> r(np, 0.0);
+        for (int k = 0; k < np; ++k) {
+            long double acc = 0;
+            for (int b = 0; b < nblk; ++b) acc += host_partial[(size_t)k * nblk + b];
+            r[k] = (double)acc;
+        }
+        return r;
+    }
+    double dot(const double *x, const double *y) { return dots({x}, {y})[0]; }
+    // out = beta*out + sum c_j v_j
+    void lincomb(const std::vector<const double *> &vs, const std::vector<double> &c, double beta, double *out) {
+        int nv = (int)vs.size();
+        check(hipMemcpyAsync(p->jd_ptrs.p + 128, vs.data(), sizeof(double *) * nv, hipMemcpyHostToDevice, s),
+              "lc ptrs");
+        check(hipMemcpyAsync(p->jd_coef.p, c.data(), sizeof(double) * nv, hipMemcpyHostToDevice, s), "lc coef");
+        check(launch_lincomb(p->jd_ptrs.p + 128, p->jd_coef.p, nv, n, beta, out, s), "lincomb");
+        // the host arrays are reused by the next call: wait for the copies
+        check(hipStreamSynchronize(s), "lincomb sync");
+    }
+};
+
+// cyclic Jacobi eigen-decomposition of a small symmetric matrix (column-major)
+void sym_eig(int n, std::vector<double> A, std::vector<double> &w, std::vector<double> &V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) off += A[(size_t)q * n + p] * A[(size_t)q * n + p];
+        if (off < 1e-30) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = A[(size_t)q * n + p];
+                if (std::fabs(apq) < 1e-300) continue;
+                double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+                double th = (aqq - app) / (2 * apq);
+                double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double akp = A[(size_t)p * n + k], akq = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * akp - sn * akq;
+                    A[(size_t)q * n + k] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = A[(size_t)k * n + p], aqk = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * apk - sn * aqk;
+                    A[(size_t)k * n + q] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = V[(size_t)p * n + k], vkq = V[(size_t)q * n + k];
+                    V[(size_t)p * n + k] = c * vkp - sn * vkq;
+                    V[(size_t)q * n + k] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; ++i) w[i] = A[(size_t)i * n + i];
+}
+
+}  // namespace
+
+extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R, const double *x0, double *u_out,
+                                  double *theta_out, int max_iter, double tol, int prec, const double *const *ortho,
+                                  int n_ortho, int *iterations) {
+    if (!plan || !x0 || !u_out || !theta_out || max_iter < 1 || max_iter > 60 || n_ortho < 0 || n_ortho > 60)
+        return MQ_ERR_ARG;
+    if (plan->hp.kind != PLAN_SIGMA || prec != MQ_PREC_FP64) {
+        plan->ctx->err = "mq_jacobi_davidson: needs an FP64 sigma plan";
+        return MQ_ERR_ARG;
+    }
+    try {
+        JD jd;
+        jd.p = plan;
+        jd.s = plan->ctx->stream;
+        jd.n = plan->hp.vec_len;
+        jd.nblk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (jd.n + 255) / 256));
+        const int64_t n = jd.n;
+        int need = 2 * max_iter + 2;
+        if (plan->jd_cap < need) {
+            plan->jd_vecs.alloc((size_t)need * n);
+            plan->jd_cap = need;
+        }
+        if (!plan->jd_partial.p) {
+            plan->jd_partial.alloc((size_t)128 * 256);
+            plan->jd_coef.alloc(256);
+            plan->jd_ptrs.alloc(256);
+        }
+        auto V = [&](int i) { return jd.vec(i); };
+        auto VA = [&](int i) { return jd.vec(max_iter + i); };
+        double *u = jd.vec(2 * max_iter), *uA = jd.vec(2 * max_iter + 1);
+        std::vector<double> oo(n_ortho);
+        for (int k = 0; k < n_ortho; ++k) oo[k] = jd.dot(ortho[k], ortho[k]);
+        auto project = [&](double *t) {  // SingleSiteVS::project (ietl_lanczos_solver.h:90-95)
+            for (int k = 0; k < n_ortho; ++k) {
+                double f = jd.dot(ortho[k], t) / oo[k];
+                jd.lincomb({ortho[k]}, {-f}, 1.0, t);
+            }
+        };
+        check(hipMemcpyAsync(V(0), x0, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "x0");
+        project(V(0));
+        std::vector<double> M((size_t)max_iter * max_iter, 0.0);
+        int it = 0;
+        double theta = 0;
+        while (true) {
+            double *t = V(it);
+            double tau = std::sqrt(jd.dot(t, t));
+            for (int i = 0; i < it; ++i) jd.lincomb({V(i)}, {-jd.dot(V(i), t)}, 1.0, t);
+            if (std::sqrt(jd.dot(t, t)) < 0.25 * tau)
+                for (int i = 0; i < it; ++i) jd.lincomb({V(i)}, {-jd.dot(V(i), t)}, 1.0, t);
+            project(t);
+            double nt = std::sqrt(jd.dot(t, t));
+            jd.lincomb({t}, {1.0 / nt - 1.0}, 1.0, t);
+            sigma_launch(plan, L, R, t, VA(it));
+            std::vector<const double *> xs, ys;
+            for (int i = 0; i <= it; ++i) {
+                xs.push_back(V(i));
+                ys.push_back(VA(it));
+            }
+            std::vector<double> col = jd.dots(xs, ys);
+            for (int i = 0; i <= it; ++i) M[(size_t)it * max_iter + i] = col[i];
+            int dim = it + 1;
+            std::vector<double> A((size_t)dim * dim);
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j <= i; ++j) {
+                    double v = M[(size_t)i * max_iter + j];
+                    A[(size_t)i * dim + j] = v;
+                    A[(size_t)j * dim + i] = v;
+                }
+            std::vector<double> w, Z;
+            sym_eig(dim, A, w, Z);
+            int lo = 0;
+            for (int i = 1; i < dim; ++i)
+                if (w[i] < w[lo]) lo = i;
+            theta = w[lo];
+            std::vector<double> sv(Z.begin() + (size_t)lo * dim, Z.begin() + (size_t)(lo + 1) * dim);
+            int big = 0;
+            for (int i = 1; i < dim; ++i)
+                if (std::fabs(sv[i]) > std::fabs(sv[big])) big = i;
+            if (sv[big] < 0)
+                for (auto &v : sv) v = -v;
+            std::vector<const double *> vv, va;
+            for (int j = 0; j < dim; ++j) {
+                vv.push_back(V(j));
+                va.push_back(VA(j));
+            }
+            jd.lincomb(vv, sv, 0.0, u);
+            jd.lincomb(va, sv, 0.0, uA);
+            project(uA);
+            jd.lincomb({u}, {-theta}, 1.0, uA);  // r = uA - theta u
+            ++it;
+            double rn = std::sqrt(jd.dot(uA, uA));
+            if (rn <= tol * std::fabs(theta) || rn < tol || it >= max_iter) break;
+            std::vector<double> d2 = jd.dots({uA, u}, {u, u});
+            jd.lincomb({uA, u}, {-1.0, d2[0] / d2[1]}, 0.0, V(it));
+        }
+        check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
+        check(hipStreamSynchronize(jd.s), "jd end");
+        *theta_out = theta;
+        if (iterations) *iterations = it;
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+// ------------------------------------------------------------------ host-side introspection
+struct mq_host_plan {
+    HostPlan hp;
+};
+
+static thread_local std::string g_host_err;
+
+extern "C" {
+
+const char *mq_host_last_error(void) { return g_host_err.c_str(); }
+
+int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out) {
+    if (!desc || !out) return MQ_ERR_ARG;
+    try {
+        std::unique_ptr<mq_host_plan> h(new mq_host_plan());
+        const int tm = 64, tn = 64, kc = 2048;
+        if (kind == PLAN_SIGMA) h->hp = build_sigma_plan(*desc, tm, tn, kc);
+        else if (kind == PLAN_LSTEP) h->hp = build_lstep_plan(*desc, tm, tn, kc);
+        else if (kind == PLAN_RSTEP) h->hp = build_rstep_plan(*desc, tm, tn, kc);
+        else return MQ_ERR_ARG;
+        *out = h.release();
+        return MQ_OK;
+    } catch (std::exception &e) {
+        g_host_err = e.what();
+        return MQ_ERR_PLAN;
+    }
+}
+
+int mq_host_plan_part(const mq_host_plan *h, int part, const void **ptr, int64_t *count, int64_t *elem) {
+    if (!h || !ptr || !count || !elem) return MQ_ERR_ARG;
+    const HostPlan &p = h->hp;
+    switch (part) {
+        case MQ_PART_ARR_L: *ptr = p.arrL.data(); *count = p.arrL.size(); *elem = sizeof(Copy2D); break;
+        case MQ_PART_ARR_R: *ptr = p.arrR.data(); *count = p.arrR.size(); *elem = sizeof(Copy2D); break;
+        case MQ_PART_RESHAPE: *ptr = p.reshape.data(); *count = p.reshape.size(); *elem = sizeof(Copy2D); break;
+        case MQ_PART_G1_GROUPS: *ptr = p.g1.groups.data(); *count = p.g1.groups.size(); *elem = sizeof(GemmGroup); break;
+        case MQ_PART_G1_TILES: *ptr = p.g1.tiles.data(); *count = p.g1.tiles.size(); *elem = sizeof(GemmTile); break;
+        case MQ_PART_STRIPS: *ptr = p.wapply.strips.data(); *count = p.wapply.strips.size(); *elem = sizeof(Strip); break;
+        case MQ_PART_SRCS: *ptr = p.wapply.srcs.data(); *count = p.wapply.srcs.size(); *elem = sizeof(Src); break;
+        case MQ_PART_G2_GROUPS: *ptr = p.g2.groups.data(); *count = p.g2.groups.size(); *elem = sizeof(GemmGroup); break;
+        case MQ_PART_G2_TILES: *ptr = p.g2.tiles.data(); *count = p.g2.tiles.size(); *elem = sizeof(GemmTile); break;
+        case MQ_PART_SCATTER: *ptr = p.scatter.data(); *count = p.scatter.size(); *elem = sizeof(Copy2D); break;
+        default: return MQ_ERR_ARG;
+    }
+    return MQ_OK;
+}
+
+int mq_host_plan_sizes(const mq_host_plan *h, int64_t s[10]) {
+    if (!h || !s) return MQ_ERR_ARG;
+    const HostPlan &p = h->hp;
+    int64_t v[10] = {p.vec_len, p.n_la, p.n_rb, p.n_prp, p.n_t, p.n_c, p.n_tmp, p.n_out, p.g1.ws, p.g2.ws};
+    for (int i = 0; i < 10; ++i) s[i] = v[i];
+    return MQ_OK;
+}
+
+int mq_host_plan_flops(const mq_host_plan *h, double *ref, double *exec, double *bytes) {
+    if (!h) return MQ_ERR_ARG;
+    if (ref) *ref = h->hp.ref_flops;
+    if (exec) *exec = h->hp.exec_flops;
+    if (bytes) *bytes = h->hp.min_bytes;
+    return MQ_OK;
+}
+
+int mq_host_plan_out_desc(const mq_host_plan *h, mq_boundary_desc *out) {
+    if (!h || !out || h->hp.kind == PLAN_SIGMA) return MQ_ERR_ARG;
+    const HostPlan &p = h->hp;
+    out->aux_dim = p.out_aux;
+    out->bptr = p.out_bptr.data();
+    out->blocks.n = (int32_t)p.out_rows.size();
+    out->blocks.lc = p.out_lc.data();
+    out->blocks.rc = p.out_rc.data();
+    out->blocks.rows = p.out_rows.data();
+    out->blocks.cols = p.out_cols.data();
+    out->blocks.offset = p.out_off.data();
+    out->total = p.n_out;
+    return MQ_OK;
+}
+
+int mq_host_plan_destroy(mq_host_plan *h) {
+    delete h;
+    return MQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// Execution plans of the hot path: task lists for the device kernels,
+// built once per site problem on the host.
+#pragma once
+
+#include "mq_host.hpp"
+
+namespace mq {
+
+// ---- device task records (plain old data, copied to the GPU as arrays) ----
+
+// dst[i + j*ldd] = scale * (trans ? src[j + i*lds] : src[i + j*lds]), i < rows, j < cols
+struct Copy2D {
+    int64_t dst, src;
+    int rows, cols, ldd, lds;
+    int trans;
+    double scale;
+};
+
+// C(M x N) = op(A)(M x K) * op(B)(K x N), column-major with leading dims.
+// transA: A stored K x M; transB: B stored N x K.  split > 1: K is cut into
+// `split` chunks whose partial tiles go to the workspace at `part`.
+struct GemmGroup {
+    int64_t a, b, c, part;
+    int M, N, K;
+    int lda, ldb, ldc;
+    int transA, transB;
+    int split;
+    int kchunk;
+};
+struct GemmTile {
+    int g;
+    int mt, nt, ks;
+};
+// partial-sum reduction of one split group: C = sum_s part[s]
+struct ReduceTask {
+    int g;
+};
+
+// dst(rows x cols, ldd) = sum over sources of coef * src(rows x cols, lds)
+struct Strip {
+    int64_t dst;
+    int rows, cols, ldd;
+    int s0, s1;
+};
+struct Src {
+    int64_t off;
+    int ld;
+    double coef;
+};
+
+struct GemmList {
+    std::vector<GemmGroup> groups;
+    std::vector<GemmTile> tiles;
+    int64_t ws = 0;  // workspace elements for split partials
+    double flops() const {
+        double f = 0;
+        for (auto &g : groups) f += 2.0 * g.M * g.N * g.K;
+        return f;
+    }
+};
+struct StripList {
+    std::vector<Strip> strips;
+    std::vector<Src> srcs;
+};
+
+enum PlanKind { PLAN_SIGMA = 0, PLAN_LSTEP = 1, PLAN_RSTEP = 2 };
+
+// Host-side plan; the API layer uploads the task arrays.
+struct HostPlan {
+    int kind = PLAN_SIGMA;
+    int symm = MQ_SYMM_SU2U1;
+    int64_t vec_len = 0;
+    // arena sizes (elements)
+    int64_t n_la = 0, n_rb = 0, n_prp = 0, n_t = 0, n_c = 0, n_tmp = 0, n_out = 0;
+    // stage task lists
+    std::vector<Copy2D> arrL;   // L boundary -> LA  (sigma, lstep)
+    std::vector<Copy2D> arrR;   // R boundary -> RB  (sigma) / RC (rstep)
+    std::vector<Copy2D> reshape;  // x -> PRP
+    GemmList g1;                // T = LA * PRP    (sigma, lstep) ; t = X * RC (rstep)
+    StripList wapply;           // T -> C          / t -> BUF
+    GemmList g2;                // y = C * RB      / tmp = C^T X  / tmp = BUF * PRP^T
+    std::vector<Copy2D> scatter;  // tmp -> output boundary (steps)
+    // output boundary structure (steps)
+    int out_aux = 0;
+    std::vector<int32_t> out_bptr, out_lc, out_rc, out_rows, out_cols;
+    std::vector<int64_t> out_off;
+    // algorithmic accounting (reference schedule)
+    double ref_flops = 0, min_bytes = 0;
+    // executed FLOPs of this implementation (for reporting)
+    double exec_flops = 0;
+};
+
+HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
+HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
+HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
+
+// FLOPs of the reference schedule (site_hamil.hpp:28-204 for sigma, move_boundary.hpp for steps)
+double reference_sigma_flops(const mq_site_desc &d);
+
+}  // namespace mq

@@ -1,0 +1,285 @@
+"""Python binding of libmqgpu (the MI355X hot path) and the GPU engine.
+
+Device memory is held in torch tensors on the current HIP device; the
+library launches on its own stream, and this module synchronises that
+stream before torch reads results.  There is no CPU fallback: if the
+library or a GPU is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .desc import BoundaryLayout, SiteDesc, MQ_PREC_FP64, f64p, i32p, i64p, mq_boundary_desc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "_lib", "libmqgpu.so")
+
+_lib = None
+
+
+class MQError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise MQError(f"libmqgpu.so not built ({LIBPATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIBPATH)
+        vp = C.c_void_p
+        L.mq_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.mq_ctx_destroy.argtypes = [vp]
+        L.mq_last_error.argtypes = [vp]
+        L.mq_last_error.restype = C.c_char_p
+        L.mq_ctx_stream.argtypes = [vp]
+        L.mq_ctx_stream.restype = vp
+        L.mq_ctx_synchronize.argtypes = [vp]
+        for f in ("mq_sigma_plan_create", "mq_lstep_plan_create", "mq_rstep_plan_create"):
+            getattr(L, f).argtypes = [vp, vp, C.POINTER(vp)]
+        L.mq_plan_destroy.argtypes = [vp]
+        L.mq_plan_vec_len.argtypes = [vp]
+        L.mq_plan_vec_len.restype = C.c_int64
+        L.mq_plan_flops.argtypes = [vp, f64p, f64p]
+        L.mq_plan_workspace_bytes.argtypes = [vp]
+        L.mq_plan_workspace_bytes.restype = C.c_int64
+        L.mq_plan_exec_flops.argtypes = [vp]
+        L.mq_plan_exec_flops.restype = C.c_double
+        L.mq_sigma.argtypes = [vp, vp, vp, vp, vp, C.c_int]
+        L.mq_plan_rebind.argtypes = [vp]
+        L.mq_step_out_desc.argtypes = [vp, C.POINTER(mq_boundary_desc)]
+        L.mq_boundary_step.argtypes = [vp, vp, vp, vp]
+        L.mq_jacobi_davidson.argtypes = [vp, vp, vp, vp, vp, f64p, C.c_int, C.c_double, C.c_int,
+                                         C.POINTER(vp), C.c_int, C.POINTER(C.c_int)]
+        L.mq_host_plan_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.mq_host_plan_part.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.mq_host_plan_sizes.argtypes = [vp, i64p]
+        L.mq_host_plan_flops.argtypes = [vp, f64p, f64p, f64p]
+        L.mq_host_plan_out_desc.argtypes = [vp, C.POINTER(mq_boundary_desc)]
+        L.mq_host_plan_destroy.argtypes = [vp]
+        L.mq_host_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------------------- host plans
+COPY2D = np.dtype([("dst", "<i8"), ("src", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("ldd", "<i4"),
+                   ("lds", "<i4"), ("trans", "<i4"), ("scale", "<f8")], align=True)
+GEMMGROUP = np.dtype([("a", "<i8"), ("b", "<i8"), ("c", "<i8"), ("part", "<i8"), ("M", "<i4"), ("N", "<i4"),
+                      ("K", "<i4"), ("lda", "<i4"), ("ldb", "<i4"), ("ldc", "<i4"), ("transA", "<i4"),
+                      ("transB", "<i4"), ("split", "<i4"), ("kchunk", "<i4")], align=True)
+GEMMTILE = np.dtype([("g", "<i4"), ("mt", "<i4"), ("nt", "<i4"), ("ks", "<i4")], align=True)
+STRIP = np.dtype([("dst", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("ldd", "<i4"), ("s0", "<i4"),
+                  ("s1", "<i4")], align=True)
+SRC = np.dtype([("off", "<i8"), ("ld", "<i4"), ("coef", "<f8")], align=True)
+PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D]
+PART_NAMES = ["arrL", "arrR", "reshape", "g1_groups", "g1_tiles", "strips", "srcs", "g2_groups", "g2_tiles",
+              "scatter"]
+KIND = {"sigma": 0, "lstep": 1, "rstep": 2}
+
+
+class HostPlan:
+    """Task lists of a plan, built on the host (no GPU needed)."""
+
+    def __init__(self, site: SiteDesc, kind: str):
+        L = lib()
+        h = C.c_void_p()
+        if L.mq_host_plan_create(site.ptr(), KIND[kind], C.byref(h)) != 0:
+            raise MQError(L.mq_host_last_error().decode())
+        try:
+            self.parts = {}
+            for i, (name, dt) in enumerate(zip(PART_NAMES, PART_DTYPES)):
+                ptr, cnt, el = C.c_void_p(), C.c_int64(), C.c_int64()
+                L.mq_host_plan_part(h, i, C.byref(ptr), C.byref(cnt), C.byref(el))
+                if el.value != dt.itemsize:
+                    raise MQError(f"task record size mismatch for {name}: {el.value} != {dt.itemsize}")
+                if cnt.value:
+                    buf = (C.c_char * (cnt.value * el.value)).from_address(ptr.value)
+                    self.parts[name] = np.frombuffer(bytes(buf), dtype=dt).copy()
+                else:
+                    self.parts[name] = np.zeros(0, dtype=dt)
+            s = np.zeros(10, np.int64)
+            L.mq_host_plan_sizes(h, s.ctypes.data_as(i64p))
+            self.sizes = dict(zip(["vec", "la", "rb", "prp", "t", "c", "tmp", "out", "ws1", "ws2"], s.tolist()))
+            a, b, c = C.c_double(), C.c_double(), C.c_double()
+            L.mq_host_plan_flops(h, C.byref(a), C.byref(b), C.byref(c))
+            self.ref_flops, self.exec_flops, self.min_bytes = a.value, b.value, c.value
+            self.out_layout = None
+            if kind != "sigma":
+                d = mq_boundary_desc()
+                L.mq_host_plan_out_desc(h, C.byref(d))
+                self.out_layout = BoundaryLayout.from_c(d)
+        finally:
+            L.mq_host_plan_destroy(h)
+
+
+# --------------------------------------------------------------------------- GPU engine
+class GpuBoundary:
+    __slots__ = ("layout", "data")
+
+    def __init__(self, layout: BoundaryLayout, data):
+        self.layout = layout
+        self.data = data  # torch tensor (float64, cuda)
+
+    def dicts(self):
+        return self.layout.to_dicts(self.data.cpu().numpy())
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class GpuEngine:
+    """Engine interface on libmqgpu: sigma / Jacobi-Davidson / boundary steps on one GPU."""
+
+    name = "gpu"
+
+    def __init__(self, device=0, prec=MQ_PREC_FP64):
+        import torch
+        if not torch.cuda.is_available():
+            raise MQError("no GPU visible: the hot path has no CPU fallback")
+        self.torch = torch
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        self.ctx = C.c_void_p()
+        if lib().mq_ctx_create(device, C.byref(self.ctx)) != 0:
+            raise MQError("mq_ctx_create failed")
+        self.prec = prec
+        self.flops = 0.0
+        self.stats = []
+
+    def close(self):
+        if self.ctx:
+            lib().mq_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def err(self):
+        return lib().mq_last_error(self.ctx).decode()
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise MQError(f"{what}: {self.err()} (code {rc})")
+
+    def sync(self):
+        self.check(lib().mq_ctx_synchronize(self.ctx), "synchronize")
+
+    def to_dev(self, x):
+        return self.torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64)).to(self.device)
+
+    def seed_boundary(self, charge):
+        return GpuBoundary(BoundaryLayout.seed(charge), self.to_dev(np.ones(1)))
+
+    def upload_boundary(self, layout, host_data):
+        return GpuBoundary(layout, self.to_dev(host_data))
+
+    def _plan(self, kind, site):
+        p = C.c_void_p()
+        fn = {"sigma": lib().mq_sigma_plan_create, "lstep": lib().mq_lstep_plan_create,
+              "rstep": lib().mq_rstep_plan_create}[kind]
+        self.check(fn(self.ctx, site.ptr(), C.byref(p)), f"{kind} plan")
+        return p
+
+    def _step(self, kind, symm, A, B: GpuBoundary, W):
+        site = SiteDesc(symm, A.phys, A.left, A.right, W, B.layout if kind == "lstep" else None,
+                        B.layout if kind == "rstep" else None)
+        p = self._plan(kind, site)
+        try:
+            d = mq_boundary_desc()
+            self.check(lib().mq_step_out_desc(p, C.byref(d)), "out desc")
+            lay = BoundaryLayout.from_c(d)
+            out = self.torch.empty(max(lay.total, 1), dtype=self.torch.float64, device=self.device)
+            a = self.to_dev(A.to_flat())
+            self.torch.cuda.synchronize()
+            self.check(lib().mq_boundary_step(p, _ptr(B.data), _ptr(a), _ptr(out)), kind)
+            self.sync()
+            fl = C.c_double()
+            lib().mq_plan_flops(p, C.byref(fl), None)
+            self.flops += fl.value
+        finally:
+            lib().mq_plan_destroy(p)
+        return GpuBoundary(lay, out)
+
+    def left_step(self, symm, A, L, W):
+        return self._step("lstep", symm, A, L, W)
+
+    def right_step(self, symm, A, R, W):
+        return self._step("rstep", symm, A, R, W)
+
+    def sigma_plan(self, symm, psi, L, R, W):
+        site = SiteDesc(symm, psi.phys, psi.left, psi.right, W, L.layout, R.layout)
+        return SigmaPlan(self, site, L, R)
+
+    def sigma(self, symm, psi, L, R, W, x=None):
+        sp = self.sigma_plan(symm, psi, L, R, W)
+        try:
+            xd = self.to_dev(psi.to_flat() if x is None else x)
+            y = sp.apply(xd)
+            self.sync()
+            return y.cpu().numpy(), sp.ref_flops
+        finally:
+            sp.close()
+
+    def jd(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=()):
+        sp = self.sigma_plan(symm, psi, L, R, W)
+        try:
+            theta, u, its = sp.jd(self.to_dev(psi.to_flat()), max_iter, tol, ortho)
+            self.flops += its * sp.ref_flops
+            return theta, u.cpu().numpy(), its
+        finally:
+            sp.close()
+
+
+class SigmaPlan:
+    def __init__(self, eng: GpuEngine, site, L: GpuBoundary, R: GpuBoundary):
+        self.eng = eng
+        self.site = site
+        self.L, self.R = L, R
+        self.p = eng._plan("sigma", site)
+        fl, by = C.c_double(), C.c_double()
+        lib().mq_plan_flops(self.p, C.byref(fl), C.byref(by))
+        self.ref_flops = fl.value
+        self.min_bytes = by.value
+        self.exec_flops = lib().mq_plan_exec_flops(self.p)
+        self.n = lib().mq_plan_vec_len(self.p)
+        self.workspace_bytes = lib().mq_plan_workspace_bytes(self.p)
+
+    def close(self):
+        if self.p:
+            lib().mq_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, x, y=None):
+        t = self.eng.torch
+        if y is None:
+            y = t.empty(self.n, dtype=t.float64, device=self.eng.device)
+        self.eng.check(lib().mq_sigma(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y),
+                                      self.eng.prec), "mq_sigma")
+        return y
+
+    def jd(self, x0, max_iter=10, tol=1e-8, ortho=()):
+        t = self.eng.torch
+        u = t.empty(self.n, dtype=t.float64, device=self.eng.device)
+        th = C.c_double()
+        its = C.c_int()
+        od = [self.eng.to_dev(o) for o in ortho]
+        arr = (C.c_void_p * max(1, len(od)))(*[o.data_ptr() for o in od])
+        t.cuda.synchronize()
+        self.eng.check(lib().mq_jacobi_davidson(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x0), _ptr(u),
+                                                C.byref(th), max_iter, tol, self.eng.prec, arr, len(od),
+                                                C.byref(its)), "mq_jacobi_davidson")
+        return th.value, u, its.value

@@ -1,0 +1,100 @@
+"""numpy executor of the device task lists (test infrastructure).
+
+Runs a HostPlan's copy / GEMM / W-application tasks on host arrays exactly
+as the HIP kernels define them, so the plan logic (offsets, transposes,
+coefficients, tile and split-K coverage) is checked against the oracle on
+machines without a GPU.
+"""
+import numpy as np
+from numpy.lib.stride_tricks import as_strided
+
+
+def _mat(buf, off, rows, cols, ld):
+    if rows == 0 or cols == 0:
+        return np.zeros((rows, cols))
+    need = off + (rows - 1) + (cols - 1) * ld + 1
+    assert off >= 0 and need <= buf.size, (off, rows, cols, ld, buf.size)
+    return as_strided(buf[off:], shape=(rows, cols), strides=(8, 8 * ld))
+
+
+def copy2d(tasks, src, dst):
+    for t in tasks:
+        r, c = int(t["rows"]), int(t["cols"])
+        if t["trans"]:
+            s = _mat(src, int(t["src"]), c, r, int(t["lds"])).T
+        else:
+            s = _mat(src, int(t["src"]), r, c, int(t["lds"]))
+        d = _mat(dst, int(t["dst"]), r, c, int(t["ldd"]))
+        d[...] = t["scale"] * s
+
+
+def wapply(strips, srcs, T, Cbuf):
+    for st in strips:
+        r, c = int(st["rows"]), int(st["cols"])
+        acc = np.zeros((r, c))
+        for k in range(int(st["s0"]), int(st["s1"])):
+            s = srcs[k]
+            acc += s["coef"] * _mat(T, int(s["off"]), r, c, int(s["ld"]))
+        _mat(Cbuf, int(st["dst"]), r, c, int(st["ldd"]))[...] = acc
+
+
+def gemm(groups, tiles, A, B, C, tm=64, tn=64):
+    # tile coverage: every (m, n, k-split) tile exactly once
+    seen = {}
+    for t in tiles:
+        key = (int(t["g"]), int(t["mt"]), int(t["nt"]), int(t["ks"]))
+        assert key not in seen
+        seen[key] = 1
+    for gi, g in enumerate(groups):
+        M, N, K = int(g["M"]), int(g["N"]), int(g["K"])
+        mt, nt = -(-M // tm), -(-N // tn)
+        for ks in range(int(g["split"])):
+            for n in range(nt):
+                for m in range(mt):
+                    assert (gi, m, n, ks) in seen
+        assert int(g["split"]) * int(g["kchunk"]) >= K
+        if g["transA"]:
+            a = _mat(A, int(g["a"]), K, M, int(g["lda"])).T
+        else:
+            a = _mat(A, int(g["a"]), M, K, int(g["lda"]))
+        if g["transB"]:
+            b = _mat(B, int(g["b"]), N, K, int(g["ldb"])).T
+        else:
+            b = _mat(B, int(g["b"]), K, N, int(g["ldb"]))
+        _mat(C, int(g["c"]), M, N, int(g["ldc"]))[...] = a @ b if K else 0.0
+
+
+def run_sigma(hp, Ldata, Rdata, x):
+    s = hp.sizes
+    P = hp.parts
+    la, rb, prp = np.zeros(max(s["la"], 1)), np.zeros(max(s["rb"], 1)), np.zeros(max(s["prp"], 1))
+    T, Cb = np.zeros(max(s["t"], 1)), np.zeros(max(s["c"], 1))
+    y = np.full(s["vec"], np.nan)
+    copy2d(P["arrL"], Ldata, la)
+    copy2d(P["arrR"], Rdata, rb)
+    copy2d(P["reshape"], x, prp)
+    gemm(P["g1_groups"], P["g1_tiles"], la, prp, T)
+    wapply(P["strips"], P["srcs"], T, Cb)
+    gemm(P["g2_groups"], P["g2_tiles"], Cb, rb, y)
+    return y
+
+
+def run_step(hp, kind, Bdata, A):
+    s = hp.sizes
+    P = hp.parts
+    la, rb, prp = np.zeros(max(s["la"], 1)), np.zeros(max(s["rb"], 1)), np.zeros(max(s["prp"], 1))
+    T, Cb, tmp = np.zeros(max(s["t"], 1)), np.zeros(max(s["c"], 1)), np.zeros(max(s["tmp"], 1))
+    out = np.full(max(s["out"], 1), np.nan)
+    copy2d(P["reshape"], A, prp)
+    if kind == "lstep":
+        copy2d(P["arrL"], Bdata, la)
+        gemm(P["g1_groups"], P["g1_tiles"], la, prp, T)
+        wapply(P["strips"], P["srcs"], T, Cb)
+        gemm(P["g2_groups"], P["g2_tiles"], Cb, A, tmp)
+    else:
+        copy2d(P["arrR"], Bdata, rb)
+        gemm(P["g1_groups"], P["g1_tiles"], A, rb, T)
+        wapply(P["strips"], P["srcs"], T, Cb)
+        gemm(P["g2_groups"], P["g2_tiles"], Cb, prp, tmp)
+    copy2d(P["scatter"], tmp, out)
+    return out[:s["out"]]
