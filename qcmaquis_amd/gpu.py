@@ -222,6 +222,7 @@ class GpuEngine:
         sp = self.sigma_plan(symm, psi, L, R, W)
         try:
             xd = self.to_dev(psi.to_flat() if x is None else x)
+            self.torch.cuda.synchronize()
             y = sp.apply(xd)
             self.sync()
             return y.cpu().numpy(), sp.ref_flops

@@ -1,0 +1,108 @@
+"""GPU hot path (libmqgpu on the MI355X) against the CPU oracle.
+
+Bars: sigma vectors and boundary steps <= 1e-12 relative (FP64); Davidson
+eigenvalues <= 1e-10; sweep energies <= 1e-10 Eh against the oracle sweep
+and within the golden tolerance of the reference's test energies."""
+import numpy as np
+import pytest
+
+from conftest import ham_from_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _sweep_pair(ham, oracle_engine, gpu_engine, m, nmicro, check):
+    from qcmaquis_amd.sweep import TwoSiteDMRG, micro_site
+    d = TwoSiteDMRG(ham, oracle_engine, m)
+    for i in range(nmicro):
+        check(d, micro_site(ham.L, i))
+        d.microiteration(0, i)
+    return d
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["lih", "benzene"])
+def test_sigma_parity_fixtures(fixtures, oracle_engine, gpu_engine, name):
+    ham = ham_from_fixture(fixtures[name])
+
+    def check(d, site):
+        psi, L, R, W = d.site_problem(site)
+        Lg = gpu_engine.upload_boundary(L.layout, L.data)
+        Rg = gpu_engine.upload_boundary(R.layout, R.data)
+        x = np.random.default_rng(site).standard_normal(psi.to_flat().size)
+        y0, f0 = oracle_engine.sigma(ham.symm, psi, L, R, W, x=x)
+        y1, f1 = gpu_engine.sigma(ham.symm, psi, Lg, Rg, W, x=x)
+        assert _rel(y1, y0) <= 1e-12, (site, _rel(y1, y0))
+        assert f0 == f1
+
+    _sweep_pair(ham, oracle_engine, gpu_engine, 60, 2 * (ham.L - 2), check)
+
+
+def test_sigma_and_steps_parity_config0(oracle_engine, gpu_engine):
+    from qcmaquis_amd.integrals import synthetic
+    from qcmaquis_amd.sweep import Hamiltonian
+    ham = Hamiltonian(8, 8, 0, synthetic(8, seed=1008))
+
+    def check(d, site):
+        psi, L, R, W = d.site_problem(site)
+        Lg = gpu_engine.upload_boundary(L.layout, L.data)
+        Rg = gpu_engine.upload_boundary(R.layout, R.data)
+        x = psi.to_flat()
+        y0, _ = oracle_engine.sigma(ham.symm, psi, L, R, W, x=x)
+        y1, _ = gpu_engine.sigma(ham.symm, psi, Lg, Rg, W, x=x)
+        assert _rel(y1, y0) <= 1e-12
+        A = d.mps[site]
+        l0 = oracle_engine.left_step(ham.symm, A, L, ham.mpo[site]).dicts()
+        l1 = gpu_engine.left_step(ham.symm, A, Lg, ham.mpo[site]).dicts()
+        B = d.mps[site + 1]
+        r0 = oracle_engine.right_step(ham.symm, B, R, ham.mpo[site + 1]).dicts()
+        r1 = gpu_engine.right_step(ham.symm, B, Rg, ham.mpo[site + 1]).dicts()
+        for got, exp in ((l1, l0), (r1, r0)):
+            assert len(got) == len(exp)
+            for b in range(len(exp)):
+                assert set(got[b]) == set(exp[b])
+                for k in exp[b]:
+                    assert np.abs(got[b][k] - exp[b][k]).max() <= 1e-12 * max(1.0, np.abs(exp[b][k]).max())
+
+    _sweep_pair(ham, oracle_engine, gpu_engine, 64, 8, check)
+
+
+def test_jacobi_davidson_parity(fixtures, oracle_engine, gpu_engine):
+    ham = ham_from_fixture(fixtures["benzene"])
+
+    def check(d, site):
+        psi, L, R, W = d.site_problem(site)
+        Lg = gpu_engine.upload_boundary(L.layout, L.data)
+        Rg = gpu_engine.upload_boundary(R.layout, R.data)
+        t0, u0, i0 = oracle_engine.jd(ham.symm, psi, L, R, W, 10, 1e-8)
+        t1, u1, i1 = gpu_engine.jd(ham.symm, psi, Lg, Rg, W, 10, 1e-8)
+        assert abs(t1 - t0) <= 1e-10 * abs(t0)
+        assert i0 == i1
+        assert min(np.abs(u1 - u0).max(), np.abs(u1 + u0).max()) <= 1e-8
+
+    _sweep_pair(ham, oracle_engine, gpu_engine, 60, 4, check)
+
+
+@pytest.mark.parametrize("name", ["h2", "lih", "benzene"])
+def test_golden_energies_gpu(fixtures, gpu_engine, name):
+    from qcmaquis_amd.sweep import TwoSiteDMRG
+    fx = fixtures[name]
+    E = TwoSiteDMRG(ham_from_fixture(fx), gpu_engine, 100).run(2)
+    assert abs(E - fx["energy"]) <= fx["rtol"] * abs(fx["energy"]), (E, fx["energy"])
+
+
+def test_config0_sweep_energies_match_oracle(oracle_engine, gpu_engine):
+    """L=8, N=8, S=0, m=64 two-site sweeps: GPU engine vs oracle, per microiteration."""
+    from qcmaquis_amd.integrals import synthetic
+    from qcmaquis_amd.sweep import Hamiltonian, TwoSiteDMRG
+    ints = synthetic(8, seed=1008)
+    e = []
+    for eng in (oracle_engine, gpu_engine):
+        ham = Hamiltonian(8, 8, 0, ints)
+        d = TwoSiteDMRG(ham, eng, 64)
+        d.run(2)
+        e.append(np.array(d.energies))
+    assert np.abs(e[0] - e[1]).max() <= 1e-10, np.abs(e[0] - e[1]).max()

@@ -1,0 +1,83 @@
+"""Host-side checks of libmqgpu without a GPU: the C ABI exports, and the
+device task lists executed by a numpy emulator against the oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, ham_from_fixture
+
+
+def test_library_exports_every_declared_symbol():
+    from qcmaquis_amd import gpu
+    hdr = open(os.path.join(ROOT, "include", "mqgpu.h")).read()
+    names = sorted(set(re.findall(r"\b(mq_[a-z0-9_]+)\s*\(", hdr)))
+    assert len(names) >= 20
+    lib = C.CDLL(gpu.LIBPATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def _check_site(ham, d, site, oracle_engine):
+    import emulate
+    from qcmaquis_amd.desc import SiteDesc
+    from qcmaquis_amd.gpu import HostPlan
+    psi, Lb, Rb, W = d.site_problem(site)
+    x = psi.to_flat()
+    y, fl = oracle_engine.sigma(ham.symm, psi, Lb, Rb, W, x=x)
+    hp = HostPlan(SiteDesc(ham.symm, psi.phys, psi.left, psi.right, W, Lb.layout, Rb.layout), "sigma")
+    ye = emulate.run_sigma(hp, Lb.data, Rb.data, x)
+    assert np.abs(y - ye).max() <= 1e-13 * max(1.0, np.abs(y).max())
+    assert hp.ref_flops == fl  # algorithmic FLOPs of the reference schedule
+    for kind, A, B, Wm in (("lstep", d.mps[site], Lb, ham.mpo[site]),
+                           ("rstep", d.mps[site + 1], Rb, ham.mpo[site + 1])):
+        if kind == "lstep":
+            ref = oracle_engine.left_step(ham.symm, A, B, Wm)
+            sd = SiteDesc(ham.symm, A.phys, A.left, A.right, Wm, B.layout, None)
+        else:
+            ref = oracle_engine.right_step(ham.symm, A, B, Wm)
+            sd = SiteDesc(ham.symm, A.phys, A.left, A.right, Wm, None, B.layout)
+        hps = HostPlan(sd, kind)
+        out = emulate.run_step(hps, kind, B.data, A.to_flat())
+        got = hps.out_layout.to_dicts(out)
+        exp = ref.dicts()
+        assert len(got) == len(exp)
+        for b in range(len(exp)):
+            assert set(got[b]) == set(exp[b]), (kind, b)
+            for k in exp[b]:
+                assert np.abs(got[b][k] - exp[b][k]).max() <= 1e-13 * max(1.0, np.abs(exp[b][k]).max())
+
+
+@pytest.mark.parametrize("name", ["lih", "benzene"])
+def test_plan_emulation_matches_oracle(fixtures, oracle_engine, name):
+    from qcmaquis_amd.sweep import TwoSiteDMRG, micro_site, n_microiterations
+    ham = ham_from_fixture(fixtures[name])
+    d = TwoSiteDMRG(ham, oracle_engine, 60)
+    for i in range(n_microiterations(ham.L)):
+        _check_site(ham, d, micro_site(ham.L, i), oracle_engine)
+        d.microiteration(0, i)
+
+
+def test_plan_emulation_synthetic(oracle_engine):
+    from qcmaquis_amd.integrals import synthetic
+    from qcmaquis_amd.sweep import Hamiltonian, TwoSiteDMRG, micro_site
+    ham = Hamiltonian(8, 8, 0, synthetic(8, seed=1008))
+    d = TwoSiteDMRG(ham, oracle_engine, 48)
+    for i in range(5):
+        if i in (2, 3, 4):
+            _check_site(ham, d, micro_site(ham.L, i), oracle_engine)
+        d.microiteration(0, i)
+
+
+def test_plan_rejects_mismatched_boundary(fixtures, oracle_engine):
+    from qcmaquis_amd.desc import BoundaryLayout, SiteDesc
+    from qcmaquis_amd.gpu import HostPlan, MQError
+    from qcmaquis_amd.sweep import TwoSiteDMRG
+    ham = ham_from_fixture(fixtures["lih"])
+    d = TwoSiteDMRG(ham, oracle_engine, 20)
+    psi, Lb, Rb, W = d.site_problem(0)
+    bad = BoundaryLayout(W.row_dim + 1, [0] * (W.row_dim + 2), [], [], [], [], [], 0)
+    with pytest.raises(MQError):
+        HostPlan(SiteDesc(ham.symm, psi.phys, psi.left, psi.right, W, bad, Rb.layout), "sigma")
