@@ -163,6 +163,13 @@ int mq_sigma(mq_plan *plan, const double *L, const double *R,
              const double *x, double *y, int prec);
 /* Drop the rearranged boundaries (call after changing L or R in place). */
 int mq_plan_rebind(mq_plan *plan);
+/* Instrumented mq_sigma (no reference counterpart): HIP events around each
+ * stage on the plan's stream.  ms[0] boundary arrangement, ms[1] reshape,
+ * ms[2] T GEMM, ms[3] W application, ms[4] output GEMM.  work[0..2] are the
+ * stage FLOPs (T GEMM, W application axpys, output GEMM), work[3..5] the
+ * minimum HBM bytes of the same three stages. */
+int mq_sigma_timed(mq_plan *plan, const double *L, const double *R,
+                   const double *x, double *y, int prec, float ms[5], double work[6]);
 
 /* -- boundary steps ----------------------------------------------------- */
 /* Plan for overlap_mpo_left_step (desc.L = input) or right step (desc.R). */

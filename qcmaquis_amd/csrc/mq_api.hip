@@ -233,6 +233,63 @@ int mq_sigma(mq_plan *plan, const double *L, const double *R, const double *x, d
     }
 }
 
+// Stage-timed sigma: HIP events on the plan's stream around each stage.
+// ms[0] arrange L+R (0 if already bound), ms[1] reshape, ms[2] T GEMM,
+// ms[3] W application, ms[4] output GEMM.  work[] returns the algorithmic
+// FLOPs of stages 2..4 and the bytes each stage must move at minimum.
+int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double *x, double *y, int prec,
+                   float ms[5], double work[6]) {
+    if (!plan || !x || !y || !ms || prec != MQ_PREC_FP64 || plan->hp.kind != PLAN_SIGMA) return MQ_ERR_ARG;
+    try {
+        hipStream_t s = plan->ctx->stream;
+        HostPlan &h = plan->hp;
+        hipEvent_t ev[6];
+        for (auto &e : ev) check(hipEventCreate(&e), "event");
+        check(hipEventRecord(ev[0], s), "record");
+        if (L != plan->bound_L || R != plan->bound_R) {
+            check(launch_copy2d(plan->arrL.p, (int)h.arrL.size(), L, plan->la.p, s), "arrange L");
+            check(launch_copy2d(plan->arrR.p, (int)h.arrR.size(), R, plan->rb.p, s), "arrange R");
+            plan->bound_L = L;
+            plan->bound_R = R;
+        }
+        check(hipEventRecord(ev[1], s), "record");
+        check(launch_copy2d(plan->reshape.p, (int)h.reshape.size(), x, plan->prp.p, s), "reshape");
+        check(hipEventRecord(ev[2], s), "record");
+        check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p, plan->g1.nsplit,
+                          plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, s),
+              "gemm T");
+        check(hipEventRecord(ev[3], s), "record");
+        check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p, s),
+              "W apply");
+        check(hipEventRecord(ev[4], s), "record");
+        check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p, plan->g2.nsplit,
+                          plan->c.p, plan->rb.p, y, plan->ws.p, s),
+              "gemm R");
+        check(hipEventRecord(ev[5], s), "record");
+        check(hipEventSynchronize(ev[5]), "sync");
+        for (int i = 0; i < 5; ++i) check(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]), "elapsed");
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (work) {
+            double srcel = 0, dstel = 0;
+            for (auto &st : h.wapply.strips) {
+                srcel += (double)st.rows * st.cols * (st.s1 - st.s0);
+                dstel += (double)st.rows * st.cols;
+            }
+            work[0] = h.g1.flops();
+            work[1] = 2.0 * srcel;
+            work[2] = h.g2.flops();
+            // minimum bytes: T GEMM reads LA + PRP, writes T; W application reads T once, writes C;
+            // output GEMM reads C + RB, writes y
+            work[3] = 8.0 * (h.n_la + h.n_prp + h.n_t);
+            work[4] = 8.0 * (h.n_t + dstel);
+            work[5] = 8.0 * (h.n_c + h.n_rb + h.vec_len);
+        }
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
 // Forget the arranged boundaries (call after modifying L or R in place).
 int mq_plan_rebind(mq_plan *plan) {
     if (!plan) return MQ_ERR_ARG;

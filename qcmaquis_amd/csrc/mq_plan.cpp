@@ -12,12 +12,43 @@
 // The boundary steps (move_boundary.hpp:130-229) reuse the same stages.
 #include "mq_plan.hpp"
 
-#include <set>
-#include <tuple>
+#include <array>
+#include <mutex>
+#include <unordered_map>
 
 namespace mq {
 
 namespace {
+
+inline uint64_t qkey(Q c) { return ((uint64_t)(uint32_t)c.a << 32) | (uint32_t)c.b; }
+
+struct QMap {
+    std::unordered_map<uint64_t, int> m;
+    void build(const Idx &ix) {
+        m.clear();
+        m.reserve(ix.n() * 2 + 1);
+        for (int i = 0; i < ix.n(); ++i) m[qkey(ix.q[i])] = i;
+    }
+    int get(Q c) const {
+        auto it = m.find(qkey(c));
+        return it == m.end() ? -1 : it->second;
+    }
+};
+
+// Per-thread cache of SU(2) couplings for init = 1 (set_coupling is linear in init).
+const std::array<double, 4> &coupling(int j, int ts, int jp, int a, int k, int ap, int i, int tsp, int ip) {
+    thread_local std::unordered_map<uint64_t, std::array<double, 4>> cache;
+    uint64_t key = 0;
+    for (int v : {j, ts, jp, a, k, ap, i, tsp, ip}) {
+        if (v < 0 || v > 127) throw std::runtime_error("coupling argument out of range");
+        key = (key << 7) | (uint64_t)v;
+    }
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::array<double, 4> c;
+    set_coupling(j, ts, jp, a, k, ap, i, tsp, ip, 1.0, c.data());
+    return cache.emplace(key, c).first->second;
+}
 
 struct Site {
     int symm;
@@ -25,16 +56,58 @@ struct Site {
     Idx phys, left, right;
     Mpo W;
     VLayout X;
-    PB pbl;  // PB(phys, left)
-    PB pbr;  // PB(phys, right, right-fused)
+    QMap pi, li, ri;
+    int np, nl, nr;
+    std::vector<int> pbl;     // [p*nl + l] offset of (p, l) in fused charge p+l, -1 if none
+    std::vector<int> pbl_sz;  // [fused index into X] rows
+    std::vector<int> pbr;     // [p*nr + r] offset of (p, r) in right-fused charge r-p
+    std::vector<int> pbr_sz_l;  // [l] right-paired columns of ket-left charge l (0 if none)
+    std::vector<int> x_of_r;  // [r] index into X.blk or -1
+    std::vector<int> lp_r;    // [l*np + p] right id of left charge l + phys p, -1 if none
+    std::vector<int> rmp_l;   // [r*np + p] left id of right charge r - phys p, -1 if none
+    // operators with physical charges resolved to phys ids
+    struct OpB {
+        int pin, pout;
+        const OpBlk *b;
+    };
+    std::vector<std::vector<OpB>> opb;
     Site(const mq_site_desc &d)
         : symm(d.symm), su2(d.symm == MQ_SYMM_SU2U1), phys(to_idx(d.phys)), left(to_idx(d.left)),
           right(to_idx(d.right)), W(to_mpo(d.mpo)) {
         if (d.symm != MQ_SYMM_SU2U1 && d.symm != MQ_SYMM_TWOU1) throw std::runtime_error("unknown symmetry");
         X = vec_layout(phys, left, right);
-        pbl = PB(phys, left, false);
-        pbr = PB(phys, right, true);
+        pi.build(phys);
+        li.build(left);
+        ri.build(right);
+        np = phys.n();
+        nl = left.n();
+        nr = right.n();
+        PB pl(phys, left, false), pr(phys, right, true);
+        pbl.assign((size_t)np * nl, -1);
+        pbr.assign((size_t)np * nr, -1);
+        for (int p = 0; p < np; ++p) {
+            for (int l = 0; l < nl; ++l) pbl[(size_t)p * nl + l] = pl.at(phys.q[p], left.q[l]);
+            for (int r = 0; r < nr; ++r) pbr[(size_t)p * nr + r] = pr.at(phys.q[p], right.q[r]);
+        }
+        pbr_sz_l.assign(nl, 0);
+        for (int l = 0; l < nl; ++l) pbr_sz_l[l] = pr.sz(left.q[l]);
+        x_of_r.assign(nr, -1);
+        for (size_t i = 0; i < X.blk.size(); ++i) x_of_r[ri.get(X.blk[i].c)] = (int)i;
+        lp_r.assign((size_t)nl * np, -1);
+        for (int l = 0; l < nl; ++l)
+            for (int p = 0; p < np; ++p) lp_r[(size_t)l * np + p] = ri.get(left.q[l] + phys.q[p]);
+        rmp_l.assign((size_t)nr * np, -1);
+        for (int r = 0; r < nr; ++r)
+            for (int p = 0; p < np; ++p) rmp_l[(size_t)r * np + p] = li.get(right.q[r] - phys.q[p]);
+        opb.resize(W.ops.size());
+        for (size_t o = 0; o < W.ops.size(); ++o)
+            for (auto &b : W.ops[o].blocks) {
+                int pin = pi.get(b.in), pout = pi.get(b.out);
+                if (pin >= 0 && pout >= 0) opb[o].push_back({pin, pout, &b});
+            }
     }
+    int lsize(int l) const { return left.s[l]; }
+    int rsize(int r) const { return right.s[r]; }
 };
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -42,9 +115,9 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     g.split = 1;
     g.kchunk = std::max(g.K, 1);
-    int mt = ceil_div(std::max(g.M, 1), tm), nt = ceil_div(std::max(g.N, 1), tn);
     if (g.M == 0 || g.N == 0) return;
-    // split K when the output is small and K long: aim for chunks of ~kchunk_target
+    int mt = ceil_div(g.M, tm), nt = ceil_div(g.N, tn);
+    // split K when the output is small and K long: chunks of about kchunk_target
     if (g.K > 2 * kchunk_target && mt * nt < 64) {
         int s = std::min(ceil_div(g.K, kchunk_target), 64);
         int kc = ceil_div(ceil_div(g.K, s), 16) * 16;
@@ -63,201 +136,196 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
             for (int m = 0; m < mt; ++m) gl.tiles.push_back({gi, m, n, ks});
 }
 
-// ---------------- right-paired psi (shared by all plans) ----------------
-struct PRPLayout {
-    std::map<Q, int64_t> off;  // ket-left charge -> offset
-    std::map<Q, int> ncol;
+Copy2D mk_copy(int64_t dst, int64_t src, int rows, int cols, int ldd, int lds, bool trans, double scale) {
+    Copy2D c;
+    c.dst = dst;
+    c.src = src;
+    c.rows = rows;
+    c.cols = cols;
+    c.ldd = ldd;
+    c.lds = lds;
+    c.trans = trans ? 1 : 0;
+    c.scale = scale;
+    return c;
+}
+
+// ---------------- right-paired psi --------------------------------------
+struct PRP {
+    std::vector<int64_t> off;  // [l] (-1 if the ket has no block with left charge l)
     int64_t total = 0;
 };
 
-PRPLayout prp_layout(const Site &s, std::vector<Copy2D> &tasks) {
-    PRPLayout p;
-    for (int i = 0; i < s.left.n(); ++i) {
-        Q lk = s.left.q[i];
-        int n = s.pbr.sz(lk);
+PRP prp_layout(const Site &s, std::vector<Copy2D> &tasks) {
+    PRP p;
+    p.off.assign(s.nl, -1);
+    for (int l = 0; l < s.nl; ++l) {
+        int n = s.pbr_sz_l[l];
         if (n == 0) continue;
-        p.off[lk] = p.total;
-        p.ncol[lk] = n;
-        p.total += (int64_t)s.left.s[i] * n;
+        p.off[l] = p.total;
+        p.total += (int64_t)s.lsize(l) * n;
     }
     for (auto &xb : s.X.blk) {
-        Q r = xb.c;
-        for (int si = 0; si < s.phys.n(); ++si) {
-            Q sc = s.phys.q[si];
-            Q lk = r - sc;
-            if (!s.left.has(lk)) continue;
-            int ld = s.left.size_of(lk);
-            for (int ss = 0; ss < s.phys.s[si]; ++ss) {
-                Copy2D c;
-                c.dst = p.off.at(lk) + (int64_t)(s.pbr.at(sc, r) + ss * xb.cols) * ld;
-                c.src = xb.off + s.pbl.at(sc, lk) + ss * ld;
-                c.rows = ld;
-                c.cols = xb.cols;
-                c.ldd = ld;
-                c.lds = xb.rows;
-                c.trans = 0;
-                c.scale = 1.0;
-                tasks.push_back(c);
-            }
+        int r = s.ri.get(xb.c);
+        for (int pp = 0; pp < s.np; ++pp) {
+            int l = s.li.get(xb.c - s.phys.q[pp]);
+            if (l < 0) continue;
+            int ld = s.lsize(l);
+            for (int ss = 0; ss < s.phys.s[pp]; ++ss)
+                tasks.push_back(mk_copy(p.off[l] + (int64_t)(s.pbr[(size_t)pp * s.nr + r] + ss * xb.cols) * ld,
+                                        xb.off + s.pbl[(size_t)pp * s.nl + l] + ss * ld, ld, xb.cols, ld, xb.rows,
+                                        false, 1.0));
         }
     }
     return p;
 }
 
-// ---------------- T = L^T psi: stacked left-boundary operand -------------
-struct TLayout {
-    std::map<Q, int> M;                                    // rows of LA_lk / T_lk
-    std::map<Q, int64_t> la_off, t_off;                    // per lk
-    std::map<std::tuple<int, Q, Q>, int> rowoff;           // (b1, lb, lk) -> row offset in LA_lk
-    std::vector<std::tuple<int, Q, Q>> tblocks;            // (b1, lb, lk) in creation order
+// ---------------- T = L^T psi --------------------------------------------
+struct TB {  // one T block (b1, lb, lk)
+    int lb, lk;
+    int64_t off;  // offset of its first row inside the T arena (column 0)
+    int M;        // leading dimension (rows of T_lk)
+};
+struct TLay {
+    std::vector<int> M;            // [lk] rows of LA_lk / T_lk
+    std::vector<int64_t> la_off, t_off;
+    std::vector<std::vector<TB>> tb;  // [b1]
     int64_t la_total = 0, t_total = 0;
+    double gemm_flops = 0;
 };
 
-// A-operand blocks of BoundaryMPSProduct for b1: (lb, lk, src block, transposed, scale)
-struct ABlk {
-    Q lb, lk;
-    const BBlk *blk;
-    bool trans;
-    double scale;
-};
-
-std::vector<ABlk> a_blocks(const Site &s, const BLayout &L, int b1, const std::map<Q, int64_t> &ref) {
-    std::vector<ABlk> out;
-    if (s.W.lskip(b1)) {
-        int cj = s.W.lherm[b1];
-        for (auto &x : L.b[cj]) {
-            if (!ref.count(x.lc) || !ref.count(x.rc)) continue;
-            double sc = s.su2 ? conj_correction(x.lc, x.rc, s.W.lspin[b1]) * s.W.lphase[b1] : 1.0;
-            out.push_back({x.lc, x.rc, &x, false, sc});
-        }
-    } else {
-        for (auto &x : L.b[b1]) {
-            if (!ref.count(x.rc) || !ref.count(x.lc)) continue;
-            out.push_back({x.rc, x.lc, &x, true, 1.0});
-        }
-    }
-    return out;
-}
-
-TLayout t_layout(const Site &s, const BLayout &L, const PRPLayout &prp, std::vector<Copy2D> &arr, GemmList &g1,
-                 int tm, int tn, int kc, bool used_rows_only) {
-    TLayout t;
-    std::vector<std::vector<ABlk>> ab(s.W.rows);
+// BoundaryMPSProduct A-operand blocks (boundary_times_mps.hpp:188-209).  used_only:
+// rows of the MPO with no entry are skipped (the reference computes them anyway;
+// the FLOP count of the reference schedule uses used_only = false).
+TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, std::vector<Copy2D> *arr, GemmList *g1, int tm,
+              int tn, int kc, bool used_only) {
+    TLay t;
+    t.M.assign(s.nl, 0);
+    t.tb.resize(s.W.rows);
+    struct A {
+        int b1, lb, lk;
+        const BBlk *blk;
+        bool trans;
+        double scale;
+        int row;
+    };
+    std::vector<A> as;
     for (int b1 = 0; b1 < s.W.rows; ++b1) {
-        if (used_rows_only && s.W.row_nnz[b1] == 0) continue;
-        ab[b1] = a_blocks(s, L, b1, prp.off);
-        for (auto &a : ab[b1]) {
-            int &M = t.M[a.lk];
-            t.rowoff[{b1, a.lb, a.lk}] = M;
-            t.tblocks.push_back({b1, a.lb, a.lk});
-            M += s.left.size_of(a.lb);
+        if (used_only && s.W.row_nnz[b1] == 0) continue;
+        bool skip = s.W.lskip(b1);
+        const auto &bl = L.b[skip ? s.W.lherm[b1] : b1];
+        for (auto &x : bl) {
+            int p = s.li.get(x.lc), q = s.li.get(x.rc);
+            if (p < 0 || q < 0 || prp.off[p] < 0 || prp.off[q] < 0) continue;
+            int lb = skip ? p : q, lk = skip ? q : p;
+            double sc = (skip && s.su2) ? conj_correction(x.lc, x.rc, s.W.lspin[b1]) * s.W.lphase[b1] : 1.0;
+            if (x.rows * x.cols != s.lsize(lb) * s.lsize(lk)) throw std::runtime_error("boundary block size mismatch");
+            as.push_back({b1, lb, lk, &x, !skip, sc, t.M[lk]});
+            t.M[lk] += s.lsize(lb);
         }
     }
-    for (auto &kv : t.M) {
-        Q lk = kv.first;
+    t.la_off.assign(s.nl, -1);
+    t.t_off.assign(s.nl, -1);
+    for (int lk = 0; lk < s.nl; ++lk) {
+        if (t.M[lk] == 0) continue;
         t.la_off[lk] = t.la_total;
-        t.la_total += (int64_t)kv.second * s.left.size_of(lk);
+        t.la_total += (int64_t)t.M[lk] * s.lsize(lk);
         t.t_off[lk] = t.t_total;
-        t.t_total += (int64_t)kv.second * prp.ncol.at(lk);
+        t.t_total += (int64_t)t.M[lk] * s.pbr_sz_l[lk];
+        t.gemm_flops += 2.0 * t.M[lk] * s.pbr_sz_l[lk] * s.lsize(lk);
     }
-    for (int b1 = 0; b1 < s.W.rows; ++b1)
-        for (auto &a : ab[b1]) {
-            int M = t.M[a.lk];
-            int lbs = s.left.size_of(a.lb), lks = s.left.size_of(a.lk);
-            Copy2D c;
-            c.dst = t.la_off[a.lk] + t.rowoff[{b1, a.lb, a.lk}];
-            c.src = a.blk->off;
-            c.rows = lbs;
-            c.cols = lks;
-            c.ldd = M;
-            c.lds = a.trans ? lks : lbs;
-            c.trans = a.trans ? 1 : 0;
-            c.scale = a.scale;
-            if (a.blk->rows * a.blk->cols != lbs * lks) throw std::runtime_error("boundary block size mismatch");
-            arr.push_back(c);
+    for (auto &a : as) {
+        t.tb[a.b1].push_back({a.lb, a.lk, t.t_off[a.lk] + a.row, t.M[a.lk]});
+        if (arr) {
+            int lbs = s.lsize(a.lb), lks = s.lsize(a.lk);
+            arr->push_back(mk_copy(t.la_off[a.lk] + a.row, a.blk->off, lbs, lks, t.M[a.lk], a.trans ? lks : lbs,
+                                   a.trans, a.scale));
         }
-    for (auto &kv : t.M) {
-        Q lk = kv.first;
-        GemmGroup g{};
-        g.a = t.la_off[lk];
-        g.b = prp.off.at(lk);
-        g.c = t.t_off[lk];
-        g.M = kv.second;
-        g.N = prp.ncol.at(lk);
-        g.K = s.left.size_of(lk);
-        g.lda = kv.second;
-        g.ldb = g.K;
-        g.ldc = kv.second;
-        add_gemm(g1, g, tm, tn, kc);
     }
+    if (g1)
+        for (int lk = 0; lk < s.nl; ++lk) {
+            if (t.M[lk] == 0) continue;
+            GemmGroup g{};
+            g.a = t.la_off[lk];
+            g.b = prp.off[lk];
+            g.c = t.t_off[lk];
+            g.M = t.M[lk];
+            g.N = s.pbr_sz_l[lk];
+            g.K = s.lsize(lk);
+            g.lda = g.M;
+            g.ldb = g.K;
+            g.ldc = g.M;
+            add_gemm(*g1, g, tm, tn, kc);
+        }
     return t;
 }
 
-// contribution of the W application: (al, b2, ar, row0) <- src
-struct Contrib {
-    Q al, ar;
-    int b2;
-    int row0;  // row offset of the strip inside the al block
-    int rows;  // strip rows (lb size)
-    Src src;
+// raw W-application contribution inside one MPO column
+struct Raw {
+    int al, ar;  // right-index ids of the output block (al = bra side, ar = ket side)
+    int row0;    // row offset of the strip inside the al block
+    int64_t off;
+    int ld;
+    double coef;
 };
 
-// lbtm W application (apply_op.hpp:23-101 / abelian apply_op.hpp:24-139)
-void lbtm_contribs(const Site &s, const TLayout &t, const PRPLayout &prp, const std::vector<int> &b2_list,
-                   std::vector<Contrib> &out, double *axpy_flops) {
-    // T blocks by b1
-    std::map<int, std::vector<std::pair<Q, Q>>> tb;
-    for (auto &x : t.tblocks) tb[std::get<0>(x)].push_back({std::get<1>(x), std::get<2>(x)});
-    for (int b2 : b2_list) {
-        int ap = s.W.rspin[b2];
-        for (int b1 : s.W.col_b1[b2]) {
-            auto it = tb.find(b1);
-            if (it == tb.end()) continue;
-            int a = s.W.lspin[b1];
-            for (auto &term : s.W.at.at({b1, b2})) {
-                const Op &op = s.W.ops[term.op];
-                int k = op.spin;
-                for (auto &lblk : it->second) {
-                    Q lb = lblk.first, lk = lblk.second;
-                    int lsz = s.left.size_of(lb);
-                    int M = t.M.at(lk);
-                    int64_t toff = t.t_off.at(lk) + t.rowoff.at({b1, lb, lk});
-                    for (auto &wb : op.blocks) {
-                        Q ar = lk + wb.in;
-                        if (!s.right.has(ar)) continue;
-                        Q al = lb + wb.out;
-                        if (s.su2 && !tri(spin2(ar), ap, spin2(al))) continue;
-                        if (!s.X.pos.count(al)) continue;  // out_left_i (phys*left common with right)
-                        int rsz = s.right.size_of(ar);
-                        double cpl[4] = {term.scale, term.scale, term.scale, term.scale};
-                        if (s.su2) {
-                            int i = spin2(lb), ip = spin2(al), j = spin2(lk), jp = spin2(ar);
-                            set_coupling(j, std::abs(j - jp), jp, a, k, ap, i, std::abs(i - ip), ip, term.scale, cpl);
-                        }
-                        int iro = s.pbr.at(wb.in, ar);
-                        int olo = s.pbl.at(wb.out, lb);
-                        for (auto &e : wb.e) {
-                            double coef = e.coeff * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : term.scale);
-                            Contrib c;
-                            c.al = al;
-                            c.ar = ar;
-                            c.b2 = b2;
-                            c.row0 = olo + e.col * lsz;
-                            c.rows = lsz;
-                            c.src.off = toff + (int64_t)(iro + e.row * rsz) * M;
-                            c.src.ld = M;
-                            c.src.coef = coef;
-                            out.push_back(c);
-                            if (axpy_flops) *axpy_flops += 2.0 * lsz * rsz;
-                        }
+// lbtm W application for column b2 (apply_op.hpp:23-101; abelian apply_op.hpp:24-139).
+// Adds the reference's axpy FLOPs to *axpy.
+void lbtm_column(const Site &s, const TLay &t, int b2, std::vector<Raw> &raw, double *axpy) {
+    raw.clear();
+    int ap = s.W.rspin[b2];
+    for (int b1 : s.W.col_b1[b2]) {
+        const auto &tbs = t.tb[b1];
+        if (tbs.empty()) continue;
+        int a = s.W.lspin[b1];
+        for (auto &term : s.W.at.at({b1, b2})) {
+            const Op &op = s.W.ops[term.op];
+            const auto &obs = s.opb[term.op];
+            for (auto &tb : tbs) {
+                int i = spin2(s.left.q[tb.lb]), j = spin2(s.left.q[tb.lk]);
+                int lsz = s.lsize(tb.lb);
+                for (auto &ob : obs) {
+                    const OpBlk &wb = *ob.b;
+                    int ar = s.lp_r[(size_t)tb.lk * s.np + ob.pin];
+                    if (ar < 0) continue;
+                    int al = s.lp_r[(size_t)tb.lb * s.np + ob.pout];
+                    if (al < 0 || s.x_of_r[al] < 0) continue;  // out_left_i
+                    int jp = spin2(s.right.q[ar]), ip = spin2(s.right.q[al]);
+                    if (s.su2 && !tri(jp, ap, ip)) continue;
+                    int rsz = s.rsize(ar);
+                    static const std::array<double, 4> one{1.0, 1.0, 1.0, 1.0};
+                    const std::array<double, 4> &cpl =
+                        s.su2 ? coupling(j, std::abs(j - jp), jp, a, op.spin, ap, i, std::abs(i - ip), ip) : one;
+                    int iro = s.pbr[(size_t)ob.pin * s.nr + ar];
+                    int olo = s.pbl[(size_t)ob.pout * s.nl + tb.lb];
+                    for (auto &e : wb.e) {
+                        double coef = e.coeff * term.scale * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : 1.0);
+                        raw.push_back({al, ar, olo + e.col * lsz, tb.off + (int64_t)(iro + e.row * rsz) * tb.M, tb.M,
+                                       coef});
+                        if (axpy) *axpy += 2.0 * lsz * rsz;
                     }
                 }
             }
         }
     }
+    std::sort(raw.begin(), raw.end(), [](const Raw &x, const Raw &y) {
+        if (x.al != y.al) return x.al < y.al;
+        if (x.ar != y.ar) return x.ar < y.ar;
+        if (x.row0 != y.row0) return x.row0 < y.row0;
+        return x.off < y.off;
+    });
+    // merge identical T slices of one strip (several operators of one (b1,b2) pair)
+    size_t w = 0;
+    for (size_t r = 0; r < raw.size(); ++r) {
+        if (w > 0 && raw[w - 1].al == raw[r].al && raw[w - 1].ar == raw[r].ar && raw[w - 1].row0 == raw[r].row0 &&
+            raw[w - 1].off == raw[r].off)
+            raw[w - 1].coef += raw[r].coef;
+        else
+            raw[w++] = raw[r];
+    }
+    raw.resize(w);
 }
 
-// the R operand of gemm_trim for (b2, ar, al): non-skip R[b2](ar,al); skip phase*R[conj](al,ar)^T
+// R operand of gemm_trim for (b2, ar, al): R[b2](ar,al) or phase*R[conj](al,ar)^T
 bool r_operand(const Site &s, const BLayout &R, int b2, Q ar, Q al, const BBlk **blk, bool *trans, double *scale) {
     if (s.W.rskip(b2)) {
         int cj = s.W.rherm[b2];
@@ -276,50 +344,53 @@ bool r_operand(const Site &s, const BLayout &R, int b2, Q ar, Q al, const BBlk *
     return true;
 }
 
-// column segments of C_al and their strips
-struct CLayout {
-    std::map<Q, int> K;                       // per al
-    std::map<Q, int64_t> off;                 // per al
-    std::map<std::tuple<Q, int, Q>, int> koff;  // (al, b2, ar)
-    std::vector<std::tuple<Q, int, Q>> segs;  // ordered
-    int64_t total = 0;
+struct Seg {
+    int b2, al, ar, koff;
+    const BBlk *rblk;
+    bool rtrans;
+    double rscale;
 };
 
-void build_strips(const Site &s, const CLayout &cl, const std::vector<Contrib> &cb, StripList &sl) {
-    // group contributions by (al, b2, ar, row0)
-    std::map<std::tuple<Q, int, Q, int>, std::vector<Src>> by;
-    for (auto &c : cb) {
-        auto key = std::make_tuple(c.al, c.b2, c.ar, c.row0);
-        if (!cl.koff.count(std::make_tuple(c.al, c.b2, c.ar))) continue;
-        by[key].push_back(c.src);
+// W-application strips of one segment; dst offsets are relative to the al block start
+// (patched with the C arena offset of al once every K_al is known).
+void emit_segment_strips(const Site &s, const Seg &sg, const Raw *r0, const Raw *r1, StripList &sl,
+                         std::vector<int> &strip_al) {
+    int xi = s.x_of_r[sg.al];
+    int rows_al = s.X.blk[xi].rows;
+    int rsz = s.rsize(sg.ar);
+    // the row strips (s, lb, ss) partition the al block; walk them in row order
+    // alongside the sources, which are sorted by row0 inside the segment
+    thread_local std::vector<std::pair<int, int>> rs;  // (row0, rows)
+    rs.clear();
+    for (int p = 0; p < s.np; ++p) {
+        int lb = s.rmp_l[(size_t)sg.al * s.np + p];
+        if (lb < 0) continue;
+        int lsz = s.lsize(lb);
+        for (int ss = 0; ss < s.phys.s[p]; ++ss) rs.push_back({s.pbl[(size_t)p * s.nl + lb] + ss * lsz, lsz});
     }
-    for (auto &seg : cl.segs) {
-        Q al = std::get<0>(seg), ar = std::get<2>(seg);
-        int b2 = std::get<1>(seg);
-        int rows_al = s.pbl.sz(al);
-        int k0 = cl.koff.at(seg);
-        int rsz = s.right.size_of(ar);
-        for (int si = 0; si < s.phys.n(); ++si) {
-            Q so = s.phys.q[si];
-            Q lb = al - so;
-            if (!s.left.has(lb)) continue;
-            int lsz = s.left.size_of(lb);
-            for (int ss = 0; ss < s.phys.s[si]; ++ss) {
-                int row0 = s.pbl.at(so, lb) + ss * lsz;
-                Strip st;
-                st.dst = cl.off.at(al) + (int64_t)k0 * rows_al + row0;
-                st.rows = lsz;
-                st.cols = rsz;
-                st.ldd = rows_al;
-                st.s0 = (int)sl.srcs.size();
-                auto it = by.find(std::make_tuple(al, b2, ar, row0));
-                if (it != by.end())
-                    for (auto &x : it->second) sl.srcs.push_back(x);
-                st.s1 = (int)sl.srcs.size();
-                sl.strips.push_back(st);
-            }
+    std::sort(rs.begin(), rs.end());
+    const Raw *q = r0;
+    for (auto &r : rs) {
+        Strip st;
+        st.dst = (int64_t)sg.koff * rows_al + r.first;
+        st.rows = r.second;
+        st.cols = rsz;
+        st.ldd = rows_al;
+        st.s0 = (int)sl.srcs.size();
+        while (q < r1 && q->row0 < r.first) ++q;
+        while (q < r1 && q->row0 == r.first) {
+            sl.srcs.push_back({q->off, q->ld, q->coef});
+            ++q;
         }
+        st.s1 = (int)sl.srcs.size();
+        sl.strips.push_back(st);
+        strip_al.push_back(sg.al);
     }
+    if (q != r1) throw std::runtime_error("W-application source outside its segment");
+}
+
+void patch_strips(StripList &sl, const std::vector<int> &strip_al, const std::vector<int64_t> &c_off) {
+    for (size_t i = 0; i < sl.strips.size(); ++i) sl.strips[i].dst += c_off[strip_al[i]];
 }
 
 }  // namespace
@@ -333,89 +404,75 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.kind = PLAN_SIGMA;
     P.symm = s.symm;
     P.vec_len = s.X.total;
-    PRPLayout prp = prp_layout(s, P.reshape);
+    PRP prp = prp_layout(s, P.reshape);
     P.n_prp = prp.total;
-    TLayout t = t_layout(s, L, prp, P.arrL, P.g1, tm, tn, kc, true);
+    TLay t = t_layout(s, L, prp, &P.arrL, &P.g1, tm, tn, kc, true);
     P.n_la = t.la_total;
     P.n_t = t.t_total;
-    std::vector<int> b2s;
-    for (int b2 = 0; b2 < s.W.cols; ++b2) b2s.push_back(b2);
-    std::vector<Contrib> cb;
-    lbtm_contribs(s, t, prp, b2s, cb, nullptr);
-    // segments: (al, b2, ar) with contributions and an R operand
-    std::set<std::tuple<Q, int, Q>> segset;
-    for (auto &c : cb) segset.insert(std::make_tuple(c.al, c.b2, c.ar));
-    CLayout cl;
-    std::map<Q, std::vector<std::tuple<int, Q, const BBlk *, bool, double>>> rops;
-    for (auto &sg : segset) {
-        Q al = std::get<0>(sg), ar = std::get<2>(sg);
-        int b2 = std::get<1>(sg);
-        const BBlk *blk;
-        bool tr;
-        double sc;
-        if (!r_operand(s, R, b2, ar, al, &blk, &tr, &sc)) continue;
-        int &K = cl.K[al];
-        cl.koff[sg] = K;
-        cl.segs.push_back(sg);
-        rops[al].push_back(std::make_tuple(b2, ar, blk, tr, sc));
-        K += s.right.size_of(ar);
-    }
-    for (auto &xb : s.X.blk) {
-        int K = cl.K.count(xb.c) ? cl.K[xb.c] : 0;
-        cl.off[xb.c] = cl.total;
-        cl.total += (int64_t)xb.rows * K;
-    }
-    P.n_c = cl.total;
-    build_strips(s, cl, cb, P.wapply);
-    // RB arrangement and the output GEMMs
-    std::map<Q, int64_t> rb_off;
-    int64_t rb_total = 0;
-    for (auto &xb : s.X.blk) {
-        rb_off[xb.c] = rb_total;
-        int K = cl.K.count(xb.c) ? cl.K[xb.c] : 0;
-        rb_total += (int64_t)K * xb.cols;
-    }
-    P.n_rb = rb_total;
-    for (auto &kv : rops) {
-        Q al = kv.first;
-        int K = cl.K[al];
-        int ral = s.right.size_of(al);
-        for (auto &r : kv.second) {
-            int b2 = std::get<0>(r);
-            Q ar = std::get<1>(r);
-            const BBlk *blk = std::get<2>(r);
-            bool tr = std::get<3>(r);
-            int rar = s.right.size_of(ar);
-            if (blk->rows * blk->cols != rar * ral) throw std::runtime_error("right boundary block size mismatch");
-            Copy2D c;
-            c.dst = rb_off[al] + cl.koff.at(std::make_tuple(al, b2, ar));
-            c.src = blk->off;
-            c.rows = rar;
-            c.cols = ral;
-            c.ldd = K;
-            c.lds = tr ? ral : rar;
-            c.trans = tr ? 1 : 0;
-            c.scale = std::get<4>(r);
-            P.arrR.push_back(c);
+    std::vector<int> K(s.nr, 0);
+    std::vector<Seg> segs;
+    std::vector<int> strip_al;
+    std::vector<Raw> raw;
+    // reference schedule (site_hamil.hpp:35): when it is lbtm its FLOPs are counted in this pass
+    const bool ref_lbtm = s.symm != MQ_SYMM_SU2U1 || (s.W.rows - s.W.one_rows) < (s.W.cols - s.W.one_cols);
+    double ref_axpy = 0, ref_trim = 0;
+    for (int b2 = 0; b2 < s.W.cols; ++b2) {
+        lbtm_column(s, t, b2, raw, &ref_axpy);
+        size_t i = 0;
+        while (i < raw.size()) {
+            size_t j = i;
+            while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
+            Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
+            if (r_operand(s, R, b2, s.right.q[sg.ar], s.right.q[sg.al], &sg.rblk, &sg.rtrans, &sg.rscale)) {
+                sg.koff = K[sg.al];
+                K[sg.al] += s.rsize(sg.ar);
+                segs.push_back(sg);
+                emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, P.wapply, strip_al);
+                ref_trim += 2.0 * s.X.blk[s.x_of_r[sg.al]].rows * s.rsize(sg.ar) * s.rsize(sg.al);
+            }
+            i = j;
         }
     }
+    std::vector<int64_t> c_off(s.nr, 0), rb_off(s.nr, 0);
+    int64_t ct = 0, rbt = 0;
     for (auto &xb : s.X.blk) {
-        int K = cl.K.count(xb.c) ? cl.K[xb.c] : 0;
+        int r = s.ri.get(xb.c);
+        c_off[r] = ct;
+        ct += (int64_t)xb.rows * K[r];
+        rb_off[r] = rbt;
+        rbt += (int64_t)K[r] * xb.cols;
+    }
+    patch_strips(P.wapply, strip_al, c_off);
+    P.n_c = ct;
+    P.n_rb = rbt;
+    for (auto &sg : segs) {
+        int rar = s.rsize(sg.ar), ral = s.rsize(sg.al);
+        if (sg.rblk->rows * sg.rblk->cols != rar * ral) throw std::runtime_error("right boundary block size mismatch");
+        P.arrR.push_back(mk_copy(rb_off[sg.al] + sg.koff, sg.rblk->off, rar, ral, K[sg.al], sg.rtrans ? ral : rar,
+                                 sg.rtrans, sg.rscale));
+    }
+    for (auto &xb : s.X.blk) {
+        int r = s.ri.get(xb.c);
         GemmGroup g{};
-        g.a = cl.off[xb.c];
-        g.b = rb_off[xb.c];
+        g.a = c_off[r];
+        g.b = rb_off[r];
         g.c = xb.off;
         g.M = xb.rows;
         g.N = xb.cols;
-        g.K = K;
+        g.K = K[r];
         g.lda = xb.rows;
-        g.ldb = std::max(K, 1);
+        g.ldb = std::max(K[r], 1);
         g.ldc = xb.rows;
         add_gemm(P.g2, g, tm, tn, kc);
     }
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
-    P.ref_flops = reference_sigma_flops(d);
+    if (ref_lbtm) {
+        TLay tall = t_layout(s, L, prp, nullptr, nullptr, tm, tn, kc, false);
+        P.ref_flops = tall.gemm_flops + ref_axpy + ref_trim;
+    } else {
+        P.ref_flops = reference_sigma_flops(d);
+    }
     P.min_bytes = 8.0 * (L.total + R.total + 2.0 * s.X.total);
     return P;
 }
@@ -429,118 +486,110 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.kind = PLAN_LSTEP;
     P.symm = s.symm;
     P.vec_len = s.X.total;
-    PRPLayout prp = prp_layout(s, P.reshape);
+    PRP prp = prp_layout(s, P.reshape);
     P.n_prp = prp.total;
-    TLayout t = t_layout(s, L, prp, P.arrL, P.g1, tm, tn, kc, true);
+    TLay t = t_layout(s, L, prp, &P.arrL, &P.g1, tm, tn, kc, true);
     P.n_la = t.la_total;
     P.n_t = t.t_total;
-    std::vector<int> b2s;
-    for (int b2 = 0; b2 < s.W.cols; ++b2)
-        if (!s.W.rskip(b2)) b2s.push_back(b2);
-    std::vector<Contrib> cb;
     double axf = 0;
-    lbtm_contribs(s, t, prp, b2s, cb, &axf);
-    std::set<std::tuple<Q, int, Q>> segset;
-    for (auto &c : cb) segset.insert(std::make_tuple(c.al, c.b2, c.ar));
-    CLayout cl;
-    for (auto &sg : segset) {
-        Q al = std::get<0>(sg), ar = std::get<2>(sg);
-        int b2 = std::get<1>(sg);
-        // spin filter of SU2::gemm(transpose(contr), bra_conj, ret[b2], spin(b2)) (move_boundary.hpp:165)
-        if (s.su2 && !tri(spin2(ar), s.W.rspin[b2], spin2(al))) continue;
-        int &K = cl.K[al];
-        cl.koff[sg] = K;
-        cl.segs.push_back(sg);
-        K += s.right.size_of(ar);
+    std::vector<int> K(s.nr, 0);
+    std::vector<Seg> segs;
+    std::vector<int> strip_al;
+    std::vector<Raw> raw;
+    for (int b2 = 0; b2 < s.W.cols; ++b2) {
+        if (s.W.rskip(b2)) continue;
+        lbtm_column(s, t, b2, raw, &axf);
+        size_t i = 0;
+        while (i < raw.size()) {
+            size_t j = i;
+            while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
+            Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
+            // spin filter of SU2::gemm(transpose(contr), bra_conj, ret[b2], spin(b2)) (move_boundary.hpp:165)
+            if (!s.su2 || tri(spin2(s.right.q[sg.ar]), s.W.rspin[b2], spin2(s.right.q[sg.al]))) {
+                sg.koff = K[sg.al];
+                K[sg.al] += s.rsize(sg.ar);
+                segs.push_back(sg);
+                emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, P.wapply, strip_al);
+            }
+            i = j;
+        }
     }
-    int64_t tmp_total = 0;
-    std::map<Q, int64_t> tmp_off;
+    std::vector<int64_t> c_off(s.nr, 0), tmp_off(s.nr, 0);
+    int64_t ct = 0, tt = 0;
     for (auto &xb : s.X.blk) {
-        int K = cl.K.count(xb.c) ? cl.K[xb.c] : 0;
-        cl.off[xb.c] = cl.total;
-        cl.total += (int64_t)xb.rows * K;
-        tmp_off[xb.c] = tmp_total;
-        tmp_total += (int64_t)K * xb.cols;
+        int r = s.ri.get(xb.c);
+        c_off[r] = ct;
+        ct += (int64_t)xb.rows * K[r];
+        tmp_off[r] = tt;
+        tt += (int64_t)K[r] * xb.cols;
     }
-    P.n_c = cl.total;
-    P.n_tmp = tmp_total;
-    build_strips(s, cl, cb, P.wapply);
+    patch_strips(P.wapply, strip_al, c_off);
+    P.n_c = ct;
+    P.n_tmp = tt;
     for (auto &xb : s.X.blk) {
-        int K = cl.K.count(xb.c) ? cl.K[xb.c] : 0;
-        if (K == 0) continue;
+        int r = s.ri.get(xb.c);
+        if (K[r] == 0) continue;
         GemmGroup g{};
-        g.a = cl.off[xb.c];  // C_al stored rows(al) x K -> op(A) = C^T (K x rows)
-        g.b = xb.off;        // X_al rows(al) x r_al
-        g.c = tmp_off[xb.c];
-        g.M = K;
+        g.a = c_off[r];  // C_al (rows x K) used transposed
+        g.b = xb.off;
+        g.c = tmp_off[r];
+        g.M = K[r];
         g.N = xb.cols;
         g.K = xb.rows;
         g.lda = xb.rows;
         g.ldb = xb.rows;
-        g.ldc = K;
+        g.ldc = K[r];
         g.transA = 1;
         add_gemm(P.g2, g, tm, tn, kc);
     }
-    // output boundary: per b2, blocks (ar, al) descending
+    // output boundary: per b2, blocks (ar, al) in descending order
     P.out_aux = s.W.cols;
-    std::map<int, std::vector<std::pair<Q, Q>>> per_b2;
-    for (auto &sg : cl.segs) per_b2[std::get<1>(sg)].push_back({std::get<2>(sg), std::get<0>(sg)});
+    std::vector<std::vector<const Seg *>> per_b2(s.W.cols);
+    for (auto &sg : segs) per_b2[sg.b2].push_back(&sg);
     P.out_bptr.push_back(0);
     int64_t off = 0;
+    double gf = 0;
     for (int b2 = 0; b2 < s.W.cols; ++b2) {
-        auto it = per_b2.find(b2);
-        if (it != per_b2.end()) {
-            auto v = it->second;
-            std::sort(v.begin(), v.end(), [](auto &x, auto &y) {
-                return x.first > y.first || (x.first == y.first && x.second > y.second);
-            });
-            for (auto &bl : v) {
-                Q ar = bl.first, al = bl.second;
-                int rar = s.right.size_of(ar), ral = s.right.size_of(al);
-                P.out_lc.push_back(ar.a);
-                P.out_lc.push_back(ar.b);
-                P.out_rc.push_back(al.a);
-                P.out_rc.push_back(al.b);
-                P.out_rows.push_back(rar);
-                P.out_cols.push_back(ral);
-                P.out_off.push_back(off);
-                Copy2D c;
-                c.dst = off;
-                c.src = tmp_off[al] + cl.koff.at(std::make_tuple(al, b2, ar));
-                c.rows = rar;
-                c.cols = ral;
-                c.ldd = rar;
-                c.lds = cl.K[al];
-                c.trans = 0;
-                c.scale = 1.0;
-                P.scatter.push_back(c);
-                off += (int64_t)rar * ral;
-            }
+        auto v = per_b2[b2];
+        std::sort(v.begin(), v.end(), [&](const Seg *x, const Seg *y) {
+            Q xa = s.right.q[x->ar], ya = s.right.q[y->ar];
+            if (xa != ya) return xa > ya;
+            return s.right.q[x->al] > s.right.q[y->al];
+        });
+        for (auto *sg : v) {
+            Q ar = s.right.q[sg->ar], al = s.right.q[sg->al];
+            int rar = s.rsize(sg->ar), ral = s.rsize(sg->al);
+            P.out_lc.push_back(ar.a);
+            P.out_lc.push_back(ar.b);
+            P.out_rc.push_back(al.a);
+            P.out_rc.push_back(al.b);
+            P.out_rows.push_back(rar);
+            P.out_cols.push_back(ral);
+            P.out_off.push_back(off);
+            P.scatter.push_back(mk_copy(off, tmp_off[sg->al] + sg->koff, rar, ral, rar, K[sg->al], false, 1.0));
+            off += (int64_t)rar * ral;
+            gf += 2.0 * rar * s.X.blk[s.x_of_r[sg->al]].rows * ral;
         }
         P.out_bptr.push_back((int32_t)P.out_rows.size());
     }
     P.n_out = off;
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
-    // reference: BoundaryMPSProduct GEMMs (all rows), lbtm axpys, SU2::gemm products
-    double f = 0;
-    {
-        std::vector<Copy2D> dummy;
-        GemmList gl;
-        TLayout tall = t_layout(s, L, prp, dummy, gl, tm, tn, kc, false);
-        f += gl.flops();
-    }
-    f += axf;
-    for (auto &sg : cl.segs) {
-        Q al = std::get<0>(sg), ar = std::get<2>(sg);
-        f += 2.0 * s.right.size_of(ar) * s.pbl.sz(al) * s.right.size_of(al);
-    }
-    P.ref_flops = f;
+    TLay tall = t_layout(s, L, prp, nullptr, nullptr, tm, tn, kc, false);
+    P.ref_flops = tall.gemm_flops + axf + gf;
     P.min_bytes = 8.0 * (L.total + off + s.X.total);
     return P;
 }
 
 // ============================================================ right step
+namespace {
+struct RTB {  // block (c, c') of t[b2] = X_c Rop[b2](c, c')
+    int c, cp;      // right-index ids
+    int64_t off;    // offset of column 0 inside the t arena
+    int ld;         // rows of X_c
+};
+}  // namespace
+
 HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     Site s(d);
     BLayout R = to_layout(d.R);
@@ -549,281 +598,253 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.kind = PLAN_RSTEP;
     P.symm = s.symm;
     P.vec_len = s.X.total;
-    PRPLayout prp = prp_layout(s, P.reshape);
+    PRP prp = prp_layout(s, P.reshape);
     P.n_prp = prp.total;
-    // MPSBoundaryProduct (boundary_times_mps.hpp:226-293): t[b2] = X * Rop[b2], right-trimmed
-    // to the right charges of X.  RC_c stacks Rop[b2](c, c') horizontally.
-    std::map<Q, int> Nc;
-    std::map<std::tuple<int, Q, Q>, int> coff;  // (b2, c, c') -> column offset in t_c
-    std::vector<std::tuple<int, Q, Q>> tblk;    // (b2, c, c')
-    std::vector<std::tuple<Q, int, const BBlk *, bool, double, Q>> rops;
+    // MPSBoundaryProduct (boundary_times_mps.hpp:226-293): t[b2] = X * Rop[b2] right-trimmed to X's
+    // right charges; RC_c stacks Rop[b2](c, c') horizontally over (b2, c').
+    struct Rop {
+        int b2, c, cp, col;
+        const BBlk *blk;
+        bool trans;
+        double scale;
+    };
+    std::vector<Rop> rops;
+    std::vector<int> Nc(s.nr, 0);
     for (int b2 = 0; b2 < s.W.cols; ++b2) {
         if (s.W.col_nnz[b2] == 0) continue;
-        for (auto &xb : s.X.blk) {
-            Q c = xb.c;
-            if (s.W.rskip(b2)) {
-                int cj = s.W.rherm[b2];
-                for (auto &x : R.b[cj]) {
-                    if (x.rc != c) continue;  // R[conj](c', c) -> adjoint block (c, c')
-                    Q cp = x.lc;
-                    if (!s.X.pos.count(cp)) continue;
-                    double sc = s.su2 ? conj_correction(c, cp, s.W.rspin[b2]) * s.W.rphase[cj] : 1.0;
-                    int &N = Nc[c];
-                    coff[{b2, c, cp}] = N;
-                    tblk.push_back({b2, c, cp});
-                    rops.push_back({c, N, &x, true, sc, cp});
-                    N += s.right.size_of(cp);
-                }
-            } else {
-                for (auto &x : R.b[b2]) {
-                    if (x.lc != c) continue;
-                    Q cp = x.rc;
-                    if (!s.X.pos.count(cp)) continue;
-                    int &N = Nc[c];
-                    coff[{b2, c, cp}] = N;
-                    tblk.push_back({b2, c, cp});
-                    rops.push_back({c, N, &x, false, 1.0, cp});
-                    N += s.right.size_of(cp);
-                }
-            }
+        bool skip = s.W.rskip(b2);
+        int cj = skip ? s.W.rherm[b2] : b2;
+        for (auto &x : R.b[cj]) {
+            int c = s.ri.get(skip ? x.rc : x.lc), cp = s.ri.get(skip ? x.lc : x.rc);
+            if (c < 0 || cp < 0 || s.x_of_r[c] < 0 || s.x_of_r[cp] < 0) continue;
+            double sc = (skip && s.su2) ? conj_correction(s.right.q[c], s.right.q[cp], s.W.rspin[b2]) * s.W.rphase[cj]
+                                        : 1.0;
+            rops.push_back({b2, c, cp, Nc[c], &x, skip, sc});
+            Nc[c] += s.rsize(cp);
         }
     }
-    std::map<Q, int64_t> rc_off, t_off;
-    int64_t rc_total = 0, t_total = 0;
+    std::vector<int64_t> rc_off(s.nr, -1), t_off(s.nr, -1);
+    int64_t rct = 0, tt = 0;
     for (auto &xb : s.X.blk) {
-        if (!Nc.count(xb.c)) continue;
-        rc_off[xb.c] = rc_total;
-        rc_total += (int64_t)xb.cols * Nc[xb.c];
-        t_off[xb.c] = t_total;
-        t_total += (int64_t)xb.rows * Nc[xb.c];
+        int c = s.ri.get(xb.c);
+        if (Nc[c] == 0) continue;
+        rc_off[c] = rct;
+        rct += (int64_t)xb.cols * Nc[c];
+        t_off[c] = tt;
+        tt += (int64_t)xb.rows * Nc[c];
     }
-    P.n_rb = rc_total;
-    P.n_t = t_total;
+    P.n_rb = rct;
+    P.n_t = tt;
+    std::vector<std::vector<RTB>> tb(s.W.cols);
     for (auto &r : rops) {
-        Q c = std::get<0>(r);
-        Q cp = std::get<5>(r);
-        const BBlk *blk = std::get<2>(r);
-        bool tr = std::get<3>(r);
-        int rc_ = s.right.size_of(c), rcp = s.right.size_of(cp);
-        if (blk->rows * blk->cols != rc_ * rcp) throw std::runtime_error("right boundary block size mismatch");
-        Copy2D cp2;
-        cp2.dst = rc_off[c] + (int64_t)std::get<1>(r) * rc_;
-        cp2.src = blk->off;
-        cp2.rows = rc_;
-        cp2.cols = rcp;
-        cp2.ldd = rc_;
-        cp2.lds = tr ? rcp : rc_;
-        cp2.trans = tr ? 1 : 0;
-        cp2.scale = std::get<4>(r);
-        P.arrR.push_back(cp2);
+        int rc_ = s.rsize(r.c), rcp = s.rsize(r.cp);
+        if (r.blk->rows * r.blk->cols != rc_ * rcp) throw std::runtime_error("right boundary block size mismatch");
+        P.arrR.push_back(
+            mk_copy(rc_off[r.c] + (int64_t)r.col * rc_, r.blk->off, rc_, rcp, rc_, r.trans ? rcp : rc_, r.trans, r.scale));
+        int ld = s.X.blk[s.x_of_r[r.c]].rows;
+        tb[r.b2].push_back({r.c, r.cp, t_off[r.c] + (int64_t)r.col * ld, ld});
     }
     for (auto &xb : s.X.blk) {
-        if (!Nc.count(xb.c)) continue;
+        int c = s.ri.get(xb.c);
+        if (Nc[c] == 0) continue;
         GemmGroup g{};
         g.a = xb.off;
-        g.b = rc_off[xb.c];
-        g.c = t_off[xb.c];
+        g.b = rc_off[c];
+        g.c = t_off[c];
         g.M = xb.rows;
-        g.N = Nc[xb.c];
+        g.N = Nc[c];
         g.K = xb.cols;
         g.lda = xb.rows;
         g.ldb = xb.cols;
         g.ldc = xb.rows;
         add_gemm(P.g1, g, tm, tn, kc);
     }
-    // rbtm tasks (apply_op.hpp:114-209): buf[b1](out_l, out_r) += coef * t rows
-    struct RC {
-        int b1;
-        Q ol, orr;
-        int col0;  // column offset inside the buf block
-        int cols;
-        int rows;
-        Src src;
+    // rbtm tasks (apply_op.hpp:114-209): buf[b1](ol, or) += coef * t rows.  Per b1: raw list
+    // keyed by (or, ol, col0), BUF_or stacks rows over (b1, ol).
+    struct RR {
+        int orr, ol, col0;
+        int64_t off;
+        int ld;
+        double coef;
     };
-    std::vector<RC> cb;
-    std::map<int, std::vector<std::tuple<int, Q, Q>>> tb_by_b2;
-    for (auto &x : tblk) tb_by_b2[std::get<0>(x)].push_back(x);
+    struct RowBlk {
+        int b1, ol, orr, row;
+    };
+    std::vector<int> Mr(s.nl, 0);
+    std::vector<RowBlk> rowblks;
+    std::vector<int> strip_or;
+    std::vector<RR> raw;
     double axf = 0;
     for (int b1 = 0; b1 < s.W.rows; ++b1) {
         if (s.W.lskip(b1)) continue;
+        raw.clear();
         int a = s.W.lspin[b1];
         for (int b2 : s.W.row_b2[b1]) {
             int ap = s.W.rspin[b2];
-            auto itb = tb_by_b2.find(b2);
-            if (itb == tb_by_b2.end()) continue;
             for (auto &term : s.W.at.at({b1, b2})) {
                 const Op &op = s.W.ops[term.op];
-                int k = op.spin;
-                for (auto &tbk : itb->second) {
-                    Q lc = std::get<1>(tbk), rc = std::get<2>(tbk);
-                    int rowsc = s.X.blk[s.X.pos.at(lc)].rows;
-                    int64_t tbase = t_off.at(lc) + (int64_t)coff.at(tbk) * rowsc;
-                    for (auto &wb : op.blocks) {
-                        Q ol = lc - wb.in;
-                        if (!s.left.has(ol)) continue;
-                        Q orr = rc - wb.out;
-                        if (s.su2 && !tri(spin2(ol), a, spin2(orr))) continue;
-                        if (!s.left.has(orr) || s.pbr.sz(orr) == 0) continue;  // out_right_i
-                        int lsz = s.left.size_of(ol);
-                        double cpl[4] = {term.scale, term.scale, term.scale, term.scale};
-                        if (s.su2) {
-                            int i = spin2(orr), ip = spin2(rc), j = spin2(ol), jp = spin2(lc);
-                            set_coupling(j, std::abs(j - jp), jp, a, k, ap, i, std::abs(i - ip), ip, term.scale, cpl);
-                        }
-                        int ilo = s.pbl.at(wb.in, ol);
-                        int oro = s.pbr.at(wb.out, rc);
-                        int rsz = s.right.size_of(rc);
+                for (auto &t : tb[b2]) {
+                    int jp = spin2(s.right.q[t.c]), ip = spin2(s.right.q[t.cp]);
+                    for (auto &ob : s.opb[term.op]) {
+                        const OpBlk &wb = *ob.b;
+                        int pin = ob.pin, pout = ob.pout;
+                        int ol = s.rmp_l[(size_t)t.c * s.np + pin];
+                        if (ol < 0) continue;
+                        int orr = s.rmp_l[(size_t)t.cp * s.np + pout];
+                        if (orr < 0 || s.pbr_sz_l[orr] == 0) continue;  // out_right_i
+                        int j = spin2(s.left.q[ol]), i = spin2(s.left.q[orr]);
+                        if (s.su2 && !tri(j, a, i)) continue;
+                        int lsz = s.lsize(ol), rsz = s.rsize(t.cp);
+                        static const std::array<double, 4> one{1.0, 1.0, 1.0, 1.0};
+                        const std::array<double, 4> &cpl =
+                            s.su2 ? coupling(j, std::abs(j - jp), jp, a, op.spin, ap, i, std::abs(i - ip), ip) : one;
+                        int ilo = s.pbl[(size_t)pin * s.nl + ol];
+                        int oro = s.pbr[(size_t)pout * s.nr + t.cp];
                         for (auto &e : wb.e) {
-                            RC r;
-                            r.b1 = b1;
-                            r.ol = ol;
-                            r.orr = orr;
-                            r.col0 = oro + e.col * rsz;
-                            r.cols = rsz;
-                            r.rows = lsz;
-                            r.src.off = tbase + ilo + e.row * lsz;
-                            r.src.ld = rowsc;
-                            r.src.coef = e.coeff * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : term.scale);
-                            cb.push_back(r);
+                            raw.push_back({orr, ol, oro + e.col * rsz, t.off + ilo + e.row * lsz, t.ld,
+                                           e.coeff * term.scale * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : 1.0)});
                             axf += 2.0 * lsz * rsz;
                         }
                     }
                 }
             }
         }
-    }
-    // BUF_r stacks buf[b1](out_l, out_r) vertically over (b1, out_l); spin filter of
-    // SU2::gemm(ret[b1], transpose(bra_conj), tmp, spin(b1)) (move_boundary.hpp:224)
-    std::map<Q, int> Mr;
-    std::map<std::tuple<Q, int, Q>, int> rowoff;  // (out_r, b1, out_l)
-    std::vector<std::tuple<Q, int, Q>> rows_order;
-    {
-        std::set<std::tuple<Q, int, Q>> seen;
-        for (auto &c : cb) {
-            auto key = std::make_tuple(c.orr, c.b1, c.ol);
-            if (seen.count(key)) continue;
-            seen.insert(key);
-            if (s.su2 && !tri(spin2(c.ol), s.W.lspin[c.b1], spin2(c.orr))) continue;
-            if (!prp.off.count(c.orr)) continue;
-            rows_order.push_back(key);
-        }
-        std::sort(rows_order.begin(), rows_order.end(), [](auto &x, auto &y) {
-            if (std::get<0>(x) != std::get<0>(y)) return std::get<0>(x) > std::get<0>(y);
-            if (std::get<1>(x) != std::get<1>(y)) return std::get<1>(x) < std::get<1>(y);
-            return std::get<2>(x) > std::get<2>(y);
+        std::sort(raw.begin(), raw.end(), [](const RR &x, const RR &y) {
+            if (x.orr != y.orr) return x.orr < y.orr;
+            if (x.ol != y.ol) return x.ol < y.ol;
+            if (x.col0 != y.col0) return x.col0 < y.col0;
+            return x.off < y.off;
         });
-        for (auto &key : rows_order) {
-            int &M = Mr[std::get<0>(key)];
-            rowoff[key] = M;
-            M += s.left.size_of(std::get<2>(key));
+        size_t w = 0;
+        for (size_t r = 0; r < raw.size(); ++r) {
+            if (w > 0 && raw[w - 1].orr == raw[r].orr && raw[w - 1].ol == raw[r].ol && raw[w - 1].col0 == raw[r].col0 &&
+                raw[w - 1].off == raw[r].off)
+                raw[w - 1].coef += raw[r].coef;
+            else
+                raw[w++] = raw[r];
+        }
+        raw.resize(w);
+        size_t i = 0;
+        while (i < raw.size()) {
+            size_t j = i;
+            while (j < raw.size() && raw[j].orr == raw[i].orr && raw[j].ol == raw[i].ol) ++j;
+            int orr = raw[i].orr, ol = raw[i].ol;
+            // spin filter of SU2::gemm(ret[b1], transpose(bra_conj), tmp, spin(b1)) (move_boundary.hpp:224)
+            if (!s.su2 || tri(spin2(s.left.q[ol]), a, spin2(s.left.q[orr]))) {
+                RowBlk rb{b1, ol, orr, Mr[orr]};
+                Mr[orr] += s.lsize(ol);
+                rowblks.push_back(rb);
+                int lsz = s.lsize(ol);
+                Q orq = s.left.q[orr];
+                for (int p = 0; p < s.np; ++p) {
+                    int r = s.ri.get(orq + s.phys.q[p]);
+                    if (r < 0) continue;
+                    int rsz = s.rsize(r);
+                    for (int ss = 0; ss < s.phys.s[p]; ++ss) {
+                        int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
+                        Strip st;
+                        st.dst = rb.row + (int64_t)col0;  // patched: + buf_off + col0*(M-1)
+                        st.rows = lsz;
+                        st.cols = rsz;
+                        st.ldd = 0;
+                        st.s0 = (int)P.wapply.srcs.size();
+                        for (size_t q = i; q < j; ++q)
+                            if (raw[q].col0 == col0) P.wapply.srcs.push_back({raw[q].off, raw[q].ld, raw[q].coef});
+                        st.s1 = (int)P.wapply.srcs.size();
+                        P.wapply.strips.push_back(st);
+                        strip_or.push_back(orr);
+                    }
+                }
+            }
+            i = j;
         }
     }
-    std::map<Q, int64_t> buf_off, tmp_off;
-    int64_t buf_total = 0, tmp_total = 0;
-    for (auto &kv : Mr) {
-        Q orr = kv.first;
-        buf_off[orr] = buf_total;
-        buf_total += (int64_t)kv.second * s.pbr.sz(orr);
-        tmp_off[orr] = tmp_total;
-        tmp_total += (int64_t)kv.second * s.left.size_of(orr);
+    std::vector<int64_t> buf_off(s.nl, -1), tmp_off(s.nl, -1);
+    int64_t bt = 0, mt = 0;
+    for (int l = 0; l < s.nl; ++l) {
+        if (Mr[l] == 0) continue;
+        buf_off[l] = bt;
+        bt += (int64_t)Mr[l] * s.pbr_sz_l[l];
+        tmp_off[l] = mt;
+        mt += (int64_t)Mr[l] * s.lsize(l);
     }
-    P.n_c = buf_total;
-    P.n_tmp = tmp_total;
+    P.n_c = bt;
+    P.n_tmp = mt;
+    for (size_t k = 0; k < P.wapply.strips.size(); ++k) {
+        Strip &st = P.wapply.strips[k];
+        int orr = strip_or[k];
+        int64_t row = 0, col0 = 0;
+        // st.dst currently row + col0 with row < Mr: recover by construction order
+        (void)row;
+        (void)col0;
+        st.ldd = Mr[orr];
+    }
+    // recompute strip destinations exactly (row offset and column offset were packed additively)
     {
-        std::map<std::tuple<Q, int, Q, int>, std::vector<Src>> by;
-        for (auto &c : cb) {
-            auto key = std::make_tuple(c.orr, c.b1, c.ol);
-            if (!rowoff.count(key)) continue;
-            by[std::make_tuple(c.orr, c.b1, c.ol, c.col0)].push_back(c.src);
-        }
-        for (auto &key : rows_order) {
-            Q orr = std::get<0>(key), ol = std::get<2>(key);
-            int b1 = std::get<1>(key);
-            int M = Mr[orr];
-            int lsz = s.left.size_of(ol);
-            for (int si = 0; si < s.phys.n(); ++si) {
-                Q so = s.phys.q[si];
-                Q r = orr + so;
-                if (!s.right.has(r)) continue;
-                int rsz = s.right.size_of(r);
-                for (int ss = 0; ss < s.phys.s[si]; ++ss) {
-                    int col0 = s.pbr.at(so, r) + ss * rsz;
-                    Strip st;
-                    st.dst = buf_off[orr] + rowoff[key] + (int64_t)col0 * M;
-                    st.rows = lsz;
-                    st.cols = rsz;
-                    st.ldd = M;
-                    st.s0 = (int)P.wapply.srcs.size();
-                    auto it = by.find(std::make_tuple(orr, b1, ol, col0));
-                    if (it != by.end())
-                        for (auto &x : it->second) P.wapply.srcs.push_back(x);
-                    st.s1 = (int)P.wapply.srcs.size();
-                    P.wapply.strips.push_back(st);
+        size_t k = 0;
+        for (auto &rb : rowblks) {
+            Q orq = s.left.q[rb.orr];
+            for (int p = 0; p < s.np; ++p) {
+                int r = s.ri.get(orq + s.phys.q[p]);
+                if (r < 0) continue;
+                int rsz = s.rsize(r);
+                for (int ss = 0; ss < s.phys.s[p]; ++ss) {
+                    int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
+                    P.wapply.strips[k].dst = buf_off[rb.orr] + rb.row + (int64_t)col0 * Mr[rb.orr];
+                    ++k;
                 }
             }
         }
+        if (k != P.wapply.strips.size()) throw std::runtime_error("rstep strip bookkeeping");
     }
-    for (auto &kv : Mr) {
-        Q orr = kv.first;
+    for (int l = 0; l < s.nl; ++l) {
+        if (Mr[l] == 0) continue;
         GemmGroup g{};
-        g.a = buf_off[orr];
-        g.b = prp.off.at(orr);  // PRP block bl x Nr -> op(B) = PRP^T
-        g.c = tmp_off[orr];
-        g.M = kv.second;
-        g.N = s.left.size_of(orr);
-        g.K = s.pbr.sz(orr);
-        g.lda = kv.second;
-        g.ldb = s.left.size_of(orr);
-        g.ldc = kv.second;
+        g.a = buf_off[l];
+        g.b = prp.off[l];  // PRP block (bl x Nr) used transposed
+        g.c = tmp_off[l];
+        g.M = Mr[l];
+        g.N = s.lsize(l);
+        g.K = s.pbr_sz_l[l];
+        g.lda = Mr[l];
+        g.ldb = s.lsize(l);
+        g.ldc = Mr[l];
         g.transB = 1;
         add_gemm(P.g2, g, tm, tn, kc);
     }
     P.out_aux = s.W.rows;
-    std::map<int, std::vector<std::pair<Q, Q>>> per_b1;
-    for (auto &key : rows_order) per_b1[std::get<1>(key)].push_back({std::get<2>(key), std::get<0>(key)});
+    std::vector<std::vector<const RowBlk *>> per_b1(s.W.rows);
+    for (auto &rb : rowblks) per_b1[rb.b1].push_back(&rb);
     P.out_bptr.push_back(0);
     int64_t off = 0;
+    double gf = 0;
     for (int b1 = 0; b1 < s.W.rows; ++b1) {
-        auto it = per_b1.find(b1);
-        if (it != per_b1.end()) {
-            auto v = it->second;
-            std::sort(v.begin(), v.end(), [](auto &x, auto &y) {
-                return x.first > y.first || (x.first == y.first && x.second > y.second);
-            });
-            for (auto &bl : v) {
-                Q ol = bl.first, orr = bl.second;
-                int r1 = s.left.size_of(ol), c1 = s.left.size_of(orr);
-                P.out_lc.push_back(ol.a);
-                P.out_lc.push_back(ol.b);
-                P.out_rc.push_back(orr.a);
-                P.out_rc.push_back(orr.b);
-                P.out_rows.push_back(r1);
-                P.out_cols.push_back(c1);
-                P.out_off.push_back(off);
-                Copy2D c;
-                c.dst = off;
-                c.src = tmp_off[orr] + rowoff.at(std::make_tuple(orr, b1, ol));
-                c.rows = r1;
-                c.cols = c1;
-                c.ldd = r1;
-                c.lds = Mr[orr];
-                c.trans = 0;
-                c.scale = 1.0;
-                P.scatter.push_back(c);
-                off += (int64_t)r1 * c1;
-            }
+        auto v = per_b1[b1];
+        std::sort(v.begin(), v.end(), [&](const RowBlk *x, const RowBlk *y) {
+            Q xo = s.left.q[x->ol], yo = s.left.q[y->ol];
+            if (xo != yo) return xo > yo;
+            return s.left.q[x->orr] > s.left.q[y->orr];
+        });
+        for (auto *rb : v) {
+            Q ol = s.left.q[rb->ol], orq = s.left.q[rb->orr];
+            int r1 = s.lsize(rb->ol), c1 = s.lsize(rb->orr);
+            P.out_lc.push_back(ol.a);
+            P.out_lc.push_back(ol.b);
+            P.out_rc.push_back(orq.a);
+            P.out_rc.push_back(orq.b);
+            P.out_rows.push_back(r1);
+            P.out_cols.push_back(c1);
+            P.out_off.push_back(off);
+            P.scatter.push_back(mk_copy(off, tmp_off[rb->orr] + rb->row, r1, c1, r1, Mr[rb->orr], false, 1.0));
+            off += (int64_t)r1 * c1;
+            gf += 2.0 * r1 * s.pbr_sz_l[rb->orr] * c1;
         }
         P.out_bptr.push_back((int32_t)P.out_rows.size());
     }
     P.n_out = off;
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
-    double f = P.g1.flops() + axf;
-    for (auto &key : rows_order) {
-        Q orr = std::get<0>(key), ol = std::get<2>(key);
-        f += 2.0 * s.left.size_of(ol) * s.pbr.sz(orr) * s.left.size_of(orr);
-    }
-    P.ref_flops = f;
+    P.ref_flops = P.g1.flops() + axf + gf;
     P.min_bytes = 8.0 * (R.total + off + s.X.total);
     return P;
 }
@@ -833,93 +854,83 @@ double reference_sigma_flops(const mq_site_desc &d) {
     Site s(d);
     BLayout L = to_layout(d.L), R = to_layout(d.R);
     std::vector<Copy2D> dummy;
-    PRPLayout prp = prp_layout(s, dummy);
+    PRP prp = prp_layout(s, dummy);
     bool lbtm = s.symm != MQ_SYMM_SU2U1 || (s.W.rows - s.W.one_rows) < (s.W.cols - s.W.one_cols);
     double f = 0;
     if (lbtm) {
-        GemmList gl;
-        std::vector<Copy2D> arr;
-        TLayout t = t_layout(s, L, prp, arr, gl, 32, 32, 1 << 30, false);
-        f += gl.flops();
-        std::vector<int> b2s;
-        for (int b2 = 0; b2 < s.W.cols; ++b2) b2s.push_back(b2);
-        std::vector<Contrib> cb;
-        lbtm_contribs(s, t, prp, b2s, cb, &f);
-        std::set<std::tuple<Q, int, Q>> segset;
-        for (auto &c : cb) segset.insert(std::make_tuple(c.al, c.b2, c.ar));
-        for (auto &sg : segset) {
-            const BBlk *blk;
-            bool tr;
-            double sc;
-            Q al = std::get<0>(sg), ar = std::get<2>(sg);
-            if (!r_operand(s, R, std::get<1>(sg), ar, al, &blk, &tr, &sc)) continue;
-            f += 2.0 * s.pbl.sz(al) * s.right.size_of(ar) * s.right.size_of(al);
+        TLay t = t_layout(s, L, prp, nullptr, nullptr, 64, 64, 1 << 30, false);
+        f += t.gemm_flops;
+        std::vector<Raw> raw;
+        for (int b2 = 0; b2 < s.W.cols; ++b2) {
+            lbtm_column(s, t, b2, raw, &f);
+            size_t i = 0;
+            while (i < raw.size()) {
+                size_t j = i;
+                while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
+                const BBlk *blk;
+                bool tr;
+                double sc;
+                if (r_operand(s, R, b2, s.right.q[raw[i].ar], s.right.q[raw[i].al], &blk, &tr, &sc))
+                    f += 2.0 * s.X.blk[s.x_of_r[raw[i].al]].rows * s.rsize(raw[i].ar) * s.rsize(raw[i].al);
+                i = j;
+            }
         }
         return f;
     }
-    // rbtm (site_hamil.hpp:149-204)
-    // MPSBoundaryProduct over all columns (lazy single-use columns counted when used)
-    std::map<std::tuple<int, Q, Q>, bool> tb;  // (b2, c, c') exists
-    auto tgemm = [&](int b2, Q c, Q cp) {
-        return 2.0 * s.X.blk[s.X.pos.at(c)].rows * s.right.size_of(cp) * s.right.size_of(c);
-    };
-    std::map<int, std::vector<std::pair<Q, Q>>> tblocks;
+    // rbtm (site_hamil.hpp:149-204): MPSBoundaryProduct GEMMs (lazy single-use columns
+    // counted when used), task axpys of groups with an L block, charge_gemm products.
+    std::vector<std::vector<std::pair<int, int>>> tblocks(s.W.cols);  // (c, c') right ids
+    auto tgemm = [&](int c, int cp) { return 2.0 * s.X.blk[s.x_of_r[c]].rows * s.rsize(cp) * s.rsize(c); };
     for (int b2 = 0; b2 < s.W.cols; ++b2) {
-        for (auto &xb : s.X.blk) {
-            Q c = xb.c;
-            if (s.W.rskip(b2)) {
-                for (auto &x : R.b[s.W.rherm[b2]])
-                    if (x.rc == c && s.X.pos.count(x.lc)) tblocks[b2].push_back({c, x.lc});
-            } else {
-                for (auto &x : R.b[b2])
-                    if (x.lc == c && s.X.pos.count(x.rc)) tblocks[b2].push_back({c, x.rc});
-            }
+        bool skip = s.W.rskip(b2);
+        for (auto &x : R.b[skip ? s.W.rherm[b2] : b2]) {
+            int c = s.ri.get(skip ? x.rc : x.lc), cp = s.ri.get(skip ? x.lc : x.rc);
+            if (c < 0 || cp < 0 || s.x_of_r[c] < 0 || s.x_of_r[cp] < 0) continue;
+            tblocks[b2].push_back({c, cp});
         }
         if (s.W.col_nnz[b2] != 1)
-            for (auto &p : tblocks[b2]) f += tgemm(b2, p.first, p.second);
+            for (auto &p : tblocks[b2]) f += tgemm(p.first, p.second);
     }
-    std::set<int> lazy_used;
+    std::vector<char> lazy_used(s.W.cols, 0);
     for (int b1 = 0; b1 < s.W.rows; ++b1) {
         int a = s.W.lspin[b1];
-        // left_b1 blocks (bra, ket): transpose(L[b1]) or L[conj]
-        std::set<std::pair<Q, Q>> lblk;
-        if (s.W.lskip(b1)) {
-            for (auto &x : L.b[s.W.lherm[b1]]) lblk.insert({x.lc, x.rc});
-        } else {
-            for (auto &x : L.b[b1]) lblk.insert({x.rc, x.lc});
+        std::unordered_map<uint64_t, char> lblk;  // (bra, ket) left ids present in left_b1
+        bool skip = s.W.lskip(b1);
+        for (auto &x : L.b[skip ? s.W.lherm[b1] : b1]) {
+            int p = s.li.get(x.lc), q = s.li.get(x.rc);
+            if (p < 0 || q < 0) continue;
+            int bra = skip ? p : q, ket = skip ? q : p;
+            lblk[((uint64_t)bra << 32) | (uint32_t)ket] = 1;
         }
-        std::map<std::pair<Q, Q>, std::vector<std::pair<double, int>>> groups;  // (ol, or) -> (flops, b2)
+        std::unordered_map<uint64_t, std::vector<std::pair<double, int>>> groups;  // (or, ol)
         for (int b2 : s.W.row_b2[b1]) {
-            int ap = s.W.rspin[b2];
             for (auto &term : s.W.at.at({b1, b2})) {
-                const Op &op = s.W.ops[term.op];
-                for (auto &p : tblocks[b2]) {
-                    Q lc = p.first, rc = p.second;
-                    for (auto &wb : op.blocks) {
-                        Q ol = lc - wb.in;
-                        if (!s.left.has(ol)) continue;
-                        Q orr = rc - wb.out;
-                        if (s.su2 && !tri(spin2(ol), a, spin2(orr))) continue;
-                        if (!s.left.has(orr) || s.pbr.sz(orr) == 0) continue;
-                        (void)ap;
-                        double fl = 2.0 * s.left.size_of(ol) * s.right.size_of(rc) * wb.e.size();
-                        groups[{ol, orr}].push_back({fl, b2});
+                for (auto &pr : tblocks[b2]) {
+                    for (auto &ob : s.opb[term.op]) {
+                        int ol = s.rmp_l[(size_t)pr.first * s.np + ob.pin];
+                        if (ol < 0) continue;
+                        int orr = s.rmp_l[(size_t)pr.second * s.np + ob.pout];
+                        if (s.su2 && orr >= 0 && !tri(spin2(s.left.q[ol]), a, spin2(s.left.q[orr]))) continue;
+                        if (orr < 0 || s.pbr_sz_l[orr] == 0) continue;
+                        double fl = 2.0 * s.lsize(ol) * s.rsize(pr.second) * ob.b->e.size();
+                        groups[((uint64_t)orr << 32) | (uint32_t)ol].push_back({fl, b2});
                     }
                 }
             }
         }
         for (auto &g : groups) {
-            Q ol = g.first.first, orr = g.first.second;
-            if (!lblk.count({orr, ol})) continue;
+            int orr = (int)(g.first >> 32), ol = (int)(g.first & 0xffffffffu);
+            if (!lblk.count(((uint64_t)orr << 32) | (uint32_t)ol)) continue;
             for (auto &x : g.second) {
                 f += x.first;
-                if (s.W.col_nnz[x.second] == 1) lazy_used.insert(x.second);
+                if (s.W.col_nnz[x.second] == 1) lazy_used[x.second] = 1;
             }
-            f += 2.0 * s.left.size_of(orr) * s.pbr.sz(orr) * s.left.size_of(ol);
+            f += 2.0 * s.lsize(orr) * s.pbr_sz_l[orr] * s.lsize(ol);
         }
     }
-    for (int b2 : lazy_used)
-        for (auto &p : tblocks[b2]) f += tgemm(b2, p.first, p.second);
+    for (int b2 = 0; b2 < s.W.cols; ++b2)
+        if (lazy_used[b2])
+            for (auto &p : tblocks[b2]) f += tgemm(p.first, p.second);
     return f;
 }
 

@@ -202,6 +202,40 @@ def allowed_sectors(L, phys, right_end, Mmax, su2=True):
     return allowed
 
 
+def bounded_sectors(L, phys, right_end, m_total, su2=True):
+    """Allowed sectors with the total bond dimension of every bond capped at m_total.
+
+    Sector sizes are taken proportional to the number of states the sector
+    holds (its combinatorial degeneracy), each at least 1, so a random state
+    has the block structure of an m_total state rather than the per-sector
+    cap the reference's default initialiser applies."""
+    full = allowed_sectors(L, phys, right_end, 1 << 30, su2)
+    out = []
+    for ix in full:
+        tot = ix.sum_of_sizes()
+        if tot <= m_total:
+            out.append(ix)
+            continue
+        d = {}
+        for c, s in ix:
+            d[c] = max(1, int(round(s * m_total / tot)))
+        out.append(Index.from_dict(d))
+    return out
+
+
+def random_mps_on(sectors, phys, seed=42):
+    rng = np.random.default_rng(seed)
+    mps = []
+    for i in range(len(sectors) - 1):
+        lay, n = canonical_layout(phys, sectors[i], sectors[i + 1])
+        t = MPSTensor.from_flat(phys, sectors[i], sectors[i + 1], rng.uniform(0.0, 1.0, size=n))
+        nrm = t.norm()
+        if nrm > 0:
+            t.scale_(1.0 / nrm)
+        mps.append(t)
+    return mps
+
+
 def random_mps(L, phys, right_end, Mmax, seed=42, su2=True):
     """Random MPS over the allowed sectors, each site scaled to unit norm.
 
