@@ -12,10 +12,10 @@
  *     -> mq_sigma
  *   contraction::Engine<M,M,S>::overlap_mpo_left_step(bra, ket, L, W)
  *       non-abelian/engine.hpp:105, common/move_boundary.hpp:130-187
- *     -> mq_boundary_left_step
+ *     -> mq_lstep_plan_create + mq_boundary_step
  *   contraction::Engine<M,M,S>::overlap_mpo_right_step(bra, ket, R, W)
  *       non-abelian/engine.hpp:116, common/move_boundary.hpp:191-229
- *     -> mq_boundary_right_step
+ *     -> mq_rstep_plan_create + mq_boundary_step
  *   solve_ietl_jcd(SiteProblem&, MPSTensor const&, BaseParameters&)
  *       dmrg/framework/dmrg/optimize/ietl_jacobi_davidson.h:18-77 with
  *       ietl::jacobi_davidson::calculate_eigenvalue, dmrg/framework/ietl/jacobi.h:364-451
@@ -194,6 +194,15 @@ int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R,
                        int max_iter, double tol, int prec,
                        const double *const *ortho, int n_ortho,
                        int *iterations);
+
+/* -- diagnostics (no reference counterpart) ------------------------------ */
+/* C(M x N, ldc) = op(A) op(B) with the library's grouped FP64 GEMM kernel on
+ * device arrays (column-major; transA: A stored K x M, transB: B stored
+ * N x K), K cut into chunks of about kchunk (>= 16) when the output is small.
+ * Synchronous.  Used by the kernel tests. */
+int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
+                const double *A, int lda, const double *B, int ldb,
+                double *C, int ldc, int kchunk);
 
 /* -- host-side plan introspection (no GPU needed) ------------------------ */
 /* Builds the same task lists the device plans execute, on the host only, so

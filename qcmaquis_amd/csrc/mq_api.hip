@@ -290,6 +290,36 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
     }
 }
 
+// C = op(A) op(B) on device arrays with the grouped GEMM kernel (one group).
+int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const double *A, int lda,
+                const double *B, int ldb, double *C, int ldc, int kchunk) {
+    if (!ctx || M < 0 || N < 0 || K < 0 || kchunk < 16) return MQ_ERR_ARG;
+    try {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        GemmGroup g{};
+        g.M = M;
+        g.N = N;
+        g.K = K;
+        g.lda = lda;
+        g.ldb = ldb;
+        g.ldc = ldc;
+        g.transA = transA ? 1 : 0;
+        g.transB = transB ? 1 : 0;
+        GemmList gl;
+        tile_gemm(gl, g, 64, 64, kchunk);
+        DevGemm dg;
+        dg.upload(gl, ctx->stream);
+        DevArray<double> ws;
+        ws.alloc((size_t)gl.ws);
+        check(launch_gemm(dg.groups.p, dg.tiles.p, dg.ntiles, dg.split.p, dg.nsplit, A, B, C, ws.p, ctx->stream),
+              "gemm");
+        check(hipStreamSynchronize(ctx->stream), "gemm sync");
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(ctx, e, MQ_ERR_HIP);
+    }
+}
+
 // Forget the arranged boundaries (call after modifying L or R in place).
 int mq_plan_rebind(mq_plan *plan) {
     if (!plan) return MQ_ERR_ARG;

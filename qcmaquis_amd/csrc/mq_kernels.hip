@@ -41,11 +41,77 @@ __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips
     }
 }
 
-// ------------------------------------------------------------------ grouped GEMM (FP64)
-// Tile 64x64, K step 16, 256 threads; each thread owns a 4x4 register block
-// spread with stride 16 over the tile.  Operands staged through LDS with
-// zero padding at the edges, so any M, N, K (including K = 0) is handled.
+// ------------------------------------------------------------------ grouped GEMM (FP64, MFMA)
+// Block tile 64x64, K step 16, 256 threads = 4 waves in a 2x2 grid, each
+// wave owning a 32x32 sub-tile computed as 2x2 v_mfma_f64_16x16x4 products.
+// The MFMA is fed with (B^T, A^T) so that its output lanes run along the rows
+// of the column-major C, which makes the stores 128-byte segments.  Operand
+// layout of v_mfma_f64_16x16x4: the first operand holds X[lane % 16][lane / 16],
+// the second Y[lane / 16][lane % 16], and accumulator register r of a lane
+// holds D[4 * r + lane / 16][lane % 16]; with X = B^T, Y = A^T that element is
+// C(row lane % 16, column 4 * r + lane / 16).  Operands are staged through a
+// double-buffered LDS tile with zero padding at the edges, so any M, N, K
+// (including K = 0) is handled; the next K tile is fetched into registers
+// while the current one is multiplied.
 constexpr int TM = 64, TN = 64, TK = 16;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+struct GemmStage {
+    double a[4], b[4];
+};
+
+__device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const double *__restrict__ A,
+                                           const double *__restrict__ B, int m0, int n0, int k0, int kend,
+                                           GemmStage &st) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        int i, k;
+        if (!g.transA) {
+            i = e % TM;
+            k = e / TM;
+        } else {
+            k = e % TK;
+            i = e / TK;
+        }
+        const int gm = m0 + i, gk = k0 + k;
+        double v = 0.0;
+        if (gm < g.M && gk < kend) v = g.transA ? A[g.a + gk + (int64_t)gm * g.lda] : A[g.a + gm + (int64_t)gk * g.lda];
+        st.a[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        int j, k;
+        if (!g.transB) {
+            k = e % TK;
+            j = e / TK;
+        } else {
+            j = e % TN;
+            k = e / TN;
+        }
+        const int gn = n0 + j, gk = k0 + k;
+        double v = 0.0;
+        if (gn < g.N && gk < kend) v = g.transB ? B[g.b + gn + (int64_t)gk * g.ldb] : B[g.b + gk + (int64_t)gn * g.ldb];
+        st.b[r] = v;
+    }
+}
+
+__device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage &st, double (*As)[TM + 2],
+                                           double (*Bs)[TN + 2]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        if (!g.transA)
+            As[e / TM][e % TM] = st.a[r];
+        else
+            As[e % TK][e / TK] = st.a[r];
+        if (!g.transB)
+            Bs[e % TK][e / TK] = st.b[r];
+        else
+            Bs[e / TN][e % TN] = st.b[r];
+    }
+}
 
 __global__ void __launch_bounds__(256) k_gemm_f64(const GemmGroup *__restrict__ groups,
                                                   const GemmTile *__restrict__ tiles, const double *__restrict__ A,
@@ -56,79 +122,59 @@ __global__ void __launch_bounds__(256) k_gemm_f64(const GemmGroup *__restrict__ 
     const int m0 = t.mt * TM, n0 = t.nt * TN;
     const int kbeg = t.ks * g.kchunk;
     const int kend = min(g.K, kbeg + g.kchunk);
-    __shared__ double As[TK][TM + 1];
-    __shared__ double Bs[TK][TN + 1];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4];
+    __shared__ double As[2][TK][TM + 2];
+    __shared__ double Bs[2][TK][TN + 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    dbl4 acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        if (!g.transA) {
-            for (int e = threadIdx.x; e < TM * TK; e += 256) {
-                int i = e % TM, k = e / TM;
-                int gm = m0 + i, gk = k0 + k;
-                As[k][i] = (gm < g.M && gk < kend) ? A[g.a + gm + (int64_t)gk * g.lda] : 0.0;
-            }
-        } else {
-            for (int e = threadIdx.x; e < TM * TK; e += 256) {
-                int k = e % TK, i = e / TK;
-                int gm = m0 + i, gk = k0 + k;
-                As[k][i] = (gm < g.M && gk < kend) ? A[g.a + gk + (int64_t)gm * g.lda] : 0.0;
-            }
-        }
-        if (!g.transB) {
-            for (int e = threadIdx.x; e < TK * TN; e += 256) {
-                int k = e % TK, j = e / TK;
-                int gn = n0 + j, gk = k0 + k;
-                Bs[k][j] = (gn < g.N && gk < kend) ? B[g.b + gk + (int64_t)gn * g.ldb] : 0.0;
-            }
-        } else {
-            for (int e = threadIdx.x; e < TK * TN; e += 256) {
-                int j = e % TN, k = e / TN;
-                int gn = n0 + j, gk = k0 + k;
-                Bs[k][j] = (gn < g.N && gk < kend) ? B[g.b + gn + (int64_t)gk * g.ldb] : 0.0;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < TK; ++k) {
-            double a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[k][tx + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[k][ty + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-        }
-        __syncthreads();
+        for (int j = 0; j < 2; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    GemmStage st;
+    int buf = 0;
+    if (kbeg < kend) {
+        gemm_fetch(g, A, B, m0, n0, kbeg, kend, st);
+        gemm_stash(g, st, As[0], Bs[0]);
     }
+    __syncthreads();
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        const bool more = k0 + TK < kend;
+        if (more) gemm_fetch(g, A, B, m0, n0, k0 + TK, kend, st);
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 4) {
+            double a0 = As[buf][kk + lk][wm + lr], a1 = As[buf][kk + lk][wm + 16 + lr];
+            double b0 = Bs[buf][kk + lk][wn + lr], b1 = Bs[buf][kk + lk][wn + 16 + lr];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+        }
+        if (more) gemm_stash(g, st, As[buf ^ 1], Bs[buf ^ 1]);
+        __syncthreads();
+        buf ^= 1;
+    }
+    double *out;
+    int64_t ld;
     if (g.split == 1) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            int gn = n0 + ty + 16 * j;
-            if (gn >= g.N) continue;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                int gm = m0 + tx + 16 * i;
-                if (gm < g.M) C[g.c + gm + (int64_t)gn * g.ldc] = acc[i][j];
-            }
-        }
+        out = C + g.c;
+        ld = g.ldc;
     } else {
-        double *P = W + g.part + (int64_t)t.ks * g.M * g.N;
+        out = W + g.part + (int64_t)t.ks * g.M * g.N;
+        ld = g.M;
+    }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            int gn = n0 + ty + 16 * j;
-            if (gn >= g.N) continue;
+    for (int i = 0; i < 2; ++i) {
+        const int gm = m0 + wm + 16 * i + lr;
+        if (gm >= g.M) continue;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                int gm = m0 + tx + 16 * i;
-                if (gm < g.M) P[gm + (int64_t)gn * g.M] = acc[i][j];
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gn = n0 + wn + 16 * j + 4 * r + lk;
+                if (gn < g.N) out[gm + gn * ld] = acc[i][j][r];
             }
-        }
     }
 }
 

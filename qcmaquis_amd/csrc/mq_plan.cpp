@@ -395,6 +395,8 @@ void patch_strips(StripList &sl, const std::vector<int> &strip_al, const std::ve
 
 }  // namespace
 
+void tile_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kc) { add_gemm(gl, g, tm, tn, kc); }
+
 // ============================================================ sigma
 HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     Site s(d);

@@ -94,6 +94,9 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kch
 HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
 HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
 
+// One GEMM group split into tiles (and K chunks) as the plans do.
+void tile_gemm(GemmList &gl, GemmGroup g, int tile_m, int tile_n, int kchunk_target);
+
 // FLOPs of the reference schedule (site_hamil.hpp:28-204 for sigma, move_boundary.hpp for steps)
 double reference_sigma_flops(const mq_site_desc &d);
 

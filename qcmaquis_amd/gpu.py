@@ -49,7 +49,10 @@ def lib():
         L.mq_plan_exec_flops.argtypes = [vp]
         L.mq_plan_exec_flops.restype = C.c_double
         L.mq_sigma.argtypes = [vp, vp, vp, vp, vp, C.c_int]
+        L.mq_sigma_timed.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_float), f64p]
         L.mq_plan_rebind.argtypes = [vp]
+        L.mq_gemm_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
+                                  C.c_int, C.c_int]
         L.mq_step_out_desc.argtypes = [vp, C.POINTER(mq_boundary_desc)]
         L.mq_boundary_step.argtypes = [vp, vp, vp, vp]
         L.mq_jacobi_davidson.argtypes = [vp, vp, vp, vp, vp, f64p, C.c_int, C.c_double, C.c_int,
@@ -175,6 +178,23 @@ class GpuEngine:
     def to_dev(self, x):
         return self.torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64)).to(self.device)
 
+    def gemm(self, a, b, transA=False, transB=False, kchunk=2048):
+        """a @ b (device float64 tensors, M x K and K x N) with the library's
+        FP64 GEMM kernel; transA / transB select the kernel path that reads
+        the operand stored transposed."""
+        t = self.torch
+        M, K = a.shape
+        N = b.shape[1]
+        A = a.contiguous() if transA else a.t().contiguous()  # column-major K x M if transA else M x K
+        B = b.contiguous() if transB else b.t().contiguous()  # column-major N x K if transB else K x N
+        Cm = t.empty((N, M), dtype=t.float64, device=self.device)
+        t.cuda.synchronize()
+        lda = K if transA else M
+        ldb = N if transB else K
+        self.check(lib().mq_gemm_f64(self.ctx, int(transA), int(transB), M, N, K, _ptr(A), max(lda, 1), _ptr(B),
+                                     max(ldb, 1), _ptr(Cm), max(M, 1), kchunk), "mq_gemm_f64")
+        return Cm.t()
+
     def seed_boundary(self, charge):
         return GpuBoundary(BoundaryLayout.seed(charge), self.to_dev(np.ones(1)))
 
@@ -271,6 +291,19 @@ class SigmaPlan:
         self.eng.check(lib().mq_sigma(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y),
                                       self.eng.prec), "mq_sigma")
         return y
+
+    def timed(self, x, y):
+        """One sigma with HIP events between its stages on the library stream.
+
+        Returns (ms, work): ms = [boundary arrangement, psi reshape, T GEMM,
+        W application, output GEMM] in milliseconds; work = [T GEMM flops,
+        W-application flops, output GEMM flops, and the minimum HBM bytes of
+        the same three stages]."""
+        ms = (C.c_float * 5)()
+        work = (C.c_double * 6)()
+        self.eng.check(lib().mq_sigma_timed(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y),
+                                            self.eng.prec, ms, work), "mq_sigma_timed")
+        return list(ms), list(work)
 
     def jd(self, x0, max_iter=10, tol=1e-8, ortho=()):
         t = self.eng.torch

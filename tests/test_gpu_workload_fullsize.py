@@ -1,0 +1,82 @@
+"""The benchmark workload at full size on the GPU, against the CPU oracle.
+
+L=24, N=24, S=0 SU(2)xU(1) Hamiltonian (synthetic integrals, seed 1024), MPS
+with the bond sectors recorded from a DMRG ramp to m=512
+(tests/golden/sectors_L24_m512.json), random values (seed 42); the centre
+two-site problem (sites 11, 12) with boundaries built by the GPU steps.
+
+Bars: sigma <= 1e-12 relative to the oracle, identical algorithmic FLOP
+count; boundary steps at the centre <= 1e-12; hermiticity of H_eff to 1e-12;
+Jacobi-Davidson eigenvalue equal to the oracle's to 1e-10 relative."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def centre(gpu_engine):
+    from qcmaquis_amd.workload import centre_problem, load_sectors
+    _, sec = load_sectors(os.path.join(HERE, "golden", "sectors_L24_m512.json"))
+    print("building the L=24 m=512 centre problem", flush=True)
+    return centre_problem(gpu_engine, 24, sectors=sec)
+
+
+def _host(b):
+    from pyoracle import HostBoundary
+    return HostBoundary(b.layout, b.data.cpu().numpy())
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_fullsize_sigma_vs_oracle(centre, gpu_engine, oracle_engine):
+    p = centre
+    x = np.random.default_rng(7).standard_normal(p.psi.to_flat().size)
+    y1, f1 = gpu_engine.sigma(p.ham.symm, p.psi, p.L, p.R, p.W, x=x)
+    y0, f0 = oracle_engine.sigma(p.ham.symm, p.psi, _host(p.L), _host(p.R), p.W, x=x)
+    print(f"psi {x.size}, F {f0 / 1e9:.2f} GF, rel err {_rel(y1, y0):.2e}", flush=True)
+    assert _rel(y1, y0) <= 1e-12
+    assert f1 == f0
+
+
+def test_fullsize_hermiticity(centre, gpu_engine):
+    p = centre
+    rng = np.random.default_rng(11)
+    n = p.psi.to_flat().size
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    ha, _ = gpu_engine.sigma(p.ham.symm, p.psi, p.L, p.R, p.W, x=a)
+    hb, _ = gpu_engine.sigma(p.ham.symm, p.psi, p.L, p.R, p.W, x=b)
+    lhs, rhs = float(a @ hb), float(ha @ b)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(a) * np.linalg.norm(hb)
+
+
+def test_fullsize_boundary_steps_vs_oracle(centre, gpu_engine, oracle_engine):
+    from qcmaquis_amd.mps import make_twosite_mps, split_twosite
+    p = centre
+    ham, c = p.ham, p.site
+    # left step with the left-normalised site c of a split psi0, right step with site c + 1
+    A, B, _ = split_twosite(make_twosite_mps(p.mps[c], p.mps[c + 1]), ham.phys, ham.phys, p.mps[c].left,
+                            p.mps[c + 1].right, 512, 1e-16, l2r=True)
+    l1 = gpu_engine.left_step(ham.symm, A, p.L, ham.mpo[c])
+    l0 = oracle_engine.left_step(ham.symm, A, _host(p.L), ham.mpo[c])
+    assert l1.layout.total == l0.layout.total
+    assert _rel(l1.data.cpu().numpy(), l0.data) <= 1e-12
+    r1 = gpu_engine.right_step(ham.symm, p.mps[c + 1], p.R, ham.mpo[c + 1])
+    r0 = oracle_engine.right_step(ham.symm, p.mps[c + 1], _host(p.R), ham.mpo[c + 1])
+    assert r1.layout.total == r0.layout.total
+    assert _rel(r1.data.cpu().numpy(), r0.data) <= 1e-12
+
+
+def test_fullsize_jacobi_davidson_vs_oracle(centre, gpu_engine, oracle_engine):
+    p = centre
+    t1, u1, i1 = gpu_engine.jd(p.ham.symm, p.psi, p.L, p.R, p.W, 4, 1e-8)
+    t0, u0, i0 = oracle_engine.jd(p.ham.symm, p.psi, _host(p.L), _host(p.R), p.W, 4, 1e-8)
+    print(f"theta gpu {t1:.12f} oracle {t0:.12f} iterations {i1} {i0}", flush=True)
+    assert i1 == i0
+    assert abs(t1 - t0) <= 1e-10 * abs(t0)
