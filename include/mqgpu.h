@@ -138,6 +138,13 @@ typedef struct mq_plan mq_plan;
 int mq_ctx_create(int device, mq_ctx **out);
 int mq_ctx_destroy(mq_ctx *ctx);
 const char *mq_last_error(const mq_ctx *ctx);
+/* Plan-construction options of a context (no reference counterpart), applied
+ * to plans created afterwards.  MQ_OPT_FUSE: how MPO columns with many terms
+ * are evaluated -- 0 term by term, 1 (default) columns whose operator entries
+ * repeat at few positions are summed on the left boundary first, 2 every
+ * column is summed that way (testing aid).  Results agree to round-off. */
+enum mq_option { MQ_OPT_FUSE = 0 };
+int mq_ctx_set_option(mq_ctx *ctx, int option, int value);
 /* HIP stream (hipStream_t) the context launches on. */
 void *mq_ctx_stream(mq_ctx *ctx);
 int mq_ctx_synchronize(mq_ctx *ctx);
@@ -208,12 +215,14 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
 /* Builds the same task lists the device plans execute, on the host only, so
  * that schedules, FLOP counts and workspace sizes can be inspected (and the
  * task lists checked) on machines without a GPU.  kind: 0 sigma, 1 left
- * step, 2 right step.  Arrays stay owned by the handle. */
+ * step, 2 right step, plus MQ_HOST_FUSE(v) to build with MQ_OPT_FUSE = v
+ * (default 1).  Arrays stay owned by the handle. */
+#define MQ_HOST_FUSE(v) (((v) + 1) << 8)
 typedef struct mq_host_plan mq_host_plan;
 enum mq_plan_part {
     MQ_PART_ARR_L = 0, MQ_PART_ARR_R = 1, MQ_PART_RESHAPE = 2, MQ_PART_G1_GROUPS = 3,
     MQ_PART_G1_TILES = 4, MQ_PART_STRIPS = 5, MQ_PART_SRCS = 6, MQ_PART_G2_GROUPS = 7,
-    MQ_PART_G2_TILES = 8, MQ_PART_SCATTER = 9
+    MQ_PART_G2_TILES = 8, MQ_PART_SCATTER = 9, MQ_PART_COMB_STRIPS = 10, MQ_PART_COMB_SRCS = 11
 };
 int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out);
 int mq_host_plan_part(const mq_host_plan *hp, int part, const void **ptr, int64_t *count, int64_t *elem_bytes);

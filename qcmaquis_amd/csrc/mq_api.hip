@@ -14,6 +14,7 @@ struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    int fuse = 1;  // MQ_OPT_FUSE
 };
 
 namespace {
@@ -49,10 +50,11 @@ struct DevGemm {
     DevArray<GemmGroup> groups;
     DevArray<GemmTile> tiles;
     DevArray<int> split;
-    int ntiles = 0, nsplit = 0;
+    int ntiles = 0, n64 = 0, nsplit = 0;
     void upload(const GemmList &g, hipStream_t s) {
         groups.upload(g.groups, s);
         tiles.upload(g.tiles, s);
+        n64 = g.n64;
         std::vector<int> sp;
         for (size_t i = 0; i < g.groups.size(); ++i)
             if (g.groups[i].split > 1) sp.push_back((int)i);
@@ -72,11 +74,14 @@ struct mq_plan {
     mq_ctx *ctx = nullptr;
     HostPlan hp;
     DevArray<Copy2D> arrL, arrR, reshape, scatter;
-    DevArray<Strip> strips;
-    DevArray<Src> srcs;
+    DevArray<Strip> strips, cstrips;
+    DevArray<Src> srcs, csrcs;
     DevGemm g1, g2;
     DevArray<double> la, rb, prp, t, c, tmp, ws;
     const double *bound_L = nullptr, *bound_R = nullptr;
+    // mixed form: FP32 copies of the panels and FP32 intermediates (allocated on first use)
+    DevArray<float> la32, rb32, prp32, t32, c32, ws32;
+    bool conv32 = false;  // la32 / rb32 hold the current binding
     // Davidson workspace
     DevArray<double> jd_vecs, jd_partial, jd_coef;
     DevArray<const double *> jd_ptrs;
@@ -114,6 +119,16 @@ int mq_ctx_destroy(mq_ctx *ctx) {
 
 const char *mq_last_error(const mq_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int mq_ctx_set_option(mq_ctx *ctx, int option, int value) {
+    if (!ctx) return MQ_ERR_ARG;
+    if (option == MQ_OPT_FUSE && value >= 0 && value <= 2) {
+        ctx->fuse = value;
+        return MQ_OK;
+    }
+    ctx->err = "mq_ctx_set_option: unknown option or value";
+    return MQ_ERR_ARG;
+}
+
 void *mq_ctx_stream(mq_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 int mq_ctx_synchronize(mq_ctx *ctx) {
@@ -133,8 +148,8 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         std::unique_ptr<mq_plan> p(new mq_plan());
         p->ctx = ctx;
         const int tm = 64, tn = 64, kc = 2048;
-        if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc);
-        else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc);
+        if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse);
+        else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse);
         else p->hp = build_rstep_plan(*desc, tm, tn, kc);
         hipStream_t s = ctx->stream;
         HostPlan &h = p->hp;
@@ -144,6 +159,8 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         p->scatter.upload(h.scatter, s);
         p->strips.upload(h.wapply.strips, s);
         p->srcs.upload(h.wapply.srcs, s);
+        p->cstrips.upload(h.combL.strips, s);
+        p->csrcs.upload(h.combL.srcs, s);
         p->g1.upload(h.g1, s);
         p->g2.upload(h.g2, s);
         p->la.alloc(h.n_la);
@@ -196,23 +213,89 @@ int64_t mq_plan_workspace_bytes(const mq_plan *plan) {
 // Executed (not algorithmic) FLOPs of this implementation.
 double mq_plan_exec_flops(const mq_plan *plan) { return plan ? plan->hp.exec_flops : -1; }
 
-static void sigma_launch(mq_plan *p, const double *L, const double *R, const double *x, double *y) {
+// L boundary -> LA panels: copies of the stored blocks, then the fused rows.
+static void arrange_left(mq_plan *p, const double *L, hipStream_t s) {
+    HostPlan &h = p->hp;
+    check(launch_copy2d(p->arrL.p, (int)h.arrL.size(), L, p->la.p, s), "arrange L");
+    check(launch_wapply(p->cstrips.p, (int)h.combL.strips.size(), p->csrcs.p, p->la.p, p->la.p, ST_FUSE, s), "fuse L rows");
+}
+
+// (Re)bind the boundaries: arrange them into the FP64 panels when the pointer
+// pair changes.
+static void bind(mq_plan *p, const double *L, const double *R, hipStream_t s) {
+    if (L == p->bound_L && R == p->bound_R) return;
+    arrange_left(p, L, s);
+    check(launch_copy2d(p->arrR.p, (int)p->hp.arrR.size(), R, p->rb.p, s), "arrange R");
+    p->bound_L = L;
+    p->bound_R = R;
+    p->conv32 = false;
+}
+
+// FP32 panels of the current binding (mixed form)
+static void bind32(mq_plan *p, hipStream_t s) {
+    HostPlan &h = p->hp;
+    if (!p->t32.p && h.n_t) {
+        p->la32.alloc(h.n_la);
+        p->rb32.alloc(h.n_rb);
+        p->prp32.alloc(h.n_prp);
+        p->t32.alloc(h.n_t);
+        p->c32.alloc(h.n_c);
+        p->ws32.alloc(std::max(h.g1.ws, h.g2.ws));
+    }
+    if (p->conv32) return;
+    check(launch_to_f32(p->la.p, p->la32.p, h.n_la, s), "LA to FP32");
+    check(launch_to_f32(p->rb.p, p->rb32.p, h.n_rb, s), "RB to FP32");
+    p->conv32 = true;
+}
+
+// Stages of one sigma after binding; ev (optional) records events between them.
+static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEvent_t *ev) {
     hipStream_t s = p->ctx->stream;
     HostPlan &h = p->hp;
-    if (L != p->bound_L || R != p->bound_R) {
-        check(launch_copy2d(p->arrL.p, (int)h.arrL.size(), L, p->la.p, s), "arrange L");
-        check(launch_copy2d(p->arrR.p, (int)h.arrR.size(), R, p->rb.p, s), "arrange R");
-        p->bound_L = L;
-        p->bound_R = R;
+    auto mark = [&](int i) {
+        if (ev) check(hipEventRecord(ev[i], s), "record");
+    };
+    if (prec == MQ_PREC_FP64) {
+        check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s), "reshape");
+        mark(2);
+        check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la.p,
+                          p->prp.p, p->t.p, p->ws.p, ST_SIGMA, s),
+              "gemm T");
+        mark(3);
+        check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t.p, p->c.p, ST_SIGMA, s),
+              "W apply");
+        mark(4);
+        check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c.p,
+                          p->rb.p, y, p->ws.p, ST_SIGMA_OUT, s),
+              "gemm R");
+    } else {
+        check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp32.p, s), "reshape");
+        mark(2);
+        check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la32.p,
+                          p->prp32.p, p->t32.p, p->ws32.p, ST_SIGMA, s),
+              "gemm T");
+        mark(3);
+        check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t32.p, p->c32.p, ST_SIGMA, s),
+              "W apply");
+        mark(4);
+        check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c32.p,
+                          p->rb32.p, y, p->ws32.p, ST_SIGMA_OUT, s),
+              "gemm R");
     }
-    check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s), "reshape");
-    check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la.p, p->prp.p,
-                      p->t.p, p->ws.p, s),
-          "gemm T");
-    check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t.p, p->c.p, s), "W apply");
-    check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c.p, p->rb.p, y,
-                      p->ws.p, s),
-          "gemm R");
+    mark(5);
+}
+
+static void sigma_launch(mq_plan *p, const double *L, const double *R, const double *x, double *y, int prec) {
+    hipStream_t s = p->ctx->stream;
+    bind(p, L, R, s);
+    if (prec != MQ_PREC_FP64) bind32(p, s);
+    sigma_stages(p, x, y, prec, nullptr);
+}
+
+static const char *prec_error(int prec) {
+    if (prec == MQ_PREC_FP64 || prec == MQ_PREC_FP32) return nullptr;
+    if (prec == MQ_PREC_BF16X3) return "MQ_PREC_BF16X3 is not implemented";
+    return "unknown precision";
 }
 
 int mq_sigma(mq_plan *plan, const double *L, const double *R, const double *x, double *y, int prec) {
@@ -221,12 +304,12 @@ int mq_sigma(mq_plan *plan, const double *L, const double *R, const double *x, d
         plan->ctx->err = "mq_sigma: plan is not a sigma plan";
         return MQ_ERR_ARG;
     }
-    if (prec != MQ_PREC_FP64) {
-        plan->ctx->err = "mq_sigma: only MQ_PREC_FP64 is implemented";
+    if (const char *e = prec_error(prec)) {
+        plan->ctx->err = std::string("mq_sigma: ") + e;
         return MQ_ERR_ARG;
     }
     try {
-        sigma_launch(plan, L, R, x, y);
+        sigma_launch(plan, L, R, x, y, prec);
         return MQ_OK;
     } catch (std::exception &e) {
         return fail(plan->ctx, e, MQ_ERR_HIP);
@@ -239,37 +322,26 @@ int mq_sigma(mq_plan *plan, const double *L, const double *R, const double *x, d
 // FLOPs of stages 2..4 and the bytes each stage must move at minimum.
 int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double *x, double *y, int prec,
                    float ms[5], double work[6]) {
-    if (!plan || !x || !y || !ms || prec != MQ_PREC_FP64 || plan->hp.kind != PLAN_SIGMA) return MQ_ERR_ARG;
+    if (!plan || !x || !y || !ms || plan->hp.kind != PLAN_SIGMA) return MQ_ERR_ARG;
+    if (const char *e = prec_error(prec)) {
+        plan->ctx->err = std::string("mq_sigma_timed: ") + e;
+        return MQ_ERR_ARG;
+    }
     try {
         hipStream_t s = plan->ctx->stream;
         HostPlan &h = plan->hp;
         hipEvent_t ev[6];
         for (auto &e : ev) check(hipEventCreate(&e), "event");
         check(hipEventRecord(ev[0], s), "record");
-        if (L != plan->bound_L || R != plan->bound_R) {
-            check(launch_copy2d(plan->arrL.p, (int)h.arrL.size(), L, plan->la.p, s), "arrange L");
-            check(launch_copy2d(plan->arrR.p, (int)h.arrR.size(), R, plan->rb.p, s), "arrange R");
-            plan->bound_L = L;
-            plan->bound_R = R;
-        }
+        bind(plan, L, R, s);
+        if (prec != MQ_PREC_FP64) bind32(plan, s);
         check(hipEventRecord(ev[1], s), "record");
-        check(launch_copy2d(plan->reshape.p, (int)h.reshape.size(), x, plan->prp.p, s), "reshape");
-        check(hipEventRecord(ev[2], s), "record");
-        check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p, plan->g1.nsplit,
-                          plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, s),
-              "gemm T");
-        check(hipEventRecord(ev[3], s), "record");
-        check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p, s),
-              "W apply");
-        check(hipEventRecord(ev[4], s), "record");
-        check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p, plan->g2.nsplit,
-                          plan->c.p, plan->rb.p, y, plan->ws.p, s),
-              "gemm R");
-        check(hipEventRecord(ev[5], s), "record");
+        sigma_stages(plan, x, y, prec, ev);
         check(hipEventSynchronize(ev[5]), "sync");
         for (int i = 0; i < 5; ++i) check(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]), "elapsed");
         for (auto &e : ev) (void)hipEventDestroy(e);
         if (work) {
+            const double es = prec == MQ_PREC_FP64 ? 8.0 : 4.0;  // bytes per stored element
             double srcel = 0, dstel = 0;
             for (auto &st : h.wapply.strips) {
                 srcel += (double)st.rows * st.cols * (st.s1 - st.s0);
@@ -278,11 +350,12 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
             work[0] = h.g1.flops();
             work[1] = 2.0 * srcel;
             work[2] = h.g2.flops();
-            // minimum bytes: T GEMM reads LA + PRP, writes T; W application reads T once, writes C;
-            // output GEMM reads C + RB, writes y
-            work[3] = 8.0 * (h.n_la + h.n_prp + h.n_t);
-            work[4] = 8.0 * (h.n_t + dstel);
-            work[5] = 8.0 * (h.n_c + h.n_rb + h.vec_len);
+            // bytes each stage must move: T GEMM reads LA + PRP and writes T; the W
+            // application reads every source slice and writes C; the output GEMM
+            // reads C + RB and writes y (FP64)
+            work[3] = es * (h.n_la + h.n_prp + h.n_t);
+            work[4] = es * (srcel + dstel);
+            work[5] = es * (h.n_c + h.n_rb) + 8.0 * h.vec_len;
         }
         return MQ_OK;
     } catch (std::exception &e) {
@@ -307,11 +380,13 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const 
         g.transB = transB ? 1 : 0;
         GemmList gl;
         tile_gemm(gl, g, 64, 64, kchunk);
+        order_tiles(gl);
         DevGemm dg;
         dg.upload(gl, ctx->stream);
         DevArray<double> ws;
         ws.alloc((size_t)gl.ws);
-        check(launch_gemm(dg.groups.p, dg.tiles.p, dg.ntiles, dg.split.p, dg.nsplit, A, B, C, ws.p, ctx->stream),
+        check(launch_gemm(dg.groups.p, dg.tiles.p, dg.ntiles, dg.split.p, dg.nsplit, A, B, C, ws.p, ST_MISC,
+                          ctx->stream),
               "gemm");
         check(hipStreamSynchronize(ctx->stream), "gemm sync");
         return MQ_OK;
@@ -350,24 +425,26 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
     try {
         check(launch_copy2d(plan->reshape.p, (int)h.reshape.size(), A, plan->prp.p, s), "reshape");
         if (h.kind == PLAN_LSTEP) {
-            check(launch_copy2d(plan->arrL.p, (int)h.arrL.size(), in, plan->la.p, s), "arrange L");
+            arrange_left(plan, in, s);
             check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
-                              plan->g1.nsplit, plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, s),
+                              plan->g1.nsplit, plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, ST_STEP, s),
                   "gemm T");
-            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p, s),
+            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p,
+                                ST_STEP, s),
                   "W apply");
             check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
-                              plan->g2.nsplit, plan->c.p, A, plan->tmp.p, plan->ws.p, s),
+                              plan->g2.nsplit, plan->c.p, A, plan->tmp.p, plan->ws.p, ST_STEP_OUT, s),
                   "gemm out");
         } else {
             check(launch_copy2d(plan->arrR.p, (int)h.arrR.size(), in, plan->rb.p, s), "arrange R");
             check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
-                              plan->g1.nsplit, A, plan->rb.p, plan->t.p, plan->ws.p, s),
+                              plan->g1.nsplit, A, plan->rb.p, plan->t.p, plan->ws.p, ST_STEP, s),
                   "gemm t");
-            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p, s),
+            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p,
+                                ST_STEP, s),
                   "W apply");
             check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
-                              plan->g2.nsplit, plan->c.p, plan->prp.p, plan->tmp.p, plan->ws.p, s),
+                              plan->g2.nsplit, plan->c.p, plan->prp.p, plan->tmp.p, plan->ws.p, ST_STEP_OUT, s),
                   "gemm out");
         }
         check(launch_copy2d(plan->scatter.p, (int)h.scatter.size(), plan->tmp.p, out, s), "scatter");
@@ -470,8 +547,12 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
                                   int n_ortho, int *iterations) {
     if (!plan || !x0 || !u_out || !theta_out || max_iter < 1 || max_iter > 60 || n_ortho < 0 || n_ortho > 60)
         return MQ_ERR_ARG;
-    if (plan->hp.kind != PLAN_SIGMA || prec != MQ_PREC_FP64) {
-        plan->ctx->err = "mq_jacobi_davidson: needs an FP64 sigma plan";
+    if (plan->hp.kind != PLAN_SIGMA) {
+        plan->ctx->err = "mq_jacobi_davidson: needs a sigma plan";
+        return MQ_ERR_ARG;
+    }
+    if (const char *e = prec_error(prec)) {
+        plan->ctx->err = std::string("mq_jacobi_davidson: ") + e;
         return MQ_ERR_ARG;
     }
     try {
@@ -516,7 +597,7 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             project(t);
             double nt = std::sqrt(jd.dot(t, t));
             jd.lincomb({t}, {1.0 / nt - 1.0}, 1.0, t);
-            sigma_launch(plan, L, R, t, VA(it));
+            sigma_launch(plan, L, R, t, VA(it), prec);
             std::vector<const double *> xs, ys;
             for (int i = 0; i <= it; ++i) {
                 xs.push_back(V(i));
@@ -559,6 +640,13 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             std::vector<double> d2 = jd.dots({uA, u}, {u, u});
             jd.lincomb({uA, u}, {-1.0, d2[0] / d2[1]}, 0.0, V(it));
         }
+        if (prec != MQ_PREC_FP64) {
+            // FP64 defect correction: the eigenvalue is the FP64 Rayleigh quotient of the
+            // final Ritz vector (its error is second order in the vector's error)
+            sigma_launch(plan, L, R, u, uA, MQ_PREC_FP64);
+            std::vector<double> d = jd.dots({u, u}, {uA, u});
+            theta = d[0] / d[1];
+        }
         check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
         check(hipStreamSynchronize(jd.s), "jd end");
         *theta_out = theta;
@@ -585,9 +673,10 @@ int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out) 
     try {
         std::unique_ptr<mq_host_plan> h(new mq_host_plan());
         const int tm = 64, tn = 64, kc = 2048;
-        if (kind == PLAN_SIGMA) h->hp = build_sigma_plan(*desc, tm, tn, kc);
-        else if (kind == PLAN_LSTEP) h->hp = build_lstep_plan(*desc, tm, tn, kc);
-        else if (kind == PLAN_RSTEP) h->hp = build_rstep_plan(*desc, tm, tn, kc);
+        const int fm = (kind >> 8) & 3, fuse = fm ? fm - 1 : 1, k = kind & 255;
+        if (k == PLAN_SIGMA) h->hp = build_sigma_plan(*desc, tm, tn, kc, fuse);
+        else if (k == PLAN_LSTEP) h->hp = build_lstep_plan(*desc, tm, tn, kc, fuse);
+        else if (k == PLAN_RSTEP) h->hp = build_rstep_plan(*desc, tm, tn, kc);
         else return MQ_ERR_ARG;
         *out = h.release();
         return MQ_OK;
@@ -611,6 +700,8 @@ int mq_host_plan_part(const mq_host_plan *h, int part, const void **ptr, int64_t
         case MQ_PART_G2_GROUPS: *ptr = p.g2.groups.data(); *count = p.g2.groups.size(); *elem = sizeof(GemmGroup); break;
         case MQ_PART_G2_TILES: *ptr = p.g2.tiles.data(); *count = p.g2.tiles.size(); *elem = sizeof(GemmTile); break;
         case MQ_PART_SCATTER: *ptr = p.scatter.data(); *count = p.scatter.size(); *elem = sizeof(Copy2D); break;
+        case MQ_PART_COMB_STRIPS: *ptr = p.combL.strips.data(); *count = p.combL.strips.size(); *elem = sizeof(Strip); break;
+        case MQ_PART_COMB_SRCS: *ptr = p.combL.srcs.data(); *count = p.combL.srcs.size(); *elem = sizeof(Src); break;
         default: return MQ_ERR_ARG;
     }
     return MQ_OK;

@@ -1,4 +1,9 @@
 // HIP kernels of the hot path, gfx950 (CDNA4, wave64).
+//
+// Every kernel is instantiated for FP64 data (the plain form) and for FP32
+// data (the mixed form, whose output sigma is still accumulated into FP64).
+// The STAGE template parameter only names the instantiation, one per
+// pipeline stage, so that profiles attribute time per stage.
 #include <hip/hip_runtime.h>
 
 #include "mq_kernels.hpp"
@@ -6,63 +11,95 @@
 namespace mq {
 
 // ------------------------------------------------------------------ copy / scatter
-// One workgroup per task; element e -> (i = e % rows, j = e / rows).
-__global__ void __launch_bounds__(256) k_copy2d(const Copy2D *__restrict__ tasks, const double *__restrict__ src,
-                                                double *__restrict__ dst) {
+// One workgroup per task; element e -> (i = e % rows, j = e / rows).  The
+// destination may be of a narrower type (FP64 -> FP32 conversion).
+template <class TS, class TD>
+__global__ void __launch_bounds__(256) k_copy2d(const Copy2D *__restrict__ tasks, const TS *__restrict__ src,
+                                                TD *__restrict__ dst) {
     const Copy2D t = tasks[blockIdx.x];
     const int n = t.rows * t.cols;
     if (t.trans) {
         for (int e = threadIdx.x; e < n; e += blockDim.x) {
             int i = e % t.rows, j = e / t.rows;
-            dst[t.dst + i + (int64_t)j * t.ldd] = t.scale * src[t.src + j + (int64_t)i * t.lds];
+            dst[t.dst + i + (int64_t)j * t.ldd] = (TD)(t.scale * src[t.src + j + (int64_t)i * t.lds]);
         }
     } else {
         for (int e = threadIdx.x; e < n; e += blockDim.x) {
             int i = e % t.rows, j = e / t.rows;
-            dst[t.dst + i + (int64_t)j * t.ldd] = t.scale * src[t.src + i + (int64_t)j * t.lds];
+            dst[t.dst + i + (int64_t)j * t.ldd] = (TD)(t.scale * src[t.src + i + (int64_t)j * t.lds]);
         }
     }
 }
 
+// flat FP64 -> FP32 conversion of an arena
+__global__ void __launch_bounds__(256) k_to_f32(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = (float)src[e];
+}
+
 // ------------------------------------------------------------------ W application
 // dst strip = sum_s coef_s * src_s, summed in task order (deterministic, no atomics).
+template <int STAGE, class T>
 __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips, const Src *__restrict__ srcs,
-                                                const double *__restrict__ T, double *__restrict__ C) {
+                                                const T *__restrict__ src, T *__restrict__ C) {
     const Strip st = strips[blockIdx.x];
     const int n = st.rows * st.cols;
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         int i = e % st.rows, j = e / st.rows;
-        double acc = 0.0;
+        T acc = 0;
         for (int s = st.s0; s < st.s1; ++s) {
             const Src sr = srcs[s];
-            acc += sr.coef * T[sr.off + i + (int64_t)j * sr.ld];
+            acc += (T)sr.coef * src[sr.off + i + (int64_t)j * sr.ld];
         }
         C[st.dst + i + (int64_t)j * st.ldd] = acc;
     }
 }
 
-// ------------------------------------------------------------------ grouped GEMM (FP64, MFMA)
-// Block tile 64x64, K step 16, 256 threads = 4 waves in a 2x2 grid, each
-// wave owning a 32x32 sub-tile computed as 2x2 v_mfma_f64_16x16x4 products.
-// The MFMA is fed with (B^T, A^T) so that its output lanes run along the rows
-// of the column-major C, which makes the stores 128-byte segments.  Operand
-// layout of v_mfma_f64_16x16x4: the first operand holds X[lane % 16][lane / 16],
-// the second Y[lane / 16][lane % 16], and accumulator register r of a lane
-// holds D[4 * r + lane / 16][lane % 16]; with X = B^T, Y = A^T that element is
-// C(row lane % 16, column 4 * r + lane / 16).  Operands are staged through a
-// double-buffered LDS tile with zero padding at the edges, so any M, N, K
-// (including K = 0) is handled; the next K tile is fetched into registers
-// while the current one is multiplied.
-constexpr int TM = 64, TN = 64, TK = 16;
+// ------------------------------------------------------------------ grouped GEMM (MFMA)
+// Block tile 64 x TN (TN = 64 or 128 per group), K step 16, 256 threads =
+// 4 waves in a 2x2 grid, each wave owning a 32 x TN/2 sub-tile computed with
+// 16x16x4 MFMA products (v_mfma_f64_16x16x4 for FP64, v_mfma_f32_16x16x4 for
+// FP32 operands).  The MFMA is fed with (B^T, A^T) so that its output lanes
+// run along the rows of the column-major C, which makes the stores 128-byte
+// segments.  Operand layout of both instructions: the first operand holds
+// X[lane % 16][lane / 16], the second Y[lane / 16][lane % 16].  Accumulator
+// register r of a lane holds D[4 * r + lane / 16][lane % 16] for the FP64
+// instruction and D[4 * (lane / 16) + r][lane % 16] for the FP32 one; with
+// X = B^T, Y = A^T, D's row index is C's column and its column index C's row.
+// Operands are staged through a double-buffered LDS tile with zero padding at
+// the edges, so any M, N, K (including K = 0) is handled; the next K tile is
+// fetched into registers while the current one is multiplied.
+constexpr int TM = 64, TK = 16;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
-struct GemmStage {
-    double a[4], b[4];
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef dbl4 acc;
+    static __device__ __forceinline__ acc run(double x, double y, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int dcol(int r, int lq) { return 4 * r + lq; }
+};
+template <>
+struct Mfma<float> {
+    typedef flt4 acc;
+    static __device__ __forceinline__ acc run(float x, float y, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int dcol(int r, int lq) { return 4 * lq + r; }
 };
 
-__device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const double *__restrict__ A,
-                                           const double *__restrict__ B, int m0, int n0, int k0, int kend,
-                                           GemmStage &st) {
+template <int TN, class T>
+struct GemmStage {
+    T a[4], b[TN / 16];
+};
+
+template <int TN, class T>
+__device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restrict__ A, const T *__restrict__ B,
+                                           int m0, int n0, int k0, int kend, GemmStage<TN, T> &st) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int e = threadIdx.x + 256 * r;
@@ -75,12 +112,12 @@ __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const double *__r
             i = e / TK;
         }
         const int gm = m0 + i, gk = k0 + k;
-        double v = 0.0;
+        T v = 0;
         if (gm < g.M && gk < kend) v = g.transA ? A[g.a + gk + (int64_t)gm * g.lda] : A[g.a + gm + (int64_t)gk * g.lda];
         st.a[r] = v;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < TN / 16; ++r) {
         const int e = threadIdx.x + 256 * r;
         int j, k;
         if (!g.transB) {
@@ -91,14 +128,15 @@ __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const double *__r
             k = e / TN;
         }
         const int gn = n0 + j, gk = k0 + k;
-        double v = 0.0;
+        T v = 0;
         if (gn < g.N && gk < kend) v = g.transB ? B[g.b + gn + (int64_t)gk * g.ldb] : B[g.b + gk + (int64_t)gn * g.ldb];
         st.b[r] = v;
     }
 }
 
-__device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage &st, double (*As)[TM + 2],
-                                           double (*Bs)[TN + 2]) {
+template <int TN, class T>
+__device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<TN, T> &st, T (*As)[TM + 2],
+                                           T (*Bs)[TN + 2]) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int e = threadIdx.x + 256 * r;
@@ -106,6 +144,10 @@ __device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage &
             As[e / TM][e % TM] = st.a[r];
         else
             As[e % TK][e / TK] = st.a[r];
+    }
+#pragma unroll
+    for (int r = 0; r < TN / 16; ++r) {
+        const int e = threadIdx.x + 256 * r;
         if (!g.transB)
             Bs[e % TK][e / TK] = st.b[r];
         else
@@ -113,82 +155,112 @@ __device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage &
     }
 }
 
-__global__ void __launch_bounds__(256) k_gemm_f64(const GemmGroup *__restrict__ groups,
-                                                  const GemmTile *__restrict__ tiles, const double *__restrict__ A,
-                                                  const double *__restrict__ B, double *__restrict__ C,
-                                                  double *__restrict__ W) {
-    const GemmTile t = tiles[blockIdx.x];
-    const GemmGroup g = groups[t.g];
+// T: operand type; TO: output type; partial tiles of split-K groups are
+// stored in T and summed by k_split_reduce.
+template <int TN, class T, class TO>
+__device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t, const T *__restrict__ A,
+                                          const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
+                                          double *lds) {
+    typedef Mfma<T> MF;
+    constexpr int NJ = TN / 32;  // 16-wide MFMA columns per wave
+    T(*As)[TK][TM + 2] = reinterpret_cast<T(*)[TK][TM + 2]>(lds);
+    T(*Bs)[TK][TN + 2] = reinterpret_cast<T(*)[TK][TN + 2]>(reinterpret_cast<T *>(lds) + 2 * TK * (TM + 2));
     const int m0 = t.mt * TM, n0 = t.nt * TN;
     const int kbeg = t.ks * g.kchunk;
     const int kend = min(g.K, kbeg + g.kchunk);
-    __shared__ double As[2][TK][TM + 2];
-    __shared__ double Bs[2][TK][TN + 2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    const int lr = lane & 15, lk = lane >> 4;
-    dbl4 acc[2][2];
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * (TN / 2);
+    const int lr = lane & 15, lq = lane >> 4;
+    typename MF::acc acc[2][NJ];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
-    GemmStage st;
+        for (int j = 0; j < NJ; ++j) acc[i][j] = typename MF::acc{0, 0, 0, 0};
+    GemmStage<TN, T> st;
     int buf = 0;
     if (kbeg < kend) {
-        gemm_fetch(g, A, B, m0, n0, kbeg, kend, st);
-        gemm_stash(g, st, As[0], Bs[0]);
+        gemm_fetch<TN, T>(g, A, B, m0, n0, kbeg, kend, st);
+        gemm_stash<TN, T>(g, st, As[0], Bs[0]);
     }
     __syncthreads();
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
         const bool more = k0 + TK < kend;
-        if (more) gemm_fetch(g, A, B, m0, n0, k0 + TK, kend, st);
+        if (more) gemm_fetch<TN, T>(g, A, B, m0, n0, k0 + TK, kend, st);
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 4) {
-            double a0 = As[buf][kk + lk][wm + lr], a1 = As[buf][kk + lk][wm + 16 + lr];
-            double b0 = Bs[buf][kk + lk][wn + lr], b1 = Bs[buf][kk + lk][wn + 16 + lr];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+            T a0 = As[buf][kk + lq][wm + lr], a1 = As[buf][kk + lq][wm + 16 + lr];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                T bj = Bs[buf][kk + lq][wn + 16 * j + lr];
+                acc[0][j] = MF::run(bj, a0, acc[0][j]);
+                acc[1][j] = MF::run(bj, a1, acc[1][j]);
+            }
         }
-        if (more) gemm_stash(g, st, As[buf ^ 1], Bs[buf ^ 1]);
+        if (more) gemm_stash<TN, T>(g, st, As[buf ^ 1], Bs[buf ^ 1]);
         __syncthreads();
         buf ^= 1;
     }
-    double *out;
-    int64_t ld;
     if (g.split == 1) {
-        out = C + g.c;
-        ld = g.ldc;
+        TO *out = C + g.c;
+        const int64_t ld = g.ldc;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int gm = m0 + wm + 16 * i + lr;
+            if (gm >= g.M) continue;
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gn = n0 + wn + 16 * j + MF::dcol(r, lq);
+                    if (gn < g.N) out[gm + gn * ld] = (TO)acc[i][j][r];
+                }
+        }
     } else {
-        out = W + g.part + (int64_t)t.ks * g.M * g.N;
-        ld = g.M;
-    }
+        T *out = W + g.part + (int64_t)t.ks * g.M * g.N;
+        const int64_t ld = g.M;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int gm = m0 + wm + 16 * i + lr;
-        if (gm >= g.M) continue;
+        for (int i = 0; i < 2; ++i) {
+            const int gm = m0 + wm + 16 * i + lr;
+            if (gm >= g.M) continue;
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < NJ; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gn = n0 + wn + 16 * j + 4 * r + lk;
-                if (gn < g.N) out[gm + gn * ld] = acc[i][j][r];
-            }
+                for (int r = 0; r < 4; ++r) {
+                    const int gn = n0 + wn + 16 * j + MF::dcol(r, lq);
+                    if (gn < g.N) out[gm + gn * ld] = acc[i][j][r];
+                }
+        }
     }
 }
 
-// C = sum over splits of the partial tiles, in split order (deterministic).
+// One launch per stage: each tile runs the 64- or 128-wide body of its group.
+template <int STAGE, class T, class TO>
+__global__ void __launch_bounds__(256) k_gemm(const GemmGroup *__restrict__ groups, const GemmTile *__restrict__ tiles,
+                                              const T *__restrict__ A, const T *__restrict__ B, TO *__restrict__ C,
+                                              T *__restrict__ W) {
+    __shared__ double lds[2 * TK * (TM + 2) + 2 * TK * (128 + 2)];
+    const GemmTile t = tiles[blockIdx.x];
+    if (t.g < 0) return;  // padding tile of the XCD-ordered list
+    const GemmGroup g = groups[t.g];
+    if (g.tn == 128)
+        gemm_tile<128, T, TO>(g, t, A, B, C, W, lds);
+    else
+        gemm_tile<64, T, TO>(g, t, A, B, C, W, lds);
+}
+
+// C = sum over splits of the partial tiles, in split order (deterministic),
+// accumulated in FP64.
+template <class T, class TO>
 __global__ void __launch_bounds__(256) k_split_reduce(const GemmGroup *__restrict__ groups,
-                                                      const int *__restrict__ gids, const double *__restrict__ W,
-                                                      double *__restrict__ C) {
+                                                      const int *__restrict__ gids, const T *__restrict__ W,
+                                                      TO *__restrict__ C) {
     const GemmGroup g = groups[gids[blockIdx.y]];
     const int64_t n = (int64_t)g.M * g.N;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         double acc = 0.0;
-        for (int s = 0; s < g.split; ++s) acc += W[g.part + s * n + e];
+        for (int s = 0; s < g.split; ++s) acc += (double)W[g.part + s * n + e];
         int i = (int)(e % g.M), j = (int)(e / g.M);
-        C[g.c + i + (int64_t)j * g.ldc] = acc;
+        C[g.c + i + (int64_t)j * g.ldc] = (TO)acc;
     }
 }
 
@@ -226,20 +298,75 @@ __global__ void __launch_bounds__(256) k_lincomb(const double *const *__restrict
 // ------------------------------------------------------------------ launchers
 hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *dst, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_copy2d, dim3(n), dim3(256), 0, s, tasks, src, dst);
+    hipLaunchKernelGGL((k_copy2d<double, double>), dim3(n), dim3(256), 0, s, tasks, src, dst);
     return hipGetLastError();
 }
-hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const double *T, double *C, hipStream_t s) {
+hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_wapply, dim3(n), dim3(256), 0, s, strips, srcs, T, C);
+    hipLaunchKernelGGL((k_copy2d<double, float>), dim3(n), dim3(256), 0, s, tasks, src, dst);
     return hipGetLastError();
 }
-hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
-                       int nsplit, const double *A, const double *B, double *C, double *W, hipStream_t s) {
-    if (ntiles > 0) hipLaunchKernelGGL(k_gemm_f64, dim3(ntiles), dim3(256), 0, s, groups, tiles, A, B, C, W);
-    if (nsplit > 0) hipLaunchKernelGGL(k_split_reduce, dim3(64, nsplit), dim3(256), 0, s, groups, split_gids, W, C);
+hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
 }
+
+template <class T>
+static hipError_t wapply_launch(const Strip *strips, int n, const Src *srcs, const T *src, T *C, int stage,
+                                hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    switch (stage) {
+        case ST_SIGMA: hipLaunchKernelGGL((k_wapply<ST_SIGMA, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
+        case ST_FUSE: hipLaunchKernelGGL((k_wapply<ST_FUSE, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
+        default: hipLaunchKernelGGL((k_wapply<ST_STEP, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const double *T, double *C, int stage,
+                         hipStream_t s) {
+    return wapply_launch<double>(strips, n, srcs, T, C, stage, s);
+}
+hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const float *T, float *C, int stage,
+                         hipStream_t s) {
+    return wapply_launch<float>(strips, n, srcs, T, C, stage, s);
+}
+
+template <int STAGE, class T, class TO>
+static void gemm_stage(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const T *A, const T *B, TO *C,
+                       T *W, hipStream_t s) {
+    if (ntiles > 0) hipLaunchKernelGGL((k_gemm<STAGE, T, TO>), dim3(ntiles), dim3(256), 0, s, groups, tiles, A, B, C, W);
+}
+
+template <class T, class TO>
+static hipError_t gemm_launch(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
+                              int nsplit, const T *A, const T *B, TO *C, T *W, int stage, hipStream_t s) {
+    switch (stage) {
+        case ST_SIGMA: gemm_stage<ST_SIGMA, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
+        case ST_SIGMA_OUT: gemm_stage<ST_SIGMA_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
+        case ST_STEP: gemm_stage<ST_STEP, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
+        case ST_STEP_OUT: gemm_stage<ST_STEP_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
+        default: gemm_stage<ST_MISC, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
+    }
+    if (nsplit > 0)
+        hipLaunchKernelGGL((k_split_reduce<T, TO>), dim3(64, nsplit), dim3(256), 0, s, groups, split_gids, W, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
+                       const double *A, const double *B, double *C, double *W, int stage, hipStream_t s) {
+    return gemm_launch<double, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, stage, s);
+}
+hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
+                       const float *A, const float *B, float *C, float *W, int stage, hipStream_t s) {
+    return gemm_launch<float, float>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, stage, s);
+}
+hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
+                       const float *A, const float *B, double *C, float *W, int stage, hipStream_t s) {
+    return gemm_launch<float, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, stage, s);
+}
+
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,
                            double *partial, hipStream_t s) {
     hipLaunchKernelGGL(k_multidot, dim3(nblocks, npairs), dim3(256), 0, s, xs, ys, n, partial);

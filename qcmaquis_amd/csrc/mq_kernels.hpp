@@ -9,10 +9,24 @@
 
 namespace mq {
 
+// Pipeline stage of a launch: each stage gets its own kernel instantiation so
+// that profiles attribute time per stage.
+enum Stage { ST_SIGMA = 0, ST_SIGMA_OUT = 1, ST_STEP = 2, ST_STEP_OUT = 3, ST_FUSE = 4, ST_MISC = 5 };
+
 hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *dst, hipStream_t s);
-hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const double *T, double *C, hipStream_t s);
+hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s);
+hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s);
+hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const double *T, double *C, int stage,
+                         hipStream_t s);
+hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const float *T, float *C, int stage,
+                         hipStream_t s);
+// grouped GEMM over a plan's tile list (FP64; FP32 operands with FP32 or FP64 output)
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
-                       const double *A, const double *B, double *C, double *W, hipStream_t s);
+                       const double *A, const double *B, double *C, double *W, int stage, hipStream_t s);
+hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
+                       const float *A, const float *B, float *C, float *W, int stage, hipStream_t s);
+hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
+                       const float *A, const float *B, double *C, float *W, int stage, hipStream_t s);
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,
                            double *partial, hipStream_t s);
 hipError_t launch_lincomb(const double *const *vs, const double *coef, int nv, int64_t n, double beta, double *out,

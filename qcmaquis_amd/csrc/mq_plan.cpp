@@ -113,10 +113,12 @@ struct Site {
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
+    (void)tn;
     g.split = 1;
     g.kchunk = std::max(g.K, 1);
+    g.tn = g.N > 64 ? 128 : 64;
     if (g.M == 0 || g.N == 0) return;
-    int mt = ceil_div(g.M, tm), nt = ceil_div(g.N, tn);
+    int mt = ceil_div(g.M, tm), nt = ceil_div(g.N, g.tn);
     // split K when the output is small and K long: chunks of about kchunk_target
     if (g.K > 2 * kchunk_target && mt * nt < 64) {
         int s = std::min(ceil_div(g.K, kchunk_target), 64);
@@ -132,8 +134,36 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     int gi = (int)gl.groups.size();
     gl.groups.push_back(g);
     for (int ks = 0; ks < g.split; ++ks)
-        for (int n = 0; n < nt; ++n)
-            for (int m = 0; m < mt; ++m) gl.tiles.push_back({gi, m, n, ks});
+        for (int m = 0; m < mt; ++m)
+            for (int n = 0; n < nt; ++n) gl.tiles.push_back({gi, m, n, ks});
+}
+
+constexpr int kXcds = 8;
+
+std::vector<GemmTile> xcd_order(const std::vector<GemmTile> &tiles) {
+    // chains: consecutive tiles with the same (group, K chunk, row block)
+    std::vector<std::pair<size_t, size_t>> chains;
+    for (size_t i = 0; i < tiles.size();) {
+        size_t j = i + 1;
+        while (j < tiles.size() && tiles[j].g == tiles[i].g && tiles[j].ks == tiles[i].ks && tiles[j].mt == tiles[i].mt)
+            ++j;
+        chains.push_back({i, j});
+        i = j;
+    }
+    std::vector<std::vector<GemmTile>> q(kXcds);
+    for (auto &c : chains) {
+        size_t best = 0;
+        for (size_t x = 1; x < q.size(); ++x)
+            if (q[x].size() < q[best].size()) best = x;
+        q[best].insert(q[best].end(), tiles.begin() + c.first, tiles.begin() + c.second);
+    }
+    size_t len = 0;
+    for (auto &v : q) len = std::max(len, v.size());
+    std::vector<GemmTile> out;
+    out.reserve(len * kXcds);
+    for (size_t i = 0; i < len; ++i)
+        for (int x = 0; x < kXcds; ++x) out.push_back(i < q[x].size() ? q[x][i] : GemmTile{-1, 0, 0, 0});
+    return out;
 }
 
 Copy2D mk_copy(int64_t dst, int64_t src, int rows, int cols, int ldd, int lds, bool trans, double scale) {
@@ -179,6 +209,121 @@ PRP prp_layout(const Site &s, std::vector<Copy2D> &tasks) {
     return p;
 }
 
+// ---------------- column program: how each MPO column reaches T ------------
+// Column b2 of the W application sums, over its terms (b1, op, scale) and the
+// sparse entries e of op, coef * T[b1](:, slice of the input state e.row)
+// into the output state e.col (apply_op.hpp:23-101).  The coupling factor of
+// an entry depends on b1 only through a = spin(b1) (set_coupling arguments,
+// apply_op.hpp:90), so all entries of one column that share a position
+//     P = (input block, output block, e.row, e.col, row spin, col spin, 2S(op), a)
+// can be summed on the left boundary first:
+//     sum_b1 w_b1 T[b1](slice) = (sum_b1 w_b1 L[b1])^T psi(slice),  w = e.coeff * scale.
+// Quantum-chemistry MPOs have a few columns (complementary operators) with
+// hundreds of terms but only a few hundred distinct positions; those columns
+// are "fused": each position becomes a fused group whose boundary rows are
+// combined once per boundary binding, and whose T slice is one small GEMM
+// per sigma.  The other columns keep their terms ("direct").
+struct CTerm {
+    int row;  // MPO row b1
+    int op;
+    double scale;
+    int spin;  // 2S of the left MPO bond (a)
+};
+struct FBlk {  // one (lb, lk) block of a fused group
+    int lb, lk, ar, al;
+    int64_t lf;  // LA-arena offset of its first combined row (column 0)
+    int64_t tf;  // T-arena offset of its first T row (column 0)
+    int ld;      // rows of the fused panel (leading dimension in both arenas)
+};
+struct FGroup {
+    int b2, pin, pout, row, col, rspin, cspin, k, a;
+    std::vector<std::pair<int, double>> mem;  // (b1, weight)
+    std::vector<FBlk> blk;
+};
+struct ColProg {
+    std::vector<std::vector<CTerm>> col;  // [b2] direct terms
+    std::vector<std::vector<int>> fcol;   // [b2] fused group ids
+    std::vector<FGroup> fg;
+    std::vector<char> direct, member;     // [b1] row has direct terms / feeds a fused group
+};
+
+// a column is fused when its entries are at least kFuseRatio times its
+// distinct positions and number at least kFuseMinEntries
+constexpr int kFuseRatio = 4;
+constexpr int kFuseMinEntries = 64;
+
+// fuse: 0 never, 1 by the thresholds above, 2 every non-empty column (used by tests
+// to exercise fused groups on small systems)
+ColProg column_program(const Site &s, bool skip_rskip_cols, int fuse) {
+    ColProg cp;
+    cp.col.resize(s.W.cols);
+    cp.fcol.resize(s.W.cols);
+    cp.direct.assign(s.W.rows, 0);
+    cp.member.assign(s.W.rows, 0);
+    struct E {
+        uint64_t key;
+        int b1;
+        double w;
+    };
+    std::vector<E> ents;
+    for (int b2 = 0; b2 < s.W.cols; ++b2) {
+        if (skip_rskip_cols && s.W.rskip(b2)) continue;
+        ents.clear();
+        for (int b1 : s.W.col_b1[b2])
+            for (auto &term : s.W.at.at({b1, b2})) {
+                const Op &op = s.W.ops[term.op];
+                for (auto &ob : s.opb[term.op])
+                    for (auto &e : ob.b->e) {
+                        if (ob.pin > 255 || ob.pout > 255 || e.row > 255 || e.col > 255 || e.rspin > 3 ||
+                            e.cspin > 3 || op.spin > 127 || s.W.lspin[b1] > 127)
+                            throw std::runtime_error("operator entry out of the fused-key range");
+                        uint64_t key = ((uint64_t)ob.pin << 56) | ((uint64_t)ob.pout << 48) |
+                                       ((uint64_t)e.row << 40) | ((uint64_t)e.col << 32) |
+                                       ((uint64_t)e.rspin << 30) | ((uint64_t)e.cspin << 28) |
+                                       ((uint64_t)op.spin << 21) | ((uint64_t)s.W.lspin[b1] << 14);
+                        ents.push_back({key, b1, e.coeff * term.scale});
+                    }
+            }
+        std::stable_sort(ents.begin(), ents.end(), [](const E &x, const E &y) { return x.key < y.key; });
+        size_t ngroups = 0;
+        for (size_t i = 0; i < ents.size(); ++i)
+            if (i == 0 || ents[i].key != ents[i - 1].key) ++ngroups;
+        bool fused = (fuse == 2 && !ents.empty()) ||
+                     (fuse == 1 && ents.size() >= (size_t)kFuseMinEntries && ents.size() >= kFuseRatio * ngroups);
+        if (!fused) {
+            for (int b1 : s.W.col_b1[b2]) {
+                for (auto &term : s.W.at.at({b1, b2})) cp.col[b2].push_back({b1, term.op, term.scale, s.W.lspin[b1]});
+                cp.direct[b1] = 1;
+            }
+            continue;
+        }
+        size_t i = 0;
+        while (i < ents.size()) {
+            size_t j = i;
+            while (j < ents.size() && ents[j].key == ents[i].key) ++j;
+            uint64_t k = ents[i].key;
+            FGroup g;
+            g.b2 = b2;
+            g.pin = (int)(k >> 56);
+            g.pout = (int)((k >> 48) & 255);
+            g.row = (int)((k >> 40) & 255);
+            g.col = (int)((k >> 32) & 255);
+            g.rspin = (int)((k >> 30) & 3);
+            g.cspin = (int)((k >> 28) & 3);
+            g.k = (int)((k >> 21) & 127);
+            g.a = (int)((k >> 14) & 127);
+            for (size_t q = i; q < j; ++q) {
+                g.mem.push_back({ents[q].b1, ents[q].w});
+                cp.member[ents[q].b1] = 1;
+            }
+            cp.fcol[b2].push_back((int)cp.fg.size());
+            cp.fg.push_back(std::move(g));
+            i = j;
+        }
+    }
+    return cp;
+}
+
 // ---------------- T = L^T psi --------------------------------------------
 struct TB {  // one T block (b1, lb, lk)
     int lb, lk;
@@ -186,59 +331,85 @@ struct TB {  // one T block (b1, lb, lk)
     int M;        // leading dimension (rows of T_lk)
 };
 struct TLay {
-    std::vector<int> M;            // [lk] rows of LA_lk / T_lk
+    std::vector<int> M, Mtot;      // [lk] rows of T_lk / rows of the arranged panel LA_lk
     std::vector<int64_t> la_off, t_off;
     std::vector<std::vector<TB>> tb;  // [b1]
     int64_t la_total = 0, t_total = 0;
     double gemm_flops = 0;
 };
 
-// BoundaryMPSProduct A-operand blocks (boundary_times_mps.hpp:188-209).  used_only:
-// rows of the MPO with no entry are skipped (the reference computes them anyway;
-// the FLOP count of the reference schedule uses used_only = false).
-TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, std::vector<Copy2D> *arr, GemmList *g1, int tm,
-              int tn, int kc, bool used_only) {
-    TLay t;
-    t.M.assign(s.nl, 0);
-    t.tb.resize(s.W.rows);
+// BoundaryMPSProduct A-operand blocks (boundary_times_mps.hpp:188-209).
+// With cp == nullptr every MPO row gets T rows (the reference's schedule, used
+// for its FLOP count).  Otherwise:
+//   LA arena = [LA_lk panels: direct rows, then staged rows | fused panels]
+//   T arena  = [T_lk panels (direct rows)                   | fused T panels]
+// A fused panel, keyed (lk, input block, input state), stacks the (group, lb)
+// rows of every fused group at that input; its T GEMM reads only the psi
+// columns of that input state (one right charge ar).  The combination strips
+// (comb) build the fused panels from the arranged rows.
+TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std::vector<Copy2D> *arr,
+              StripList *comb, GemmList *g1, int tm, int tn, int kc) {
     struct A {
-        int b1, lb, lk;
+        int lb, lk;
         const BBlk *blk;
         bool trans;
         double scale;
         int row;
     };
-    std::vector<A> as;
-    for (int b1 = 0; b1 < s.W.rows; ++b1) {
-        if (used_only && s.W.row_nnz[b1] == 0) continue;
+    const int nb1 = s.W.rows;
+    TLay t;
+    t.M.assign(s.nl, 0);
+    t.Mtot.assign(s.nl, 0);
+    t.tb.resize(nb1);
+    std::vector<std::vector<A>> blk(nb1);
+    std::vector<char> real(nb1, 0);
+    for (int b1 = 0; b1 < nb1; ++b1) {
+        real[b1] = cp ? (cp->direct[b1] || cp->member[b1]) : 1;
+        if (!real[b1]) continue;
         bool skip = s.W.lskip(b1);
-        const auto &bl = L.b[skip ? s.W.lherm[b1] : b1];
-        for (auto &x : bl) {
+        for (auto &x : L.b[skip ? s.W.lherm[b1] : b1]) {
             int p = s.li.get(x.lc), q = s.li.get(x.rc);
             if (p < 0 || q < 0 || prp.off[p] < 0 || prp.off[q] < 0) continue;
             int lb = skip ? p : q, lk = skip ? q : p;
             double sc = (skip && s.su2) ? conj_correction(x.lc, x.rc, s.W.lspin[b1]) * s.W.lphase[b1] : 1.0;
             if (x.rows * x.cols != s.lsize(lb) * s.lsize(lk)) throw std::runtime_error("boundary block size mismatch");
-            as.push_back({b1, lb, lk, &x, !skip, sc, t.M[lk]});
-            t.M[lk] += s.lsize(lb);
+            blk[b1].push_back({lb, lk, &x, !skip, sc, -1});
         }
     }
+    for (int b1 = 0; b1 < nb1; ++b1)
+        if (!cp || cp->direct[b1])
+            for (auto &a : blk[b1]) {
+                a.row = t.M[a.lk];
+                t.M[a.lk] += s.lsize(a.lb);
+            }
+    t.Mtot = t.M;
+    for (int b1 = 0; b1 < nb1; ++b1)
+        if (cp && !cp->direct[b1] && cp->member[b1])
+            for (auto &a : blk[b1]) {
+                a.row = t.Mtot[a.lk];
+                t.Mtot[a.lk] += s.lsize(a.lb);
+            }
     t.la_off.assign(s.nl, -1);
     t.t_off.assign(s.nl, -1);
     for (int lk = 0; lk < s.nl; ++lk) {
-        if (t.M[lk] == 0) continue;
+        if (t.Mtot[lk] == 0) continue;
         t.la_off[lk] = t.la_total;
-        t.la_total += (int64_t)t.M[lk] * s.lsize(lk);
+        t.la_total += (int64_t)t.Mtot[lk] * s.lsize(lk);
+        if (t.M[lk] == 0) continue;
         t.t_off[lk] = t.t_total;
         t.t_total += (int64_t)t.M[lk] * s.pbr_sz_l[lk];
         t.gemm_flops += 2.0 * t.M[lk] * s.pbr_sz_l[lk] * s.lsize(lk);
     }
-    for (auto &a : as) {
-        t.tb[a.b1].push_back({a.lb, a.lk, t.t_off[a.lk] + a.row, t.M[a.lk]});
-        if (arr) {
-            int lbs = s.lsize(a.lb), lks = s.lsize(a.lk);
-            arr->push_back(mk_copy(t.la_off[a.lk] + a.row, a.blk->off, lbs, lks, t.M[a.lk], a.trans ? lks : lbs,
-                                   a.trans, a.scale));
+    for (int b1 = 0; b1 < nb1; ++b1) {
+        if (!real[b1]) continue;
+        bool in_t = !cp || cp->direct[b1];
+        for (auto &a : blk[b1]) {
+            if (in_t) t.tb[b1].push_back({a.lb, a.lk, t.t_off[a.lk] + a.row, t.M[a.lk]});
+            if (arr) {
+                int lbs = s.lsize(a.lb), lks = s.lsize(a.lk);
+                arr->push_back(mk_copy(t.la_off[a.lk] + a.row, a.blk->off, lbs, lks, t.Mtot[a.lk],
+                                       a.trans ? lks : lbs, a.trans, a.scale));
+            }
         }
     }
     if (g1)
@@ -251,11 +422,125 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, std::vector<Copy2
             g.M = t.M[lk];
             g.N = s.pbr_sz_l[lk];
             g.K = s.lsize(lk);
-            g.lda = g.M;
+            g.lda = t.Mtot[lk];
             g.ldb = g.K;
             g.ldc = g.M;
             add_gemm(*g1, g, tm, tn, kc);
         }
+    if (!cp || cp->fg.empty()) return t;
+
+    // ---- fused groups: blocks, panels, combination strips, T GEMMs
+    std::vector<std::unordered_map<uint64_t, int>> pos_of(nb1);  // (lb, lk) -> index into blk[b1]
+    for (int b1 = 0; b1 < nb1; ++b1)
+        if (cp->member[b1])
+            for (size_t i = 0; i < blk[b1].size(); ++i)
+                pos_of[b1][((uint64_t)blk[b1][i].lb << 32) | (uint32_t)blk[b1][i].lk] = (int)i;
+    struct Panel {
+        int lk, pin, row, ar, rows = 0;
+        int64_t lf = 0, tf = 0;
+    };
+    std::map<std::tuple<int, int, int>, int> panel_id;
+    std::vector<Panel> panels;
+    std::vector<std::vector<std::pair<int, int>>> fb_panel(cp->fg.size());  // per group block: (panel, row)
+    std::vector<std::pair<int, int>> v;
+    for (size_t gi = 0; gi < cp->fg.size(); ++gi) {
+        FGroup &g = cp->fg[gi];
+        const int ap = s.W.rspin[g.b2];
+        v.clear();
+        for (auto &m : g.mem)
+            for (auto &a : blk[m.first]) v.push_back({a.lk, a.lb});
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        for (auto &kb : v) {
+            int lk = kb.first, lb = kb.second;
+            int ar = s.lp_r[(size_t)lk * s.np + g.pin];
+            if (ar < 0 || s.pbr_sz_l[lk] == 0) continue;
+            int al = s.lp_r[(size_t)lb * s.np + g.pout];
+            if (al < 0 || s.x_of_r[al] < 0) continue;  // out_left_i
+            if (s.su2 && !tri(spin2(s.right.q[ar]), ap, spin2(s.right.q[al]))) continue;
+            auto key = std::make_tuple(lk, g.pin, g.row);
+            auto it = panel_id.find(key);
+            int pid;
+            if (it == panel_id.end()) {
+                pid = (int)panels.size();
+                panel_id.emplace(key, pid);
+                Panel pn;
+                pn.lk = lk;
+                pn.pin = g.pin;
+                pn.row = g.row;
+                pn.ar = ar;
+                panels.push_back(pn);
+            } else {
+                pid = it->second;
+            }
+            g.blk.push_back({lb, lk, ar, al, 0, 0, 0});
+            fb_panel[gi].push_back({pid, panels[pid].rows});
+            panels[pid].rows += s.lsize(lb);
+        }
+    }
+    for (auto &pn : panels) {
+        pn.lf = t.la_total;
+        t.la_total += (int64_t)pn.rows * s.lsize(pn.lk);
+        pn.tf = t.t_total;
+        t.t_total += (int64_t)pn.rows * s.rsize(pn.ar);
+    }
+    std::vector<uint64_t> comb_key;  // (lk, lb): strips reading one LA_lk row range run together
+    for (size_t gi = 0; gi < cp->fg.size(); ++gi) {
+        FGroup &g = cp->fg[gi];
+        for (size_t k = 0; k < g.blk.size(); ++k) {
+            FBlk &fb = g.blk[k];
+            const Panel &pn = panels[fb_panel[gi][k].first];
+            int row = fb_panel[gi][k].second;
+            fb.lf = pn.lf + row;
+            fb.tf = pn.tf + row;
+            fb.ld = pn.rows;
+            if (!comb) continue;
+            Strip st;
+            st.dst = fb.lf;
+            st.rows = s.lsize(fb.lb);
+            st.cols = s.lsize(fb.lk);
+            st.ldd = pn.rows;
+            st.s0 = (int)comb->srcs.size();
+            uint64_t bk = ((uint64_t)fb.lb << 32) | (uint32_t)fb.lk;
+            for (auto &m : g.mem) {
+                auto it = pos_of[m.first].find(bk);
+                if (it == pos_of[m.first].end()) continue;
+                const A &a = blk[m.first][it->second];
+                comb->srcs.push_back({t.la_off[a.lk] + a.row, t.Mtot[a.lk], m.second});
+            }
+            st.s1 = (int)comb->srcs.size();
+            comb->strips.push_back(st);
+            comb_key.push_back(bk);
+        }
+    }
+    if (comb) {
+        if (comb_key.size() != comb->strips.size()) throw std::runtime_error("combination strip list out of step");
+        std::vector<size_t> ord(comb->strips.size());
+        for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+            uint64_t kx = (comb_key[x] << 32) | (comb_key[x] >> 32), ky = (comb_key[y] << 32) | (comb_key[y] >> 32);
+            return kx < ky;
+        });
+        std::vector<Strip> sorted(ord.size());
+        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = comb->strips[ord[i]];
+        comb->strips.swap(sorted);
+    }
+    for (auto &pn : panels) {
+        int rsz = s.rsize(pn.ar), ksz = s.lsize(pn.lk);
+        t.gemm_flops += 2.0 * pn.rows * rsz * ksz;
+        if (!g1) continue;
+        GemmGroup g{};
+        g.a = pn.lf;
+        g.b = prp.off[pn.lk] + (int64_t)(s.pbr[(size_t)pn.pin * s.nr + pn.ar] + pn.row * rsz) * ksz;
+        g.c = pn.tf;
+        g.M = pn.rows;
+        g.N = rsz;
+        g.K = ksz;
+        g.lda = pn.rows;
+        g.ldb = ksz;
+        g.ldc = pn.rows;
+        add_gemm(*g1, g, tm, tn, kc);
+    }
     return t;
 }
 
@@ -268,52 +553,67 @@ struct Raw {
     double coef;
 };
 
-// lbtm W application for column b2 (apply_op.hpp:23-101; abelian apply_op.hpp:24-139).
-// Adds the reference's axpy FLOPs to *axpy.
-void lbtm_column(const Site &s, const TLay &t, int b2, std::vector<Raw> &raw, double *axpy) {
+// Visits every (term, T block, operator block) of column b2 that passes the
+// charge and spin-triangle checks of the lbtm kernel (apply_op.hpp:70-77;
+// abelian apply_op.hpp:24-139): f(term, tb, ob, ar, al).
+template <class F>
+void for_column_blocks(const Site &s, const TLay &t, const std::vector<CTerm> &terms, int b2, F &&f) {
+    int ap = s.W.rspin[b2];
+    for (auto &term : terms) {
+        const auto &tbs = t.tb[term.row];
+        for (auto &tb : tbs)
+            for (auto &ob : s.opb[term.op]) {
+                int ar = s.lp_r[(size_t)tb.lk * s.np + ob.pin];
+                if (ar < 0) continue;
+                int al = s.lp_r[(size_t)tb.lb * s.np + ob.pout];
+                if (al < 0 || s.x_of_r[al] < 0) continue;  // out_left_i
+                if (s.su2 && !tri(spin2(s.right.q[ar]), ap, spin2(s.right.q[al]))) continue;
+                f(term, tb, ob, ar, al);
+            }
+    }
+}
+
+// lbtm W application for column b2 (apply_op.hpp:23-101; abelian apply_op.hpp:24-139):
+// raw sources of the direct terms and of the fused groups, sorted by strip.
+void lbtm_column(const Site &s, const TLay &t, const ColProg *cp, const std::vector<CTerm> &terms, int b2,
+                 std::vector<Raw> &raw) {
     raw.clear();
     int ap = s.W.rspin[b2];
-    for (int b1 : s.W.col_b1[b2]) {
-        const auto &tbs = t.tb[b1];
-        if (tbs.empty()) continue;
-        int a = s.W.lspin[b1];
-        for (auto &term : s.W.at.at({b1, b2})) {
-            const Op &op = s.W.ops[term.op];
-            const auto &obs = s.opb[term.op];
-            for (auto &tb : tbs) {
-                int i = spin2(s.left.q[tb.lb]), j = spin2(s.left.q[tb.lk]);
-                int lsz = s.lsize(tb.lb);
-                for (auto &ob : obs) {
-                    const OpBlk &wb = *ob.b;
-                    int ar = s.lp_r[(size_t)tb.lk * s.np + ob.pin];
-                    if (ar < 0) continue;
-                    int al = s.lp_r[(size_t)tb.lb * s.np + ob.pout];
-                    if (al < 0 || s.x_of_r[al] < 0) continue;  // out_left_i
-                    int jp = spin2(s.right.q[ar]), ip = spin2(s.right.q[al]);
-                    if (s.su2 && !tri(jp, ap, ip)) continue;
-                    int rsz = s.rsize(ar);
-                    static const std::array<double, 4> one{1.0, 1.0, 1.0, 1.0};
-                    const std::array<double, 4> &cpl =
-                        s.su2 ? coupling(j, std::abs(j - jp), jp, a, op.spin, ap, i, std::abs(i - ip), ip) : one;
-                    int iro = s.pbr[(size_t)ob.pin * s.nr + ar];
-                    int olo = s.pbl[(size_t)ob.pout * s.nl + tb.lb];
-                    for (auto &e : wb.e) {
-                        double coef = e.coeff * term.scale * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : 1.0);
-                        raw.push_back({al, ar, olo + e.col * lsz, tb.off + (int64_t)(iro + e.row * rsz) * tb.M, tb.M,
-                                       coef});
-                        if (axpy) *axpy += 2.0 * lsz * rsz;
-                    }
-                }
+    static const std::array<double, 4> one{1.0, 1.0, 1.0, 1.0};
+    for_column_blocks(s, t, terms, b2, [&](const CTerm &term, const TB &tb, const Site::OpB &ob, int ar, int al) {
+        const Op &op = s.W.ops[term.op];
+        int i = spin2(s.left.q[tb.lb]), j = spin2(s.left.q[tb.lk]);
+        int jp = spin2(s.right.q[ar]), ip = spin2(s.right.q[al]);
+        int lsz = s.lsize(tb.lb), rsz = s.rsize(ar);
+        const std::array<double, 4> &cpl =
+            s.su2 ? coupling(j, std::abs(j - jp), jp, term.spin, op.spin, ap, i, std::abs(i - ip), ip) : one;
+        int iro = s.pbr[(size_t)ob.pin * s.nr + ar];
+        int olo = s.pbl[(size_t)ob.pout * s.nl + tb.lb];
+        for (auto &e : ob.b->e) {
+            double coef = e.coeff * term.scale * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : 1.0);
+            raw.push_back({al, ar, olo + e.col * lsz, tb.off + (int64_t)(iro + e.row * rsz) * tb.M, tb.M, coef});
+        }
+    });
+    if (cp)
+        for (int gi : cp->fcol[b2]) {
+            const FGroup &g = cp->fg[gi];
+            for (auto &fb : g.blk) {
+                int i = spin2(s.left.q[fb.lb]), j = spin2(s.left.q[fb.lk]);
+                int jp = spin2(s.right.q[fb.ar]), ip = spin2(s.right.q[fb.al]);
+                const std::array<double, 4> &cpl =
+                    s.su2 ? coupling(j, std::abs(j - jp), jp, g.a, g.k, ap, i, std::abs(i - ip), ip) : one;
+                double coef = s.su2 ? cpl[case_of(g.rspin, g.cspin)] : 1.0;
+                int row0 = s.pbl[(size_t)g.pout * s.nl + fb.lb] + g.col * s.lsize(fb.lb);
+                raw.push_back({fb.al, fb.ar, row0, fb.tf, fb.ld, coef});
             }
         }
-    }
     std::sort(raw.begin(), raw.end(), [](const Raw &x, const Raw &y) {
         if (x.al != y.al) return x.al < y.al;
         if (x.ar != y.ar) return x.ar < y.ar;
         if (x.row0 != y.row0) return x.row0 < y.row0;
         return x.off < y.off;
     });
-    // merge identical T slices of one strip (several operators of one (b1,b2) pair)
+    // merge identical T slices of one strip (several operators of one (row, b2) pair)
     size_t w = 0;
     for (size_t r = 0; r < raw.size(); ++r) {
         if (w > 0 && raw[w - 1].al == raw[r].al && raw[w - 1].ar == raw[r].ar && raw[w - 1].row0 == raw[r].row0 &&
@@ -323,6 +623,24 @@ void lbtm_column(const Site &s, const TLay &t, int b2, std::vector<Raw> &raw, do
             raw[w++] = raw[r];
     }
     raw.resize(w);
+}
+
+// Column terms of the reference (every MPO term on its own row).
+std::vector<CTerm> reference_terms(const Site &s, int b2) {
+    std::vector<CTerm> v;
+    for (int b1 : s.W.col_b1[b2])
+        for (auto &term : s.W.at.at({b1, b2})) v.push_back({b1, term.op, term.scale, s.W.lspin[b1]});
+    return v;
+}
+
+// axpy FLOPs of the reference's lbtm kernel for column b2 (2 len per sparse entry)
+double reference_axpy_flops(const Site &s, const TLay &tall, int b2) {
+    double f = 0;
+    for_column_blocks(s, tall, reference_terms(s, b2), b2,
+                      [&](const CTerm &, const TB &tb, const Site::OpB &ob, int ar, int) {
+                          f += 2.0 * s.lsize(tb.lb) * s.rsize(ar) * ob.b->e.size();
+                      });
+    return f;
 }
 
 // R operand of gemm_trim for (b2, ar, al): R[b2](ar,al) or phase*R[conj](al,ar)^T
@@ -397,8 +715,20 @@ void patch_strips(StripList &sl, const std::vector<int> &strip_al, const std::ve
 
 void tile_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kc) { add_gemm(gl, g, tm, tn, kc); }
 
+void order_tiles(GemmList &gl) {
+    std::vector<GemmTile> t64, t128;
+    for (auto &t : gl.tiles) {
+        if (t.g < 0) continue;
+        (gl.groups[t.g].tn == 128 ? t128 : t64).push_back(t);
+    }
+    auto a = xcd_order(t64), b = xcd_order(t128);
+    gl.n64 = (int)a.size();
+    gl.tiles = a;
+    gl.tiles.insert(gl.tiles.end(), b.begin(), b.end());
+}
+
 // ============================================================ sigma
-HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc) {
+HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, int fuse) {
     Site s(d);
     BLayout L = to_layout(d.L), R = to_layout(d.R);
     if (L.aux != s.W.rows || R.aux != s.W.cols) throw std::runtime_error("boundary aux_dim does not match the MPO");
@@ -408,7 +738,8 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.vec_len = s.X.total;
     PRP prp = prp_layout(s, P.reshape);
     P.n_prp = prp.total;
-    TLay t = t_layout(s, L, prp, &P.arrL, &P.g1, tm, tn, kc, true);
+    ColProg cp = column_program(s, false, fuse);
+    TLay t = t_layout(s, L, prp, &cp, &P.arrL, &P.combL, &P.g1, tm, tn, kc);
     P.n_la = t.la_total;
     P.n_t = t.t_total;
     std::vector<int> K(s.nr, 0);
@@ -417,9 +748,12 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     std::vector<Raw> raw;
     // reference schedule (site_hamil.hpp:35): when it is lbtm its FLOPs are counted in this pass
     const bool ref_lbtm = s.symm != MQ_SYMM_SU2U1 || (s.W.rows - s.W.one_rows) < (s.W.cols - s.W.one_cols);
+    TLay tall;
+    if (ref_lbtm) tall = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, tm, tn, kc);
     double ref_axpy = 0, ref_trim = 0;
     for (int b2 = 0; b2 < s.W.cols; ++b2) {
-        lbtm_column(s, t, b2, raw, &ref_axpy);
+        lbtm_column(s, t, &cp, cp.col[b2], b2, raw);
+        if (ref_lbtm) ref_axpy += reference_axpy_flops(s, tall, b2);
         size_t i = 0;
         while (i < raw.size()) {
             size_t j = i;
@@ -470,17 +804,18 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
     if (ref_lbtm) {
-        TLay tall = t_layout(s, L, prp, nullptr, nullptr, tm, tn, kc, false);
         P.ref_flops = tall.gemm_flops + ref_axpy + ref_trim;
     } else {
         P.ref_flops = reference_sigma_flops(d);
     }
     P.min_bytes = 8.0 * (L.total + R.total + 2.0 * s.X.total);
+    order_tiles(P.g1);
+    order_tiles(P.g2);
     return P;
 }
 
 // ============================================================ left step
-HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
+HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, int fuse) {
     Site s(d);
     BLayout L = to_layout(d.L);
     if (L.aux != s.W.rows) throw std::runtime_error("left boundary aux_dim does not match the MPO");
@@ -490,7 +825,9 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.vec_len = s.X.total;
     PRP prp = prp_layout(s, P.reshape);
     P.n_prp = prp.total;
-    TLay t = t_layout(s, L, prp, &P.arrL, &P.g1, tm, tn, kc, true);
+    ColProg cp = column_program(s, true, fuse);
+    TLay t = t_layout(s, L, prp, &cp, &P.arrL, &P.combL, &P.g1, tm, tn, kc);
+    TLay tall = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, tm, tn, kc);
     P.n_la = t.la_total;
     P.n_t = t.t_total;
     double axf = 0;
@@ -500,7 +837,8 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     std::vector<Raw> raw;
     for (int b2 = 0; b2 < s.W.cols; ++b2) {
         if (s.W.rskip(b2)) continue;
-        lbtm_column(s, t, b2, raw, &axf);
+        lbtm_column(s, t, &cp, cp.col[b2], b2, raw);
+        axf += reference_axpy_flops(s, tall, b2);
         size_t i = 0;
         while (i < raw.size()) {
             size_t j = i;
@@ -577,9 +915,11 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.n_out = off;
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
-    TLay tall = t_layout(s, L, prp, nullptr, nullptr, tm, tn, kc, false);
+    for (auto &st : P.combL.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
     P.ref_flops = tall.gemm_flops + axf + gf;
     P.min_bytes = 8.0 * (L.total + off + s.X.total);
+    order_tiles(P.g1);
+    order_tiles(P.g2);
     return P;
 }
 
@@ -848,6 +1188,8 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
     P.ref_flops = P.g1.flops() + axf + gf;
     P.min_bytes = 8.0 * (R.total + off + s.X.total);
+    order_tiles(P.g1);
+    order_tiles(P.g2);
     return P;
 }
 
@@ -860,11 +1202,13 @@ double reference_sigma_flops(const mq_site_desc &d) {
     bool lbtm = s.symm != MQ_SYMM_SU2U1 || (s.W.rows - s.W.one_rows) < (s.W.cols - s.W.one_cols);
     double f = 0;
     if (lbtm) {
-        TLay t = t_layout(s, L, prp, nullptr, nullptr, 64, 64, 1 << 30, false);
+        TLay t = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, 64, 64, 1 << 30);
         f += t.gemm_flops;
         std::vector<Raw> raw;
         for (int b2 = 0; b2 < s.W.cols; ++b2) {
-            lbtm_column(s, t, b2, raw, &f);
+            std::vector<CTerm> terms = reference_terms(s, b2);
+            lbtm_column(s, t, nullptr, terms, b2, raw);
+            f += reference_axpy_flops(s, t, b2);
             size_t i = 0;
             while (i < raw.size()) {
                 size_t j = i;

@@ -18,7 +18,8 @@ struct Copy2D {
 
 // C(M x N) = op(A)(M x K) * op(B)(K x N), column-major with leading dims.
 // transA: A stored K x M; transB: B stored N x K.  split > 1: K is cut into
-// `split` chunks whose partial tiles go to the workspace at `part`.
+// `split` chunks whose partial tiles go to the workspace at `part`.  tn: tile
+// width (64 or 128) of the group's tiles; tiles are 64 rows high.
 struct GemmGroup {
     int64_t a, b, c, part;
     int M, N, K;
@@ -26,7 +27,9 @@ struct GemmGroup {
     int transA, transB;
     int split;
     int kchunk;
+    int tn;
 };
+// one output tile; g < 0 marks a padding tile that does nothing
 struct GemmTile {
     int g;
     int mt, nt, ks;
@@ -48,9 +51,15 @@ struct Src {
     double coef;
 };
 
+// Tiles are ordered for the XCDs: workgroup i of a launch runs on XCD i % 8,
+// and the tiles of one (group, K chunk, row block) -- which share their A
+// panel -- are placed on one XCD, consecutively, so the panel is read from
+// HBM once and then served by that XCD's L2.  tiles[0, n64) use 64-wide
+// tiles, tiles[n64, end) 128-wide ones (two launches).
 struct GemmList {
     std::vector<GemmGroup> groups;
     std::vector<GemmTile> tiles;
+    int n64 = 0;
     int64_t ws = 0;  // workspace elements for split partials
     double flops() const {
         double f = 0;
@@ -74,6 +83,7 @@ struct HostPlan {
     int64_t n_la = 0, n_rb = 0, n_prp = 0, n_t = 0, n_c = 0, n_tmp = 0, n_out = 0;
     // stage task lists
     std::vector<Copy2D> arrL;   // L boundary -> LA  (sigma, lstep)
+    StripList combL;            // fused rows of LA: weighted sums of arranged rows (sigma, lstep)
     std::vector<Copy2D> arrR;   // R boundary -> RB  (sigma) / RC (rstep)
     std::vector<Copy2D> reshape;  // x -> PRP
     GemmList g1;                // T = LA * PRP    (sigma, lstep) ; t = X * RC (rstep)
@@ -90,12 +100,15 @@ struct HostPlan {
     double exec_flops = 0;
 };
 
-HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
-HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
+// fuse: 0 no fused MPO columns, 1 fuse the high fan-in columns, 2 fuse every column
+HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, int fuse = 1);
+HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, int fuse = 1);
 HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
 
 // One GEMM group split into tiles (and K chunks) as the plans do.
 void tile_gemm(GemmList &gl, GemmGroup g, int tile_m, int tile_n, int kchunk_target);
+// XCD-aware tile order (call once every group has been added).
+void order_tiles(GemmList &gl);
 
 // FLOPs of the reference schedule (site_hamil.hpp:28-204 for sigma, move_boundary.hpp for steps)
 double reference_sigma_flops(const mq_site_desc &d);

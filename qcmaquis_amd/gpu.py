@@ -38,6 +38,7 @@ def lib():
         L.mq_ctx_stream.argtypes = [vp]
         L.mq_ctx_stream.restype = vp
         L.mq_ctx_synchronize.argtypes = [vp]
+        L.mq_ctx_set_option.argtypes = [vp, C.c_int, C.c_int]
         for f in ("mq_sigma_plan_create", "mq_lstep_plan_create", "mq_rstep_plan_create"):
             getattr(L, f).argtypes = [vp, vp, C.POINTER(vp)]
         L.mq_plan_destroy.argtypes = [vp]
@@ -73,24 +74,24 @@ COPY2D = np.dtype([("dst", "<i8"), ("src", "<i8"), ("rows", "<i4"), ("cols", "<i
                    ("lds", "<i4"), ("trans", "<i4"), ("scale", "<f8")], align=True)
 GEMMGROUP = np.dtype([("a", "<i8"), ("b", "<i8"), ("c", "<i8"), ("part", "<i8"), ("M", "<i4"), ("N", "<i4"),
                       ("K", "<i4"), ("lda", "<i4"), ("ldb", "<i4"), ("ldc", "<i4"), ("transA", "<i4"),
-                      ("transB", "<i4"), ("split", "<i4"), ("kchunk", "<i4")], align=True)
+                      ("transB", "<i4"), ("split", "<i4"), ("kchunk", "<i4"), ("tn", "<i4")], align=True)
 GEMMTILE = np.dtype([("g", "<i4"), ("mt", "<i4"), ("nt", "<i4"), ("ks", "<i4")], align=True)
 STRIP = np.dtype([("dst", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("ldd", "<i4"), ("s0", "<i4"),
                   ("s1", "<i4")], align=True)
 SRC = np.dtype([("off", "<i8"), ("ld", "<i4"), ("coef", "<f8")], align=True)
-PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D]
+PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D, STRIP, SRC]
 PART_NAMES = ["arrL", "arrR", "reshape", "g1_groups", "g1_tiles", "strips", "srcs", "g2_groups", "g2_tiles",
-              "scatter"]
+              "scatter", "comb_strips", "comb_srcs"]
 KIND = {"sigma": 0, "lstep": 1, "rstep": 2}
 
 
 class HostPlan:
     """Task lists of a plan, built on the host (no GPU needed)."""
 
-    def __init__(self, site: SiteDesc, kind: str):
+    def __init__(self, site: SiteDesc, kind: str, fuse=1):
         L = lib()
         h = C.c_void_p()
-        if L.mq_host_plan_create(site.ptr(), KIND[kind], C.byref(h)) != 0:
+        if L.mq_host_plan_create(site.ptr(), KIND[kind] | ((fuse + 1) << 8), C.byref(h)) != 0:
             raise MQError(L.mq_host_last_error().decode())
         try:
             self.parts = {}
@@ -140,7 +141,7 @@ class GpuEngine:
 
     name = "gpu"
 
-    def __init__(self, device=0, prec=MQ_PREC_FP64):
+    def __init__(self, device=0, prec=MQ_PREC_FP64, fuse=1):
         import torch
         if not torch.cuda.is_available():
             raise MQError("no GPU visible: the hot path has no CPU fallback")
@@ -150,6 +151,7 @@ class GpuEngine:
         self.ctx = C.c_void_p()
         if lib().mq_ctx_create(device, C.byref(self.ctx)) != 0:
             raise MQError("mq_ctx_create failed")
+        self.check(lib().mq_ctx_set_option(self.ctx, 0, fuse), "mq_ctx_set_option")
         self.prec = prec
         self.flops = 0.0
         self.stats = []
@@ -284,15 +286,16 @@ class SigmaPlan:
         except Exception:
             pass
 
-    def apply(self, x, y=None):
+    def apply(self, x, y=None, prec=None):
         t = self.eng.torch
         if y is None:
             y = t.empty(self.n, dtype=t.float64, device=self.eng.device)
-        self.eng.check(lib().mq_sigma(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y),
-                                      self.eng.prec), "mq_sigma")
+        prec = self.eng.prec if prec is None else prec
+        self.eng.check(lib().mq_sigma(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y), prec),
+                       "mq_sigma")
         return y
 
-    def timed(self, x, y):
+    def timed(self, x, y, prec=None):
         """One sigma with HIP events between its stages on the library stream.
 
         Returns (ms, work): ms = [boundary arrangement, psi reshape, T GEMM,
@@ -301,12 +304,14 @@ class SigmaPlan:
         the same three stages]."""
         ms = (C.c_float * 5)()
         work = (C.c_double * 6)()
-        self.eng.check(lib().mq_sigma_timed(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y),
-                                            self.eng.prec, ms, work), "mq_sigma_timed")
+        prec = self.eng.prec if prec is None else prec
+        self.eng.check(lib().mq_sigma_timed(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y), prec,
+                                            ms, work), "mq_sigma_timed")
         return list(ms), list(work)
 
-    def jd(self, x0, max_iter=10, tol=1e-8, ortho=()):
+    def jd(self, x0, max_iter=10, tol=1e-8, ortho=(), prec=None):
         t = self.eng.torch
+        prec = self.eng.prec if prec is None else prec
         u = t.empty(self.n, dtype=t.float64, device=self.eng.device)
         th = C.c_double()
         its = C.c_int()
@@ -314,6 +319,6 @@ class SigmaPlan:
         arr = (C.c_void_p * max(1, len(od)))(*[o.data_ptr() for o in od])
         t.cuda.synchronize()
         self.eng.check(lib().mq_jacobi_davidson(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x0), _ptr(u),
-                                                C.byref(th), max_iter, tol, self.eng.prec, arr, len(od),
+                                                C.byref(th), max_iter, tol, prec, arr, len(od),
                                                 C.byref(its)), "mq_jacobi_davidson")
         return th.value, u, its.value

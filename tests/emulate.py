@@ -38,15 +38,20 @@ def wapply(strips, srcs, T, Cbuf):
         _mat(Cbuf, int(st["dst"]), r, c, int(st["ldd"]))[...] = acc
 
 
-def gemm(groups, tiles, A, B, C, tm=64, tn=64):
-    # tile coverage: every (m, n, k-split) tile exactly once
+def gemm(groups, tiles, A, B, C, tm=64):
+    # tile coverage: every (m, n, k-split) tile exactly once (g < 0: padding tiles)
     seen = {}
     for t in tiles:
+        if t["g"] < 0:
+            continue
         key = (int(t["g"]), int(t["mt"]), int(t["nt"]), int(t["ks"]))
         assert key not in seen
         seen[key] = 1
+    assert len(seen) == sum(1 for t in tiles if t["g"] >= 0)
     for gi, g in enumerate(groups):
         M, N, K = int(g["M"]), int(g["N"]), int(g["K"])
+        tn = int(g["tn"])
+        assert tn in (64, 128)
         mt, nt = -(-M // tm), -(-N // tn)
         for ks in range(int(g["split"])):
             for n in range(nt):
@@ -71,6 +76,7 @@ def run_sigma(hp, Ldata, Rdata, x):
     T, Cb = np.zeros(max(s["t"], 1)), np.zeros(max(s["c"], 1))
     y = np.full(s["vec"], np.nan)
     copy2d(P["arrL"], Ldata, la)
+    wapply(P["comb_strips"], P["comb_srcs"], la, la)
     copy2d(P["arrR"], Rdata, rb)
     copy2d(P["reshape"], x, prp)
     gemm(P["g1_groups"], P["g1_tiles"], la, prp, T)
@@ -88,6 +94,7 @@ def run_step(hp, kind, Bdata, A):
     copy2d(P["reshape"], A, prp)
     if kind == "lstep":
         copy2d(P["arrL"], Bdata, la)
+        wapply(P["comb_strips"], P["comb_srcs"], la, la)
         gemm(P["g1_groups"], P["g1_tiles"], la, prp, T)
         wapply(P["strips"], P["srcs"], T, Cb)
         gemm(P["g2_groups"], P["g2_tiles"], Cb, A, tmp)

@@ -5,6 +5,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(1, 1, 1), (16, 16, 4), (64, 64, 16), (65, 63, 17), (5, 130, 3), (200, 7, 1000), (37, 41, 0),
+          (64, 128, 32), (70, 129, 20), (300, 65, 7),
           (128, 96, 5000), (966, 101, 9000)]
 
 

@@ -106,3 +106,35 @@ def test_config0_sweep_energies_match_oracle(oracle_engine, gpu_engine):
         d.run(2)
         e.append(np.array(d.energies))
     assert np.abs(e[0] - e[1]).max() <= 1e-10, np.abs(e[0] - e[1]).max()
+
+
+@pytest.mark.parametrize("fuse", [0, 2])
+def test_sigma_and_left_step_fuse_modes(oracle_engine, fuse):
+    """The same sigma and left step with MPO columns applied term by term
+    (fuse 0) and with every column summed on the left boundary first (fuse 2)."""
+    from qcmaquis_amd.gpu import GpuEngine
+    from qcmaquis_amd.integrals import synthetic
+    from qcmaquis_amd.sweep import Hamiltonian
+    ham = Hamiltonian(8, 8, 0, synthetic(8, seed=1008))
+    eng = GpuEngine(0, fuse=fuse)
+    try:
+        def check(d, site):
+            psi, L, R, W = d.site_problem(site)
+            Lg = eng.upload_boundary(L.layout, L.data)
+            Rg = eng.upload_boundary(R.layout, R.data)
+            x = np.random.default_rng(site).standard_normal(psi.to_flat().size)
+            y0, _ = oracle_engine.sigma(ham.symm, psi, L, R, W, x=x)
+            y1, _ = eng.sigma(ham.symm, psi, Lg, Rg, W, x=x)
+            assert _rel(y1, y0) <= 1e-12
+            A = d.mps[site]
+            l0 = oracle_engine.left_step(ham.symm, A, L, ham.mpo[site]).dicts()
+            l1 = eng.left_step(ham.symm, A, Lg, ham.mpo[site]).dicts()
+            assert len(l1) == len(l0)
+            for b in range(len(l0)):
+                assert set(l1[b]) == set(l0[b])
+                for k in l0[b]:
+                    assert np.abs(l1[b][k] - l0[b][k]).max() <= 1e-12 * max(1.0, np.abs(l0[b][k]).max())
+
+        _sweep_pair(ham, oracle_engine, None, 48, 6, check)
+    finally:
+        eng.close()

@@ -27,10 +27,15 @@ def _check_site(ham, d, site, oracle_engine):
     psi, Lb, Rb, W = d.site_problem(site)
     x = psi.to_flat()
     y, fl = oracle_engine.sigma(ham.symm, psi, Lb, Rb, W, x=x)
-    hp = HostPlan(SiteDesc(ham.symm, psi.phys, psi.left, psi.right, W, Lb.layout, Rb.layout), "sigma")
-    ye = emulate.run_sigma(hp, Lb.data, Rb.data, x)
-    assert np.abs(y - ye).max() <= 1e-13 * max(1.0, np.abs(y).max())
-    assert hp.ref_flops == fl  # algorithmic FLOPs of the reference schedule
+    # fuse 0: every MPO term applied on its own; 1: high fan-in columns summed on the
+    # left boundary first; 2: every column summed that way
+    for fuse in (0, 1, 2):
+        hp = HostPlan(SiteDesc(ham.symm, psi.phys, psi.left, psi.right, W, Lb.layout, Rb.layout), "sigma", fuse)
+        if fuse == 2:
+            assert len(hp.parts["comb_strips"]) > 0
+        ye = emulate.run_sigma(hp, Lb.data, Rb.data, x)
+        assert np.abs(y - ye).max() <= 1e-13 * max(1.0, np.abs(y).max()), fuse
+        assert hp.ref_flops == fl  # algorithmic FLOPs of the reference schedule
     for kind, A, B, Wm in (("lstep", d.mps[site], Lb, ham.mpo[site]),
                            ("rstep", d.mps[site + 1], Rb, ham.mpo[site + 1])):
         if kind == "lstep":
@@ -39,15 +44,16 @@ def _check_site(ham, d, site, oracle_engine):
         else:
             ref = oracle_engine.right_step(ham.symm, A, B, Wm)
             sd = SiteDesc(ham.symm, A.phys, A.left, A.right, Wm, None, B.layout)
-        hps = HostPlan(sd, kind)
-        out = emulate.run_step(hps, kind, B.data, A.to_flat())
-        got = hps.out_layout.to_dicts(out)
-        exp = ref.dicts()
-        assert len(got) == len(exp)
-        for b in range(len(exp)):
-            assert set(got[b]) == set(exp[b]), (kind, b)
-            for k in exp[b]:
-                assert np.abs(got[b][k] - exp[b][k]).max() <= 1e-13 * max(1.0, np.abs(exp[b][k]).max())
+        for fuse in ((0, 1, 2) if kind == "lstep" else (1,)):
+            hps = HostPlan(sd, kind, fuse)
+            out = emulate.run_step(hps, kind, B.data, A.to_flat())
+            got = hps.out_layout.to_dicts(out)
+            exp = ref.dicts()
+            assert len(got) == len(exp)
+            for b in range(len(exp)):
+                assert set(got[b]) == set(exp[b]), (kind, b)
+                for k in exp[b]:
+                    assert np.abs(got[b][k] - exp[b][k]).max() <= 1e-13 * max(1.0, np.abs(exp[b][k]).max()), fuse
 
 
 @pytest.mark.parametrize("name", ["lih", "benzene"])
