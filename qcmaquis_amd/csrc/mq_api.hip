@@ -50,11 +50,10 @@ struct DevGemm {
     DevArray<GemmGroup> groups;
     DevArray<GemmTile> tiles;
     DevArray<int> split;
-    int ntiles = 0, n64 = 0, nsplit = 0;
+    int ntiles = 0, nsplit = 0;
     void upload(const GemmList &g, hipStream_t s) {
         groups.upload(g.groups, s);
         tiles.upload(g.tiles, s);
-        n64 = g.n64;
         std::vector<int> sp;
         for (size_t i = 0; i < g.groups.size(); ++i)
             if (g.groups[i].split > 1) sp.push_back((int)i);

@@ -38,38 +38,62 @@ __global__ void __launch_bounds__(256) k_to_f32(const double *__restrict__ src, 
 }
 
 // ------------------------------------------------------------------ W application
-// dst strip = sum_s coef_s * src_s, summed in task order (deterministic, no atomics).
+// dst strip = sum_s coef_s * src_s, summed in task order (deterministic, no
+// atomics).  Each thread owns WU elements of a pass, so every source issues
+// WU independent loads before they are accumulated; element e of the strip
+// is (i, j) = (e mod rows, e / rows), the quotient taken from a float
+// reciprocal and corrected (exact for strips below 2^24 elements).
+constexpr int WU = 4;
+
 template <int STAGE, class T>
 __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips, const Src *__restrict__ srcs,
                                                 const T *__restrict__ src, T *__restrict__ C) {
     const Strip st = strips[blockIdx.x];
-    const int n = st.rows * st.cols;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        int i = e % st.rows, j = e / st.rows;
-        T acc = 0;
+    const int rows = st.rows, n = st.rows * st.cols;
+    const float inv = 1.0f / (float)max(rows, 1);
+    for (int base = 0; base < n; base += 256 * WU) {
+        int ii[WU], jj[WU];
+        T acc[WU];
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+            const int e = base + threadIdx.x + 256 * u;
+            int j = (int)((float)e * inv);
+            if (j * rows > e) --j;
+            else if ((j + 1) * rows <= e) ++j;
+            ii[u] = e - j * rows;
+            jj[u] = e < n ? j : -1;
+            acc[u] = 0;
+        }
         for (int s = st.s0; s < st.s1; ++s) {
             const Src sr = srcs[s];
-            acc += (T)sr.coef * src[sr.off + i + (int64_t)j * sr.ld];
+            const T c = (T)sr.coef;
+            T v[WU];
+#pragma unroll
+            for (int u = 0; u < WU; ++u) v[u] = jj[u] >= 0 ? src[sr.off + ii[u] + (int64_t)jj[u] * sr.ld] : (T)0;
+#pragma unroll
+            for (int u = 0; u < WU; ++u) acc[u] += c * v[u];
         }
-        C[st.dst + i + (int64_t)j * st.ldd] = acc;
+#pragma unroll
+        for (int u = 0; u < WU; ++u)
+            if (jj[u] >= 0) C[st.dst + ii[u] + (int64_t)jj[u] * st.ldd] = acc[u];
     }
 }
 
 // ------------------------------------------------------------------ grouped GEMM (MFMA)
-// Block tile 64 x TN (TN = 64 or 128 per group), K step 16, 256 threads =
-// 4 waves in a 2x2 grid, each wave owning a 32 x TN/2 sub-tile computed with
-// 16x16x4 MFMA products (v_mfma_f64_16x16x4 for FP64, v_mfma_f32_16x16x4 for
-// FP32 operands).  The MFMA is fed with (B^T, A^T) so that its output lanes
-// run along the rows of the column-major C, which makes the stores 128-byte
-// segments.  Operand layout of both instructions: the first operand holds
-// X[lane % 16][lane / 16], the second Y[lane / 16][lane % 16].  Accumulator
-// register r of a lane holds D[4 * r + lane / 16][lane % 16] for the FP64
-// instruction and D[4 * (lane / 16) + r][lane % 16] for the FP32 one; with
-// X = B^T, Y = A^T, D's row index is C's column and its column index C's row.
-// Operands are staged through a double-buffered LDS tile with zero padding at
-// the edges, so any M, N, K (including K = 0) is handled; the next K tile is
-// fetched into registers while the current one is multiplied.
-constexpr int TM = 64, TK = 16;
+// Block tile TM x TN (64 or 128 each, chosen per group), K step 16, 256
+// threads = 4 waves in a 2x2 grid, each wave owning a TM/2 x TN/2 sub-tile
+// computed with 16x16x4 MFMA products (v_mfma_f64_16x16x4 for FP64,
+// v_mfma_f32_16x16x4 for FP32 operands).  The MFMA is fed with (B^T, A^T) so
+// that its output lanes run along the rows of the column-major C, which makes
+// the stores 128-byte segments.  Operand layout of both instructions: the
+// first operand holds X[lane % 16][lane / 16], the second Y[lane / 16][lane % 16].
+// Accumulator register r of a lane holds D[4 * r + lane / 16][lane % 16] for
+// the FP64 instruction and D[4 * (lane / 16) + r][lane % 16] for the FP32 one;
+// with X = B^T, Y = A^T, D's row index is C's column and its column index
+// C's row.  Operands are staged through a double-buffered LDS tile with zero
+// padding at the edges, so any M, N, K (including K = 0) is handled; the next
+// K tile is fetched into registers while the current one is multiplied.
+constexpr int TK = 16, TMAX = 128;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
@@ -92,16 +116,16 @@ struct Mfma<float> {
     static __device__ __forceinline__ int dcol(int r, int lq) { return 4 * lq + r; }
 };
 
-template <int TN, class T>
+template <int TM, int TN, class T>
 struct GemmStage {
-    T a[4], b[TN / 16];
+    T a[TM / 16], b[TN / 16];
 };
 
-template <int TN, class T>
+template <int TM, int TN, class T>
 __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restrict__ A, const T *__restrict__ B,
-                                           int m0, int n0, int k0, int kend, GemmStage<TN, T> &st) {
+                                           int m0, int n0, int k0, int kend, GemmStage<TM, TN, T> &st) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < TM / 16; ++r) {
         const int e = threadIdx.x + 256 * r;
         int i, k;
         if (!g.transA) {
@@ -134,11 +158,11 @@ __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restri
     }
 }
 
-template <int TN, class T>
-__device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<TN, T> &st, T (*As)[TM + 2],
+template <int TM, int TN, class T>
+__device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<TM, TN, T> &st, T (*As)[TM + 2],
                                            T (*Bs)[TN + 2]) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < TM / 16; ++r) {
         const int e = threadIdx.x + 256 * r;
         if (!g.transA)
             As[e / TM][e % TM] = st.a[r];
@@ -157,46 +181,48 @@ __device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<T
 
 // T: operand type; TO: output type; partial tiles of split-K groups are
 // stored in T and summed by k_split_reduce.
-template <int TN, class T, class TO>
+template <int TM, int TN, class T, class TO>
 __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t, const T *__restrict__ A,
                                           const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
                                           double *lds) {
     typedef Mfma<T> MF;
-    constexpr int NJ = TN / 32;  // 16-wide MFMA columns per wave
+    constexpr int MI = TM / 32, NJ = TN / 32;  // 16x16 MFMA blocks per wave
     T(*As)[TK][TM + 2] = reinterpret_cast<T(*)[TK][TM + 2]>(lds);
     T(*Bs)[TK][TN + 2] = reinterpret_cast<T(*)[TK][TN + 2]>(reinterpret_cast<T *>(lds) + 2 * TK * (TM + 2));
     const int m0 = t.mt * TM, n0 = t.nt * TN;
     const int kbeg = t.ks * g.kchunk;
     const int kend = min(g.K, kbeg + g.kchunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * (TN / 2);
+    const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
     const int lr = lane & 15, lq = lane >> 4;
-    typename MF::acc acc[2][NJ];
+    typename MF::acc acc[MI][NJ];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j) acc[i][j] = typename MF::acc{0, 0, 0, 0};
-    GemmStage<TN, T> st;
+    GemmStage<TM, TN, T> st;
     int buf = 0;
     if (kbeg < kend) {
-        gemm_fetch<TN, T>(g, A, B, m0, n0, kbeg, kend, st);
-        gemm_stash<TN, T>(g, st, As[0], Bs[0]);
+        gemm_fetch<TM, TN, T>(g, A, B, m0, n0, kbeg, kend, st);
+        gemm_stash<TM, TN, T>(g, st, As[0], Bs[0]);
     }
     __syncthreads();
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
         const bool more = k0 + TK < kend;
-        if (more) gemm_fetch<TN, T>(g, A, B, m0, n0, k0 + TK, kend, st);
+        if (more) gemm_fetch<TM, TN, T>(g, A, B, m0, n0, k0 + TK, kend, st);
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 4) {
-            T a0 = As[buf][kk + lq][wm + lr], a1 = As[buf][kk + lq][wm + 16 + lr];
+            T av[MI];
+#pragma unroll
+            for (int i = 0; i < MI; ++i) av[i] = As[buf][kk + lq][wm + 16 * i + lr];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 T bj = Bs[buf][kk + lq][wn + 16 * j + lr];
-                acc[0][j] = MF::run(bj, a0, acc[0][j]);
-                acc[1][j] = MF::run(bj, a1, acc[1][j]);
+#pragma unroll
+                for (int i = 0; i < MI; ++i) acc[i][j] = MF::run(bj, av[i], acc[i][j]);
             }
         }
-        if (more) gemm_stash<TN, T>(g, st, As[buf ^ 1], Bs[buf ^ 1]);
+        if (more) gemm_stash<TM, TN, T>(g, st, As[buf ^ 1], Bs[buf ^ 1]);
         __syncthreads();
         buf ^= 1;
     }
@@ -204,7 +230,7 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
         TO *out = C + g.c;
         const int64_t ld = g.ldc;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < MI; ++i) {
             const int gm = m0 + wm + 16 * i + lr;
             if (gm >= g.M) continue;
 #pragma unroll
@@ -219,7 +245,7 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
         T *out = W + g.part + (int64_t)t.ks * g.M * g.N;
         const int64_t ld = g.M;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < MI; ++i) {
             const int gm = m0 + wm + 16 * i + lr;
             if (gm >= g.M) continue;
 #pragma unroll
@@ -233,19 +259,26 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
     }
 }
 
-// One launch per stage: each tile runs the 64- or 128-wide body of its group.
+// One launch per stage: each tile runs the body of its group's tile shape.
 template <int STAGE, class T, class TO>
 __global__ void __launch_bounds__(256) k_gemm(const GemmGroup *__restrict__ groups, const GemmTile *__restrict__ tiles,
                                               const T *__restrict__ A, const T *__restrict__ B, TO *__restrict__ C,
                                               T *__restrict__ W) {
-    __shared__ double lds[2 * TK * (TM + 2) + 2 * TK * (128 + 2)];
+    __shared__ double lds[(2 * TK * (TMAX + 2) * 2 * sizeof(T) + sizeof(double) - 1) / sizeof(double)];
     const GemmTile t = tiles[blockIdx.x];
     if (t.g < 0) return;  // padding tile of the XCD-ordered list
     const GemmGroup g = groups[t.g];
-    if (g.tn == 128)
-        gemm_tile<128, T, TO>(g, t, A, B, C, W, lds);
-    else
-        gemm_tile<64, T, TO>(g, t, A, B, C, W, lds);
+    if (g.tm == 128) {
+        if (g.tn == 128)
+            gemm_tile<128, 128, T, TO>(g, t, A, B, C, W, lds);
+        else
+            gemm_tile<128, 64, T, TO>(g, t, A, B, C, W, lds);
+    } else {
+        if (g.tn == 128)
+            gemm_tile<64, 128, T, TO>(g, t, A, B, C, W, lds);
+        else
+            gemm_tile<64, 64, T, TO>(g, t, A, B, C, W, lds);
+    }
 }
 
 // C = sum over splits of the partial tiles, in split order (deterministic),

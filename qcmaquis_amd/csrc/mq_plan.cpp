@@ -116,9 +116,11 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     (void)tn;
     g.split = 1;
     g.kchunk = std::max(g.K, 1);
+    g.tm = g.M > 64 ? 128 : 64;
     g.tn = g.N > 64 ? 128 : 64;
+    (void)tm;
     if (g.M == 0 || g.N == 0) return;
-    int mt = ceil_div(g.M, tm), nt = ceil_div(g.N, g.tn);
+    int mt = ceil_div(g.M, g.tm), nt = ceil_div(g.N, g.tn);
     // split K when the output is small and K long: chunks of about kchunk_target
     if (g.K > 2 * kchunk_target && mt * nt < 64) {
         int s = std::min(ceil_div(g.K, kchunk_target), 64);
@@ -430,11 +432,6 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     if (!cp || cp->fg.empty()) return t;
 
     // ---- fused groups: blocks, panels, combination strips, T GEMMs
-    std::vector<std::unordered_map<uint64_t, int>> pos_of(nb1);  // (lb, lk) -> index into blk[b1]
-    for (int b1 = 0; b1 < nb1; ++b1)
-        if (cp->member[b1])
-            for (size_t i = 0; i < blk[b1].size(); ++i)
-                pos_of[b1][((uint64_t)blk[b1][i].lb << 32) | (uint32_t)blk[b1][i].lk] = (int)i;
     struct Panel {
         int lk, pin, row, ar, rows = 0;
         int64_t lf = 0, tf = 0;
@@ -442,41 +439,42 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     std::map<std::tuple<int, int, int>, int> panel_id;
     std::vector<Panel> panels;
     std::vector<std::vector<std::pair<int, int>>> fb_panel(cp->fg.size());  // per group block: (panel, row)
-    std::vector<std::pair<int, int>> v;
+    const int nl = s.nl;
+    std::vector<int> slot((size_t)nl * nl, -1);  // (lk, lb) -> block index inside the current group
     for (size_t gi = 0; gi < cp->fg.size(); ++gi) {
         FGroup &g = cp->fg[gi];
         const int ap = s.W.rspin[g.b2];
-        v.clear();
+        // union of the members' (lb, lk) blocks, in (lk, lb) order
+        std::vector<char> mark((size_t)nl * nl, 0);
         for (auto &m : g.mem)
-            for (auto &a : blk[m.first]) v.push_back({a.lk, a.lb});
-        std::sort(v.begin(), v.end());
-        v.erase(std::unique(v.begin(), v.end()), v.end());
-        for (auto &kb : v) {
-            int lk = kb.first, lb = kb.second;
-            int ar = s.lp_r[(size_t)lk * s.np + g.pin];
-            if (ar < 0 || s.pbr_sz_l[lk] == 0) continue;
-            int al = s.lp_r[(size_t)lb * s.np + g.pout];
-            if (al < 0 || s.x_of_r[al] < 0) continue;  // out_left_i
-            if (s.su2 && !tri(spin2(s.right.q[ar]), ap, spin2(s.right.q[al]))) continue;
-            auto key = std::make_tuple(lk, g.pin, g.row);
-            auto it = panel_id.find(key);
-            int pid;
-            if (it == panel_id.end()) {
-                pid = (int)panels.size();
-                panel_id.emplace(key, pid);
-                Panel pn;
-                pn.lk = lk;
-                pn.pin = g.pin;
-                pn.row = g.row;
-                pn.ar = ar;
-                panels.push_back(pn);
-            } else {
-                pid = it->second;
+            for (auto &a : blk[m.first]) mark[(size_t)a.lk * nl + a.lb] = 1;
+        for (int lk = 0; lk < nl; ++lk)
+            for (int lb = 0; lb < nl; ++lb) {
+                if (!mark[(size_t)lk * nl + lb]) continue;
+                int ar = s.lp_r[(size_t)lk * s.np + g.pin];
+                if (ar < 0 || s.pbr_sz_l[lk] == 0) continue;
+                int al = s.lp_r[(size_t)lb * s.np + g.pout];
+                if (al < 0 || s.x_of_r[al] < 0) continue;  // out_left_i
+                if (s.su2 && !tri(spin2(s.right.q[ar]), ap, spin2(s.right.q[al]))) continue;
+                auto key = std::make_tuple(lk, g.pin, g.row);
+                auto it = panel_id.find(key);
+                int pid;
+                if (it == panel_id.end()) {
+                    pid = (int)panels.size();
+                    panel_id.emplace(key, pid);
+                    Panel pn;
+                    pn.lk = lk;
+                    pn.pin = g.pin;
+                    pn.row = g.row;
+                    pn.ar = ar;
+                    panels.push_back(pn);
+                } else {
+                    pid = it->second;
+                }
+                g.blk.push_back({lb, lk, ar, al, 0, 0, 0});
+                fb_panel[gi].push_back({pid, panels[pid].rows});
+                panels[pid].rows += s.lsize(lb);
             }
-            g.blk.push_back({lb, lk, ar, al, 0, 0, 0});
-            fb_panel[gi].push_back({pid, panels[pid].rows});
-            panels[pid].rows += s.lsize(lb);
-        }
     }
     for (auto &pn : panels) {
         pn.lf = t.la_total;
@@ -485,6 +483,7 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
         t.t_total += (int64_t)pn.rows * s.rsize(pn.ar);
     }
     std::vector<uint64_t> comb_key;  // (lk, lb): strips reading one LA_lk row range run together
+    std::vector<int> cnt, pos;
     for (size_t gi = 0; gi < cp->fg.size(); ++gi) {
         FGroup &g = cp->fg[gi];
         for (size_t k = 0; k < g.blk.size(); ++k) {
@@ -494,24 +493,42 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
             fb.lf = pn.lf + row;
             fb.tf = pn.tf + row;
             fb.ld = pn.rows;
-            if (!comb) continue;
+        }
+        if (!comb) continue;
+        // sources of each block, in member order: count, then fill
+        const size_t nb = g.blk.size();
+        for (size_t k = 0; k < nb; ++k) slot[(size_t)g.blk[k].lk * nl + g.blk[k].lb] = (int)k;
+        cnt.assign(nb, 0);
+        for (auto &m : g.mem)
+            for (auto &a : blk[m.first]) {
+                int k = slot[(size_t)a.lk * nl + a.lb];
+                if (k >= 0) ++cnt[k];
+            }
+        pos.assign(nb, 0);
+        int base = (int)comb->srcs.size();
+        for (size_t k = 0; k < nb; ++k) {
+            pos[k] = base;
+            base += cnt[k];
+        }
+        comb->srcs.resize(base);
+        for (size_t k = 0; k < nb; ++k) {
+            const FBlk &fb = g.blk[k];
             Strip st;
             st.dst = fb.lf;
             st.rows = s.lsize(fb.lb);
             st.cols = s.lsize(fb.lk);
-            st.ldd = pn.rows;
-            st.s0 = (int)comb->srcs.size();
-            uint64_t bk = ((uint64_t)fb.lb << 32) | (uint32_t)fb.lk;
-            for (auto &m : g.mem) {
-                auto it = pos_of[m.first].find(bk);
-                if (it == pos_of[m.first].end()) continue;
-                const A &a = blk[m.first][it->second];
-                comb->srcs.push_back({t.la_off[a.lk] + a.row, t.Mtot[a.lk], m.second});
-            }
-            st.s1 = (int)comb->srcs.size();
+            st.ldd = fb.ld;
+            st.s0 = pos[k];
+            st.s1 = pos[k] + cnt[k];
             comb->strips.push_back(st);
-            comb_key.push_back(bk);
+            comb_key.push_back(((uint64_t)fb.lb << 32) | (uint32_t)fb.lk);
         }
+        for (auto &m : g.mem)
+            for (auto &a : blk[m.first]) {
+                int k = slot[(size_t)a.lk * nl + a.lb];
+                if (k >= 0) comb->srcs[pos[k]++] = {t.la_off[a.lk] + a.row, t.Mtot[a.lk], m.second};
+            }
+        for (size_t k = 0; k < nb; ++k) slot[(size_t)g.blk[k].lk * nl + g.blk[k].lb] = -1;
     }
     if (comb) {
         if (comb_key.size() != comb->strips.size()) throw std::runtime_error("combination strip list out of step");
@@ -716,15 +733,10 @@ void patch_strips(StripList &sl, const std::vector<int> &strip_al, const std::ve
 void tile_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kc) { add_gemm(gl, g, tm, tn, kc); }
 
 void order_tiles(GemmList &gl) {
-    std::vector<GemmTile> t64, t128;
-    for (auto &t : gl.tiles) {
-        if (t.g < 0) continue;
-        (gl.groups[t.g].tn == 128 ? t128 : t64).push_back(t);
-    }
-    auto a = xcd_order(t64), b = xcd_order(t128);
-    gl.n64 = (int)a.size();
-    gl.tiles = a;
-    gl.tiles.insert(gl.tiles.end(), b.begin(), b.end());
+    std::vector<GemmTile> t;
+    for (auto &x : gl.tiles)
+        if (x.g >= 0) t.push_back(x);
+    gl.tiles = xcd_order(t);
 }
 
 // ============================================================ sigma

@@ -18,8 +18,8 @@ struct Copy2D {
 
 // C(M x N) = op(A)(M x K) * op(B)(K x N), column-major with leading dims.
 // transA: A stored K x M; transB: B stored N x K.  split > 1: K is cut into
-// `split` chunks whose partial tiles go to the workspace at `part`.  tn: tile
-// width (64 or 128) of the group's tiles; tiles are 64 rows high.
+// `split` chunks whose partial tiles go to the workspace at `part`.  tm, tn:
+// tile height and width (64 or 128) of the group's tiles.
 struct GemmGroup {
     int64_t a, b, c, part;
     int M, N, K;
@@ -27,7 +27,7 @@ struct GemmGroup {
     int transA, transB;
     int split;
     int kchunk;
-    int tn;
+    int tm, tn;
 };
 // one output tile; g < 0 marks a padding tile that does nothing
 struct GemmTile {
@@ -54,12 +54,10 @@ struct Src {
 // Tiles are ordered for the XCDs: workgroup i of a launch runs on XCD i % 8,
 // and the tiles of one (group, K chunk, row block) -- which share their A
 // panel -- are placed on one XCD, consecutively, so the panel is read from
-// HBM once and then served by that XCD's L2.  tiles[0, n64) use 64-wide
-// tiles, tiles[n64, end) 128-wide ones (two launches).
+// HBM once and then served by that XCD's L2.
 struct GemmList {
     std::vector<GemmGroup> groups;
     std::vector<GemmTile> tiles;
-    int n64 = 0;
     int64_t ws = 0;  // workspace elements for split partials
     double flops() const {
         double f = 0;

@@ -88,8 +88,9 @@ class MPOTensor:
 
 
 class TaggedMPOMaker:
-    def __init__(self, L, table: OpTable, identities, fillings, identities_full=None, su2=True):
+    def __init__(self, L, table: OpTable, identities, fillings, identities_full=None, su2=True, herm=True):
         self.L = L
+        self.herm = herm  # detect hermitian-partner MPO bonds (needs exact conjugate tags)
         self.table = table
         self.ident = identities
         self.ident_full = identities_full if identities_full is not None else identities
@@ -284,7 +285,7 @@ class TaggedMPOMaker:
                 rr = 0 if p == L - 1 else right[k2]
                 for (tag, scale) in self.prempo[p][kk]:
                     pre.append((ll, rr, tag, scale))
-                ck2, phase = self._conjugate_key(k2, p)
+                ck2, phase = self._conjugate_key(k2, p) if self.herm else (k2, None)
                 if ck2 != k2:
                     herm_pairs[k2] = ck2
                     herm_phases[k2] = phase

@@ -19,29 +19,42 @@ from .integrals import to_aligned
 from .mpo import TaggedMPOMaker, make_twosite_mpo
 from .mps import (MPSTensor, left_normalize_and_return, make_twosite_mps, multiply_from_left,
                   multiply_from_right, normalize_right, random_mps, right_normalize_and_return, split_twosite)
-from .operators import OpTable, SU2Ops, su2_phys_index
+from .operators import OpTable, SU2Ops, TwoU1Ops, su2_phys_index, twou1_phys_index
 from .terms import su2_terms
+from .twou1 import twou1_terms
 
 
 class Hamiltonian:
     """Quantum-chemistry Hamiltonian as an MPO (harness input)."""
 
     def __init__(self, L, nelec, spin, integrals, symm="su2u1"):
-        if symm != "su2u1":
-            raise NotImplementedError("only the SU2U1 model is built in this harness")
+        """spin is 2S.  symm "su2u1" (models/chem/su2u1) or "2u1" (models/chem/2u1,
+        spin-orbital basis with N_alpha - N_beta = 2S)."""
         self.L, self.nelec, self.spin = L, nelec, spin
-        self.symm = MQ_SYMM_SU2U1
-        self.su2 = True
-        ints = to_aligned(integrals)
         table = OpTable()
-        ops = SU2Ops(table)
-        self.terms = su2_terms(ints, L, nelec, ops)
-        mk = TaggedMPOMaker(L, table, ops.ident, ops.fill, ops.ident_full)
+        if symm == "su2u1":
+            self.symm = MQ_SYMM_SU2U1
+            self.su2 = True
+            ops = SU2Ops(table)
+            self.terms = su2_terms(to_aligned(integrals), L, nelec, ops)
+            mk = TaggedMPOMaker(L, table, ops.ident, ops.fill, ops.ident_full)
+            self.phys = su2_phys_index()
+            self.right_end = (nelec, spin)
+        elif symm == "2u1":
+            if (nelec + spin) % 2:
+                raise ValueError("N and 2S must have the same parity")
+            self.symm = MQ_SYMM_TWOU1
+            self.su2 = False
+            ops = TwoU1Ops(table)
+            self.terms = twou1_terms(integrals, L, ops)
+            mk = TaggedMPOMaker(L, table, ops.ident, ops.fill, su2=False, herm=False)
+            self.phys = twou1_phys_index()
+            self.right_end = ((nelec + spin) // 2, (nelec - spin) // 2)
+        else:
+            raise ValueError(f"unknown symmetry {symm!r}")
         for t in self.terms:
             mk.add_term(t)
         self.mpo, self.ecore = mk.create_mpo()
-        self.phys = su2_phys_index()
-        self.right_end = (nelec, spin)
         self._ts = {}
 
     def ts_mpo(self, p):

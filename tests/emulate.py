@@ -38,7 +38,7 @@ def wapply(strips, srcs, T, Cbuf):
         _mat(Cbuf, int(st["dst"]), r, c, int(st["ldd"]))[...] = acc
 
 
-def gemm(groups, tiles, A, B, C, tm=64):
+def gemm(groups, tiles, A, B, C):
     # tile coverage: every (m, n, k-split) tile exactly once (g < 0: padding tiles)
     seen = {}
     for t in tiles:
@@ -50,8 +50,8 @@ def gemm(groups, tiles, A, B, C, tm=64):
     assert len(seen) == sum(1 for t in tiles if t["g"] >= 0)
     for gi, g in enumerate(groups):
         M, N, K = int(g["M"]), int(g["N"]), int(g["K"])
-        tn = int(g["tn"])
-        assert tn in (64, 128)
+        tm, tn = int(g["tm"]), int(g["tn"])
+        assert tm in (64, 128) and tn in (64, 128)
         mt, nt = -(-M // tm), -(-N // tn)
         for ks in range(int(g["split"])):
             for n in range(nt):
