@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--sectors", default=os.path.join(ROOT, "tests", "golden", "sectors_L24_m512.json"))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-delta-e", action="store_true", help="skip the converged-solve energy comparison")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="first run a plain device copy of this many GiB (reference for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sigma-vector path is built and timed on a single MI355X; "
@@ -89,6 +92,18 @@ def main():
     from qcmaquis_amd.workload import centre_problem, load_sectors
 
     eng = GpuEngine(0)
+    if a.copy_calibration > 0:
+        n = int(a.copy_calibration * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"copy calibration: {n * 8 / 2**30:.2f} GiB read + written in {dt * 1e3:.2f} ms "
+            f"({2 * n * 8 / dt / 1e9:.0f} GB/s)")
+        del src, dst
     t0 = time.perf_counter()
     sectors, sec_src = None, f"degeneracy-proportional sectors capped at m={a.m}"
     if a.sectors and os.path.exists(a.sectors):
@@ -141,18 +156,20 @@ def main():
     # energy agreement at convergence, the regime of the solves that fix a sweep's
     # final energy: the lowest eigenvector is first converged in FP64 (restarted
     # solves of 20 iterations), then both forms run one more solve from it
-    u = x
-    for _ in range(4):
-        _, u, _ = sp.jd(u, 20, 1e-12, prec=MQ_PREC_FP64)
-    conv = {}
-    for name, prec in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)):
-        th, _, its = sp.jd(u, 10, 1e-10, prec=prec)
-        conv[name] = dict(theta=th + ham.ecore, iterations=its)
-    dE = abs(conv["mixed"]["theta"] - conv["fp64"]["theta"])
-    log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it) mixed "
-        f"{conv['mixed']['theta']:.12f} ({conv['mixed']['iterations']} it) dE {dE:.2e}; "
-        f"mixed sigma rel err {sig_err:.2e}")
-    mixed_ok = dE <= 1e-8
+    conv, dE = None, None
+    if not a.no_delta_e:
+        u = x
+        for _ in range(4):
+            _, u, _ = sp.jd(u, 20, 1e-12, prec=MQ_PREC_FP64)
+        conv = {}
+        for name, prec in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)):
+            th, _, its = sp.jd(u, 10, 1e-10, prec=prec)
+            conv[name] = dict(theta=th + ham.ecore, iterations=its)
+        dE = abs(conv["mixed"]["theta"] - conv["fp64"]["theta"])
+        log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it) mixed "
+            f"{conv['mixed']['theta']:.12f} ({conv['mixed']['iterations']} it) dE {dE:.2e}; "
+            f"mixed sigma rel err {sig_err:.2e}")
+    mixed_ok = dE is not None and dE <= 1e-8
     ch, prec_name = (r32, "mixed") if mixed_ok else (r64, "fp64")
     dt, avg, work = ch["dt"], ch["avg"], ch["work"]
 

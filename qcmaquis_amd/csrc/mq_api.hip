@@ -85,6 +85,14 @@ struct mq_plan {
     DevArray<double> jd_vecs, jd_partial, jd_coef;
     DevArray<const double *> jd_ptrs;
     int jd_cap = 0;
+    // pinned host staging of the Davidson coefficients / pointers / partial sums
+    double *h_coef = nullptr, *h_partial = nullptr;
+    const double **h_ptrs = nullptr;
+    ~mq_plan() {
+        if (h_coef) (void)hipHostFree(h_coef);
+        if (h_partial) (void)hipHostFree(h_partial);
+        if (h_ptrs) (void)hipHostFree((void *)h_ptrs);
+    }
 };
 
 static int fail(mq_ctx *ctx, const std::exception &e, int code) {
@@ -458,46 +466,79 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
 // ------------------------------------------------------------------ Jacobi-Davidson
 namespace {
 
+// Davidson vector operations on the plan's stream.  Pointer lists and
+// coefficients are staged in pinned host slots that stay untouched until the
+// next synchronisation, so linear combinations are enqueued without waiting;
+// only dot products synchronise (one launch for a whole batch of pairs).
+constexpr int kJdSlots = 4096, kJdPartialPairs = 128, kJdBlocks = 256;
+
 struct JD {
     mq_plan *p;
     hipStream_t s;
     int64_t n;
     int nblk;
-    std::vector<double> host_partial;
+    int ptr_used = 0, coef_used = 0;
 
     double *vec(int i) { return p->jd_vecs.p + (int64_t)i * n; }
 
-    // dots of pairs (x_k, y_k), summed on the host in a fixed order
+    void init() {
+        if (!p->h_coef) {
+            check(hipHostMalloc((void **)&p->h_coef, sizeof(double) * kJdSlots), "pinned coef");
+            check(hipHostMalloc((void **)&p->h_partial, sizeof(double) * kJdPartialPairs * kJdBlocks),
+                  "pinned partials");
+            check(hipHostMalloc((void **)&p->h_ptrs, sizeof(double *) * kJdSlots), "pinned ptrs");
+            p->jd_partial.alloc((size_t)kJdPartialPairs * kJdBlocks);
+            p->jd_coef.alloc(kJdSlots);
+            p->jd_ptrs.alloc(kJdSlots);
+        }
+    }
+    const double **stage_ptrs(const std::vector<const double *> &v, int *slot) {
+        if (ptr_used + (int)v.size() > kJdSlots) sync();
+        *slot = ptr_used;
+        for (size_t i = 0; i < v.size(); ++i) p->h_ptrs[ptr_used + i] = v[i];
+        check(hipMemcpyAsync(p->jd_ptrs.p + ptr_used, p->h_ptrs + ptr_used, sizeof(double *) * v.size(),
+                             hipMemcpyHostToDevice, s),
+              "stage ptrs");
+        ptr_used += (int)v.size();
+        return p->jd_ptrs.p + *slot;
+    }
+    void sync() {
+        check(hipStreamSynchronize(s), "davidson sync");
+        ptr_used = coef_used = 0;
+    }
+    // dots of pairs (x_k, y_k), summed on the host in a fixed order (one synchronisation)
     std::vector<double> dots(const std::vector<const double *> &xs, const std::vector<const double *> &ys) {
         int np = (int)xs.size();
+        if (np > kJdPartialPairs) throw std::runtime_error("too many dot products in one batch");
         std::vector<const double *> ptrs(xs);
         ptrs.insert(ptrs.end(), ys.begin(), ys.end());
-        check(hipMemcpyAsync(p->jd_ptrs.p, ptrs.data(), sizeof(double *) * ptrs.size(), hipMemcpyHostToDevice, s),
-              "dot ptrs");
-        check(launch_multidot(p->jd_ptrs.p, p->jd_ptrs.p + np, np, n, nblk, p->jd_partial.p, s), "multidot");
-        host_partial.resize((size_t)np * nblk);
-        check(hipMemcpyAsync(host_partial.data(), p->jd_partial.p, sizeof(double) * np * nblk, hipMemcpyDeviceToHost,
-                             s),
+        int slot;
+        const double *const *d = stage_ptrs(ptrs, &slot);
+        check(launch_multidot(d, d + np, np, n, nblk, p->jd_partial.p, s), "multidot");
+        check(hipMemcpyAsync(p->h_partial, p->jd_partial.p, sizeof(double) * np * nblk, hipMemcpyDeviceToHost, s),
               "dot partials");
-        check(hipStreamSynchronize(s), "dot sync");
+        sync();
         std::vector<double> r(np, 0.0);
         for (int k = 0; k < np; ++k) {
             long double acc = 0;
-            for (int b = 0; b < nblk; ++b) acc += host_partial[(size_t)k * nblk + b];
+            for (int b = 0; b < nblk; ++b) acc += p->h_partial[(size_t)k * nblk + b];
             r[k] = (double)acc;
         }
         return r;
     }
     double dot(const double *x, const double *y) { return dots({x}, {y})[0]; }
-    // out = beta*out + sum c_j v_j
+    // out = beta*out + sum c_j v_j, enqueued without waiting
     void lincomb(const std::vector<const double *> &vs, const std::vector<double> &c, double beta, double *out) {
         int nv = (int)vs.size();
-        check(hipMemcpyAsync(p->jd_ptrs.p + 128, vs.data(), sizeof(double *) * nv, hipMemcpyHostToDevice, s),
-              "lc ptrs");
-        check(hipMemcpyAsync(p->jd_coef.p, c.data(), sizeof(double) * nv, hipMemcpyHostToDevice, s), "lc coef");
-        check(launch_lincomb(p->jd_ptrs.p + 128, p->jd_coef.p, nv, n, beta, out, s), "lincomb");
-        // the host arrays are reused by the next call: wait for the copies
-        check(hipStreamSynchronize(s), "lincomb sync");
+        if (coef_used + nv > kJdSlots || ptr_used + nv > kJdSlots) sync();
+        int slot;
+        const double *const *d = stage_ptrs(vs, &slot);
+        int cs = coef_used;
+        for (int j = 0; j < nv; ++j) p->h_coef[cs + j] = c[j];
+        check(hipMemcpyAsync(p->jd_coef.p + cs, p->h_coef + cs, sizeof(double) * nv, hipMemcpyHostToDevice, s),
+              "stage coef");
+        coef_used += nv;
+        check(launch_lincomb(d, p->jd_coef.p + cs, nv, n, beta, out, s), "lincomb");
     }
 };
 
@@ -559,17 +600,13 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
         jd.p = plan;
         jd.s = plan->ctx->stream;
         jd.n = plan->hp.vec_len;
-        jd.nblk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (jd.n + 255) / 256));
+        jd.nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kJdBlocks, (jd.n + 255) / 256));
+        jd.init();
         const int64_t n = jd.n;
         int need = 2 * max_iter + 2;
         if (plan->jd_cap < need) {
             plan->jd_vecs.alloc((size_t)need * n);
             plan->jd_cap = need;
-        }
-        if (!plan->jd_partial.p) {
-            plan->jd_partial.alloc((size_t)128 * 256);
-            plan->jd_coef.alloc(256);
-            plan->jd_ptrs.alloc(256);
         }
         auto V = [&](int i) { return jd.vec(i); };
         auto VA = [&](int i) { return jd.vec(max_iter + i); };
@@ -577,10 +614,15 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
         std::vector<double> oo(n_ortho);
         for (int k = 0; k < n_ortho; ++k) oo[k] = jd.dot(ortho[k], ortho[k]);
         auto project = [&](double *t) {  // SingleSiteVS::project (ietl_lanczos_solver.h:90-95)
+            if (n_ortho == 0) return;
+            std::vector<const double *> xs, ys;
             for (int k = 0; k < n_ortho; ++k) {
-                double f = jd.dot(ortho[k], t) / oo[k];
-                jd.lincomb({ortho[k]}, {-f}, 1.0, t);
+                xs.push_back(ortho[k]);
+                ys.push_back(t);
             }
+            std::vector<double> d = jd.dots(xs, ys), c(n_ortho);
+            for (int k = 0; k < n_ortho; ++k) c[k] = -d[k] / oo[k];
+            jd.lincomb(std::vector<const double *>(ortho, ortho + n_ortho), c, 1.0, t);
         };
         check(hipMemcpyAsync(V(0), x0, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "x0");
         project(V(0));
@@ -588,16 +630,39 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
         int it = 0;
         double theta = 0;
         while (true) {
+            // orthogonalise t against V (jacobi.h:384-393: Gram-Schmidt, repeated when the
+            // norm drops below a quarter of its value), then normalise
             double *t = V(it);
-            double tau = std::sqrt(jd.dot(t, t));
-            for (int i = 0; i < it; ++i) jd.lincomb({V(i)}, {-jd.dot(V(i), t)}, 1.0, t);
-            if (std::sqrt(jd.dot(t, t)) < 0.25 * tau)
-                for (int i = 0; i < it; ++i) jd.lincomb({V(i)}, {-jd.dot(V(i), t)}, 1.0, t);
-            project(t);
-            double nt = std::sqrt(jd.dot(t, t));
+            std::vector<const double *> xs{t}, ys{t};
+            for (int i = 0; i < it; ++i) {
+                xs.push_back(V(i));
+                ys.push_back(t);
+            }
+            std::vector<double> d = jd.dots(xs, ys);
+            const double tau = std::sqrt(d[0]);
+            double nt2 = d[0];
+            for (int pass = 0; pass < 2 && it > 0; ++pass) {
+                std::vector<const double *> vv;
+                std::vector<double> c;
+                for (int i = 0; i < it; ++i) {
+                    vv.push_back(V(i));
+                    c.push_back(-d[1 + i]);
+                }
+                jd.lincomb(vv, c, 1.0, t);
+                d = jd.dots(xs, ys);
+                nt2 = d[0];
+                if (std::sqrt(nt2) >= 0.25 * tau) break;
+            }
+            if (n_ortho > 0) {
+                project(t);
+                nt2 = jd.dot(t, t);
+            }
+            const double nt = std::sqrt(nt2);
             jd.lincomb({t}, {1.0 / nt - 1.0}, 1.0, t);
             sigma_launch(plan, L, R, t, VA(it), prec);
-            std::vector<const double *> xs, ys;
+            // Rayleigh column (jacobi.h:401-402)
+            xs.clear();
+            ys.clear();
             for (int i = 0; i <= it; ++i) {
                 xs.push_back(V(i));
                 ys.push_back(VA(it));
@@ -634,10 +699,11 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             project(uA);
             jd.lincomb({u}, {-theta}, 1.0, uA);  // r = uA - theta u
             ++it;
-            double rn = std::sqrt(jd.dot(uA, uA));
+            // |r|, r.u and u.u in one batch: convergence test and correction t = -r + (r.u)/(u.u) u
+            std::vector<double> rr = jd.dots({uA, uA, u}, {uA, u, u});
+            double rn = std::sqrt(rr[0]);
             if (rn <= tol * std::fabs(theta) || rn < tol || it >= max_iter) break;
-            std::vector<double> d2 = jd.dots({uA, u}, {u, u});
-            jd.lincomb({uA, u}, {-1.0, d2[0] / d2[1]}, 0.0, V(it));
+            jd.lincomb({uA, u}, {-1.0, rr[1] / rr[2]}, 0.0, V(it));
         }
         if (prec != MQ_PREC_FP64) {
             // FP64 defect correction: the eigenvalue is the FP64 Rayleigh quotient of the
@@ -647,7 +713,7 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             theta = d[0] / d[1];
         }
         check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
-        check(hipStreamSynchronize(jd.s), "jd end");
+        jd.sync();
         *theta_out = theta;
         if (iterations) *iterations = it;
         return MQ_OK;

@@ -13,7 +13,10 @@
 #include "mq_plan.hpp"
 
 #include <array>
+#include <atomic>
+#include <exception>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 
 namespace mq {
@@ -111,6 +114,39 @@ struct Site {
 };
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// f(i) for i < n on up to 16 host threads (dynamic chunks); f must only write
+// data owned by index i.  Exceptions are rethrown on the calling thread.
+template <class F>
+void parallel_for(int n, F &&f) {
+    int nt = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    nt = std::min(nt, std::max(1, n / 64));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::exception_ptr err;
+    std::mutex mu;
+    auto work = [&]() {
+        try {
+            for (;;) {
+                int i0 = next.fetch_add(16);
+                if (i0 >= n) break;
+                for (int i = i0; i < std::min(n, i0 + 16); ++i) f(i);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!err) err = std::current_exception();
+            next.store(n);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < nt; ++k) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
 
 void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     (void)tn;
@@ -432,20 +468,26 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     if (!cp || cp->fg.empty()) return t;
 
     // ---- fused groups: blocks, panels, combination strips, T GEMMs
+    // Per-group work runs on a thread pool; every order-dependent step (panel and
+    // row assignment, source offsets) is sequential in group order, so the plan
+    // does not depend on the thread count.
     struct Panel {
         int lk, pin, row, ar, rows = 0;
         int64_t lf = 0, tf = 0;
     };
-    std::map<std::tuple<int, int, int>, int> panel_id;
-    std::vector<Panel> panels;
-    std::vector<std::vector<std::pair<int, int>>> fb_panel(cp->fg.size());  // per group block: (panel, row)
+    struct Cand {
+        int lk, lb, ar, al;
+    };
     const int nl = s.nl;
-    std::vector<int> slot((size_t)nl * nl, -1);  // (lk, lb) -> block index inside the current group
-    for (size_t gi = 0; gi < cp->fg.size(); ++gi) {
-        FGroup &g = cp->fg[gi];
+    const int ng = (int)cp->fg.size();
+    // 1. (parallel) union of the members' (lb, lk) blocks, in (lk, lb) order, kept when
+    //    the charges and spins let the block contribute
+    std::vector<std::vector<Cand>> cand(ng);
+    parallel_for(ng, [&](int gi) {
+        thread_local std::vector<char> mark;
+        mark.assign((size_t)nl * nl, 0);
+        const FGroup &g = cp->fg[gi];
         const int ap = s.W.rspin[g.b2];
-        // union of the members' (lb, lk) blocks, in (lk, lb) order
-        std::vector<char> mark((size_t)nl * nl, 0);
         for (auto &m : g.mem)
             for (auto &a : blk[m.first]) mark[(size_t)a.lk * nl + a.lb] = 1;
         for (int lk = 0; lk < nl; ++lk)
@@ -456,25 +498,31 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
                 int al = s.lp_r[(size_t)lb * s.np + g.pout];
                 if (al < 0 || s.x_of_r[al] < 0) continue;  // out_left_i
                 if (s.su2 && !tri(spin2(s.right.q[ar]), ap, spin2(s.right.q[al]))) continue;
-                auto key = std::make_tuple(lk, g.pin, g.row);
-                auto it = panel_id.find(key);
-                int pid;
-                if (it == panel_id.end()) {
-                    pid = (int)panels.size();
-                    panel_id.emplace(key, pid);
-                    Panel pn;
-                    pn.lk = lk;
-                    pn.pin = g.pin;
-                    pn.row = g.row;
-                    pn.ar = ar;
-                    panels.push_back(pn);
-                } else {
-                    pid = it->second;
-                }
-                g.blk.push_back({lb, lk, ar, al, 0, 0, 0});
-                fb_panel[gi].push_back({pid, panels[pid].rows});
-                panels[pid].rows += s.lsize(lb);
+                cand[gi].push_back({lk, lb, ar, al});
             }
+    });
+    // 2. (sequential) panels keyed by (lk, input block, input state < 256), rows in group order
+    const int nrow_states = 256;
+    std::vector<int> panel_id((size_t)nl * s.np * nrow_states, -1);
+    std::vector<Panel> panels;
+    std::vector<std::vector<std::pair<int, int>>> fb_panel(ng);  // per group block: (panel, row)
+    for (int gi = 0; gi < ng; ++gi) {
+        FGroup &g = cp->fg[gi];
+        for (auto &c : cand[gi]) {
+            int &pid = panel_id[((size_t)c.lk * s.np + g.pin) * nrow_states + g.row];
+            if (pid < 0) {
+                pid = (int)panels.size();
+                Panel pn;
+                pn.lk = c.lk;
+                pn.pin = g.pin;
+                pn.row = g.row;
+                pn.ar = c.ar;
+                panels.push_back(pn);
+            }
+            g.blk.push_back({c.lb, c.lk, c.ar, c.al, 0, 0, 0});
+            fb_panel[gi].push_back({pid, panels[pid].rows});
+            panels[pid].rows += s.lsize(c.lb);
+        }
     }
     for (auto &pn : panels) {
         pn.lf = t.la_total;
@@ -482,11 +530,12 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
         pn.tf = t.t_total;
         t.t_total += (int64_t)pn.rows * s.rsize(pn.ar);
     }
-    std::vector<uint64_t> comb_key;  // (lk, lb): strips reading one LA_lk row range run together
-    std::vector<int> cnt, pos;
-    for (size_t gi = 0; gi < cp->fg.size(); ++gi) {
+    // 3. (parallel) block offsets and, per block, the number of member sources
+    std::vector<std::vector<int>> cnt(ng);
+    parallel_for(ng, [&](int gi) {
         FGroup &g = cp->fg[gi];
-        for (size_t k = 0; k < g.blk.size(); ++k) {
+        const size_t nb = g.blk.size();
+        for (size_t k = 0; k < nb; ++k) {
             FBlk &fb = g.blk[k];
             const Panel &pn = panels[fb_panel[gi][k].first];
             int row = fb_panel[gi][k].second;
@@ -494,41 +543,59 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
             fb.tf = pn.tf + row;
             fb.ld = pn.rows;
         }
-        if (!comb) continue;
-        // sources of each block, in member order: count, then fill
-        const size_t nb = g.blk.size();
+        if (!comb) return;
+        thread_local std::vector<int> slot;
+        slot.assign((size_t)nl * nl, -1);
         for (size_t k = 0; k < nb; ++k) slot[(size_t)g.blk[k].lk * nl + g.blk[k].lb] = (int)k;
-        cnt.assign(nb, 0);
+        cnt[gi].assign(nb, 0);
         for (auto &m : g.mem)
             for (auto &a : blk[m.first]) {
                 int k = slot[(size_t)a.lk * nl + a.lb];
-                if (k >= 0) ++cnt[k];
+                if (k >= 0) ++cnt[gi][k];
             }
-        pos.assign(nb, 0);
-        int base = (int)comb->srcs.size();
-        for (size_t k = 0; k < nb; ++k) {
-            pos[k] = base;
-            base += cnt[k];
-        }
-        comb->srcs.resize(base);
-        for (size_t k = 0; k < nb; ++k) {
-            const FBlk &fb = g.blk[k];
-            Strip st;
-            st.dst = fb.lf;
-            st.rows = s.lsize(fb.lb);
-            st.cols = s.lsize(fb.lk);
-            st.ldd = fb.ld;
-            st.s0 = pos[k];
-            st.s1 = pos[k] + cnt[k];
-            comb->strips.push_back(st);
-            comb_key.push_back(((uint64_t)fb.lb << 32) | (uint32_t)fb.lk);
-        }
-        for (auto &m : g.mem)
-            for (auto &a : blk[m.first]) {
-                int k = slot[(size_t)a.lk * nl + a.lb];
-                if (k >= 0) comb->srcs[pos[k]++] = {t.la_off[a.lk] + a.row, t.Mtot[a.lk], m.second};
+    });
+    std::vector<uint64_t> comb_key;  // (lk, lb): strips reading one LA_lk row range run together
+    if (comb) {
+        // 4. (sequential) strips and source ranges in group order
+        std::vector<int> gbase(ng + 1, (int)comb->srcs.size());
+        for (int gi = 0; gi < ng; ++gi) {
+            const FGroup &g = cp->fg[gi];
+            int base = gbase[gi];
+            for (size_t k = 0; k < g.blk.size(); ++k) {
+                const FBlk &fb = g.blk[k];
+                Strip st;
+                st.dst = fb.lf;
+                st.rows = s.lsize(fb.lb);
+                st.cols = s.lsize(fb.lk);
+                st.ldd = fb.ld;
+                st.s0 = base;
+                st.s1 = base + cnt[gi][k];
+                base = st.s1;
+                comb->strips.push_back(st);
+                comb_key.push_back(((uint64_t)fb.lb << 32) | (uint32_t)fb.lk);
             }
-        for (size_t k = 0; k < nb; ++k) slot[(size_t)g.blk[k].lk * nl + g.blk[k].lb] = -1;
+            gbase[gi + 1] = base;
+        }
+        comb->srcs.resize(gbase[ng]);
+        // 5. (parallel) sources of each block, in member order, into the group's range
+        parallel_for(ng, [&](int gi) {
+            const FGroup &g = cp->fg[gi];
+            const size_t nb = g.blk.size();
+            thread_local std::vector<int> slot, pos;
+            slot.assign((size_t)nl * nl, -1);
+            pos.assign(nb, 0);
+            int base = gbase[gi];
+            for (size_t k = 0; k < nb; ++k) {
+                slot[(size_t)g.blk[k].lk * nl + g.blk[k].lb] = (int)k;
+                pos[k] = base;
+                base += cnt[gi][k];
+            }
+            for (auto &m : g.mem)
+                for (auto &a : blk[m.first]) {
+                    int k = slot[(size_t)a.lk * nl + a.lb];
+                    if (k >= 0) comb->srcs[pos[k]++] = {t.la_off[a.lk] + a.row, t.Mtot[a.lk], m.second};
+                }
+        });
     }
     if (comb) {
         if (comb_key.size() != comb->strips.size()) throw std::runtime_error("combination strip list out of step");

@@ -151,3 +151,28 @@ class TwoSiteDMRG:
         for s in range(nsweeps):
             self.sweep(s)
         return self.energies[-1]
+
+
+def expval(engine, ham: Hamiltonian, mps, from_right=False):
+    """<mps|H|mps> / <mps|mps> by propagating a boundary through the whole MPS
+    with the engine's boundary steps (mp_tensors/mps_mpo_ops.h:42-103: the trace
+    of the fully propagated boundary plus E_core times the overlap)."""
+    L = ham.L
+    if not from_right:
+        b = engine.seed_boundary((0, 0))
+        for i in range(L):
+            b = engine.left_step(ham.symm, mps[i], b, ham.mpo[i])
+    else:
+        b = engine.seed_boundary(ham.right_end)
+        for i in range(L - 1, -1, -1):
+            b = engine.right_step(ham.symm, mps[i], b, ham.mpo[i])
+    if b.layout.aux != 1 or b.layout.nblocks != 1:
+        raise RuntimeError("the propagated boundary is not a scalar")
+    h = float(b.data[0])
+    # overlap: norm of the last tensor once all others are left-normalised
+    m = [t.copy() for t in mps]
+    for i in range(L - 1):
+        Rf = left_normalize_and_return(m[i])
+        multiply_from_left(m[i + 1], Rf)
+    nrm2 = m[-1].norm() ** 2
+    return h / nrm2 + ham.ecore

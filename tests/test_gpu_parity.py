@@ -138,3 +138,18 @@ def test_sigma_and_left_step_fuse_modes(oracle_engine, fuse):
         _sweep_pair(ham, oracle_engine, None, 48, 6, check)
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("symm", ["su2u1", "2u1"])
+def test_expval_gpu(fixtures, oracle_engine, gpu_engine, symm):
+    """expval through the GPU boundary steps equals the GPU sweep energy and the
+    oracle's expval of the same MPS."""
+    from qcmaquis_amd.sweep import Hamiltonian, TwoSiteDMRG, expval
+    fx = fixtures["benzene"]
+    ham = Hamiltonian(fx["L"], fx["nelec"], fx["spin"], [(tuple(k), v) for k, v in fx["integrals"]], symm=symm)
+    d = TwoSiteDMRG(ham, gpu_engine, 400)
+    E = d.run(2)
+    e_gpu = expval(gpu_engine, ham, d.mps)
+    e_ora = expval(oracle_engine, ham, d.mps, from_right=True)
+    assert abs(e_gpu - E) <= 1e-10 * abs(E), (e_gpu, E)
+    assert abs(e_gpu - e_ora) <= 1e-12 * abs(E), (e_gpu, e_ora)

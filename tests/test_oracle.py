@@ -166,3 +166,18 @@ def test_config0_dmrg_reaches_fci(oracle_engine):
     ham = Hamiltonian(L, L, 0, ints)
     E = TwoSiteDMRG(ham, oracle_engine, 400).run(3)
     assert abs(E - e0) < 1e-9, (E, e0)
+
+
+@pytest.mark.parametrize("symm", ["su2u1", "2u1"])
+def test_expval_equals_sweep_energy(fixtures, oracle_engine, symm):
+    """expval (mps_mpo_ops.h:42-103) of the converged, untruncated LiH MPS equals
+    the final sweep energy, from the left and from the right."""
+    from qcmaquis_amd.sweep import Hamiltonian, TwoSiteDMRG, expval
+    fx = fixtures["lih"]
+    ham = Hamiltonian(fx["L"], fx["nelec"], fx["spin"], [(tuple(k), v) for k, v in fx["integrals"]], symm=symm)
+    d = TwoSiteDMRG(ham, oracle_engine, 200)
+    E = d.run(2)
+    e_l = expval(oracle_engine, ham, d.mps)
+    e_r = expval(oracle_engine, ham, d.mps, from_right=True)
+    assert abs(e_l - E) <= 1e-10 * abs(E), (e_l, E)
+    assert abs(e_r - E) <= 1e-10 * abs(E), (e_r, E)
