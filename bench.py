@@ -173,19 +173,33 @@ def main():
     ch, prec_name = (r32, "mixed") if mixed_ok else (r64, "fp64")
     dt, avg, work = ch["dt"], ch["avg"], ch["work"]
 
-    # roofline of the dominant kernel of the reported form
+    # roofline of the dominant kernel of the reported form: the binding roof is the
+    # one with the larger lower bound on the time (algorithmic flops / MFMA peak vs
+    # algorithmic bytes / HBM peak)
     dom = max(range(2, 5), key=lambda i: avg[i])
-    kname = {2: "k_gemm<0> (T GEMM)", 3: "k_wapply<0> (W application)", 4: "k_gemm<1> (output GEMM)"}[dom]
-    if dom == 3:
-        ach = work[4] / (avg[3] * 1e-3) / 1e9
-        roof = dict(bound="hbm", kernel=kname, achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=ach / PEAK_HBM_GBS, traffic=None)
+    tname = "float" if prec_name == "mixed" else "double"
+    kname = {2: f"k_gemm<0, {tname}, {tname}>", 3: f"k_wapply<0, {tname}>", 4: f"k_gemm<1, {tname}, double>"}[dom]
+    flops = {2: work[0], 3: work[1], 4: work[2]}[dom]
+    nbytes = {2: work[3], 3: work[4], 4: work[5]}[dom]
+    peak_f = PEAK_FP64_TFLOPS if prec_name == "fp64" else PEAK_FP32_TFLOPS
+    t_s = avg[dom] * 1e-3
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "sigma_L24_m512_traffic.json")
+    if sec_src == "tests/golden/sectors_L24_m512.json" and a.L == 24 and os.path.exists(tfile):
+        rec = json.load(open(tfile)).get(kname)
+        if rec:
+            traffic = rec["traffic_bytes"]
+    if flops / (peak_f * 1e12) >= nbytes / (PEAK_HBM_GBS * 1e9) and dom != 3:
+        ach = flops / t_s / 1e12
+        roof = dict(bound="mfma", kernel=kname, achieved=ach, peak=peak_f, unit="TFLOP/s", frac=ach / peak_f,
+                    traffic=traffic)
     else:
-        fl = work[0] if dom == 2 else work[2]
-        peak = PEAK_FP64_TFLOPS if prec_name == "fp64" else PEAK_FP32_TFLOPS
-        ach = fl / (avg[dom] * 1e-3) / 1e12
-        roof = dict(bound="mfma", kernel=kname, achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                    traffic=None)
+        ach = nbytes / t_s / 1e9
+        roof = dict(bound="hbm", kernel=kname, achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
+                    frac=ach / PEAK_HBM_GBS, traffic=traffic)
+    roof["flops"] = flops
+    roof["algorithmic_bytes"] = nbytes
+    roof["avg_launch_ms"] = avg[dom]
 
     cpu, err = None, None
     if not a.no_cpu:
