@@ -171,19 +171,30 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     }
     int gi = (int)gl.groups.size();
     gl.groups.push_back(g);
-    for (int ks = 0; ks < g.split; ++ks)
-        for (int m = 0; m < mt; ++m)
-            for (int n = 0; n < nt; ++n) gl.tiles.push_back({gi, m, n, ks});
+    // consecutive tiles share an operand panel: the A row block (runs along n), or the
+    // B column block when the group has a single column tile (runs along m)
+    if (nt == 1) {
+        for (int ks = 0; ks < g.split; ++ks)
+            for (int m = 0; m < mt; ++m) gl.tiles.push_back({gi, m, 0, ks});
+    } else {
+        for (int ks = 0; ks < g.split; ++ks)
+            for (int m = 0; m < mt; ++m)
+                for (int n = 0; n < nt; ++n) gl.tiles.push_back({gi, m, n, ks});
+    }
 }
 
 constexpr int kXcds = 8;
 
-std::vector<GemmTile> xcd_order(const std::vector<GemmTile> &tiles) {
-    // chains: consecutive tiles with the same (group, K chunk, row block)
+std::vector<GemmTile> xcd_order(const std::vector<GemmTile> &tiles, const std::vector<GemmGroup> &groups) {
+    // chains: consecutive tiles of one (group, K chunk) sharing an operand panel --
+    // the same row block, or, in groups with a single column tile, the same column
+    // block (cut into chains of at most 16 tiles so the chains still balance)
     std::vector<std::pair<size_t, size_t>> chains;
     for (size_t i = 0; i < tiles.size();) {
         size_t j = i + 1;
-        while (j < tiles.size() && tiles[j].g == tiles[i].g && tiles[j].ks == tiles[i].ks && tiles[j].mt == tiles[i].mt)
+        const bool along_m = groups[tiles[i].g].N <= groups[tiles[i].g].tn;
+        while (j < tiles.size() && tiles[j].g == tiles[i].g && tiles[j].ks == tiles[i].ks &&
+               (along_m ? (j - i < 16) : tiles[j].mt == tiles[i].mt))
             ++j;
         chains.push_back({i, j});
         i = j;
@@ -803,7 +814,7 @@ void order_tiles(GemmList &gl) {
     std::vector<GemmTile> t;
     for (auto &x : gl.tiles)
         if (x.g >= 0) t.push_back(x);
-    gl.tiles = xcd_order(t);
+    gl.tiles = xcd_order(t, gl.groups);
 }
 
 // ============================================================ sigma

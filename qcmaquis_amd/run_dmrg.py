@@ -1,7 +1,7 @@
 """Two-site DMRG sweeps on one GPU.
 
     python -m qcmaquis_amd.run_dmrg --L 24 --ramp 64,128,256,512,512,512 \
-        [--fcidump FILE] [--prec fp64|mixed|both] [--jd-maxiter 8] [--out result.json]
+        [--fcidump FILE] [--prec fp64|mixed|both|compare] [--jd-maxiter 8] [--out result.json]
 
 Without --fcidump the synthetic L-orbital half-filled singlet Hamiltonian
 (qcmaquis_amd.integrals.synthetic, seed 1000 + L) is used.  The sweep order,
@@ -9,7 +9,10 @@ truncation and Davidson semantics follow the reference's two-site optimizer;
 each local problem is solved by the GPU Jacobi-Davidson (mixed: FP32 sigma
 vectors with one FP64 defect-correction sigma per solve).  With --prec both
 the two forms run from the same initial MPS and the energy differences per
-sweep are reported.
+sweep are reported.  With --prec compare the sweep runs in FP64 and every
+local problem is solved a second time, in the mixed form, on the same plan
+from the same start vector: the largest |theta_mixed - theta_fp64| per sweep
+measures the mixed form's energy error on the sweep's own trajectory.
 """
 import argparse
 import json
@@ -19,6 +22,51 @@ import time
 from .desc import MQ_PREC_FP32, MQ_PREC_FP64
 from .integrals import read_fcidump, synthetic
 from .sweep import Hamiltonian, TwoSiteDMRG
+
+
+class _CompareEngine:
+    """GPU engine whose local solves run in FP64 (driving the sweep) and, on the
+    same plan and start vector, in the mixed form; records |theta_mixed -
+    theta_fp64| per solve."""
+
+    def __init__(self, eng):
+        self.eng = eng
+        self.diffs = []
+
+    def __getattr__(self, k):
+        return getattr(self.eng, k)
+
+    def jd(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=()):
+        sp = self.eng.sigma_plan(symm, psi, L, R, W)
+        try:
+            x = self.eng.to_dev(psi.to_flat())
+            t64, u, its = sp.jd(x, max_iter, tol, ortho, prec=MQ_PREC_FP64)
+            t32, _, _ = sp.jd(x, max_iter, tol, ortho, prec=MQ_PREC_FP32)
+            self.diffs.append(abs(t32 - t64))
+            return t64, u.cpu().numpy(), its
+        finally:
+            sp.close()
+
+
+def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print):
+    """FP64 sweeps; every local problem is also solved in the mixed form."""
+    from .gpu import GpuEngine
+    eng = _CompareEngine(GpuEngine(0, prec=MQ_PREC_FP64))
+    try:
+        d = TwoSiteDMRG(ham, eng, ramp[0], jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed)
+        out = []
+        t0 = time.perf_counter()
+        for s, m in enumerate(ramp):
+            d.m = m
+            eng.diffs = []
+            E = d.sweep(s)
+            out.append(dict(sweep=s, m=m, energy=E, max_abs_dtheta=max(eng.diffs), mean_abs_dtheta=sum(eng.diffs) /
+                            len(eng.diffs), solves=len(eng.diffs), seconds=time.perf_counter() - t0))
+            log(f"  sweep {s} m {m} E(fp64) {E:.12f} max |theta_mixed - theta_fp64| {max(eng.diffs):.3e} Eh "
+                f"over {len(eng.diffs)} solves, t {time.perf_counter() - t0:.1f}s")
+        return out
+    finally:
+        eng.eng.close()
 
 
 def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print):
@@ -46,7 +94,7 @@ def main(argv=None):
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--fcidump")
     ap.add_argument("--ramp", default="64,64,64,64")
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "both", "compare"])
     ap.add_argument("--jd-maxiter", type=int, default=8)
     ap.add_argument("--jd-tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=42)
@@ -58,6 +106,13 @@ def main(argv=None):
     else:
         ham = Hamiltonian(a.L, a.L, 0, synthetic(a.L, seed=1000 + a.L))
     ramp = [int(v) for v in a.ramp.split(",")]
+    if a.prec == "compare":
+        res = run_compare(ham, ramp, a.jd_maxiter, a.jd_tol, a.seed)
+        rec = dict(L=ham.L, ramp=ramp, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, mode="compare", results=res)
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(rec, f, indent=1)
+        return rec
     precs = {"fp64": [("fp64", MQ_PREC_FP64)], "mixed": [("mixed", MQ_PREC_FP32)],
              "both": [("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)]}[a.prec]
     res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed) for name, p in precs}

@@ -86,6 +86,43 @@ def test_jacobi_davidson_parity(fixtures, oracle_engine, gpu_engine):
     _sweep_pair(ham, oracle_engine, gpu_engine, 60, 4, check)
 
 
+def test_jacobi_davidson_long_solves(fixtures, oracle_engine, gpu_engine):
+    """Long solves (up to 40 iterations, no restart) from a random start: the GPU
+    eigenvalue equals the oracle's and the lowest eigenvalue of the explicit
+    H_eff, in FP64 and -- after the FP64 defect correction -- in the mixed form."""
+    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    from qcmaquis_amd.integrals import synthetic
+    from qcmaquis_amd.sweep import Hamiltonian
+    ham = Hamiltonian(8, 8, 0, synthetic(8, seed=1008))
+
+    def check(d, site):
+        psi, L, R, W = d.site_problem(site)
+        n = psi.to_flat().size
+        if n < 200:
+            return
+        H = np.zeros((n, n))
+        for k in range(n):
+            e = np.zeros(n)
+            e[k] = 1.0
+            H[:, k], _ = oracle_engine.sigma(ham.symm, psi, L, R, W, x=e)
+        lam = np.linalg.eigvalsh(0.5 * (H + H.T))[0]
+        Lg = gpu_engine.upload_boundary(L.layout, L.data)
+        Rg = gpu_engine.upload_boundary(R.layout, R.data)
+        rng = np.random.default_rng(site)
+        x = rng.standard_normal(n)
+        sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+        try:
+            xd = gpu_engine.to_dev(x)
+            t64, _, i64 = sp.jd(xd, 40, 1e-11, prec=MQ_PREC_FP64)
+            t32, _, _ = sp.jd(xd, 40, 1e-11, prec=MQ_PREC_FP32)
+        finally:
+            sp.close()
+        assert abs(t64 - lam) <= 1e-9 * max(1.0, abs(lam)), (site, t64, lam, i64)
+        assert abs(t32 - lam) <= 1e-8, (site, t32, lam)
+
+    _sweep_pair(ham, oracle_engine, gpu_engine, 48, 6, check)
+
+
 @pytest.mark.parametrize("name", ["h2", "lih", "benzene"])
 def test_golden_energies_gpu(fixtures, gpu_engine, name):
     from qcmaquis_amd.sweep import TwoSiteDMRG
