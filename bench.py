@@ -73,7 +73,10 @@ def main():
     ap.add_argument("--jd-iters", type=int, default=10)
     ap.add_argument("--L", type=int, default=24)
     ap.add_argument("--m", type=int, default=512)
-    ap.add_argument("--sectors", default=os.path.join(ROOT, "tests", "golden", "sectors_L24_m512.json"))
+    ap.add_argument("--sectors", default=None,
+                    help="recorded bond sectors (default: tests/golden/sectors_L24_m512.json for SU2U1 L=24)")
+    ap.add_argument("--symm", default="su2u1", choices=["su2u1", "2u1"])
+    ap.add_argument("--decay", type=float, default=None, help="integral decay length (exp(-|p-q|/decay))")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-delta-e", action="store_true", help="skip the converged-solve energy comparison")
@@ -106,12 +109,14 @@ def main():
         del src, dst
     t0 = time.perf_counter()
     sectors, sec_src = None, f"degeneracy-proportional sectors capped at m={a.m}"
+    if a.sectors is None and a.symm == "su2u1" and a.L == 24 and a.decay is None:
+        a.sectors = os.path.join(ROOT, "tests", "golden", "sectors_L24_m512.json")
     if a.sectors and os.path.exists(a.sectors):
         rec, sectors = load_sectors(a.sectors)
-        if rec["L"] != a.L:
-            raise SystemExit(f"{a.sectors} is for L={rec['L']}, not L={a.L}")
+        if rec["L"] != a.L or rec.get("symm", "su2u1") != a.symm or rec.get("decay") != a.decay:
+            raise SystemExit(f"{a.sectors} does not describe L={a.L} {a.symm} decay={a.decay}")
         sec_src = os.path.relpath(a.sectors, ROOT)
-    sp_ = centre_problem(eng, a.L, m=a.m, sectors=sectors)
+    sp_ = centre_problem(eng, a.L, m=a.m, sectors=sectors, symm=a.symm, decay=a.decay)
     ham, psi, Lb, Rb, W, site = sp_.ham, sp_.psi, sp_.L, sp_.R, sp_.W, sp_.site
     log(f"setup {time.perf_counter() - t0:.1f}s: site {site}, psi {psi.to_flat().size}, W {W.row_dim}x{W.col_dim}")
     t0 = time.perf_counter()
@@ -225,7 +230,9 @@ def main():
         "dtype": "f32" if prec_name == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"SU2U1 two-site sigma, L={a.L} N={a.L} S=0 m={a.m}, centre bond (sites {site},{site + 1})",
+            "workload": f"{'SU2U1' if a.symm == 'su2u1' else 'TwoU1'} two-site sigma, L={a.L} N={a.L} S=0 "
+                        f"m={a.m}{'' if a.decay is None else f' decay={a.decay:g}'}, centre bond "
+                        f"(sites {site},{site + 1})",
             "sectors": sec_src,
             "psi_len": int(sp.n),
             "mpo_bond": [int(W.row_dim), int(W.col_dim)],

@@ -32,14 +32,14 @@ class SiteProblem:
         self.ham, self.psi, self.L, self.R, self.W, self.site, self.mps = ham, psi, L, R, W, site, mps
 
 
-def centre_problem(engine, L, m=None, sectors=None, site=None, seed=42, ham=None):
+def centre_problem(engine, L, m=None, sectors=None, site=None, seed=42, ham=None, symm="su2u1", decay=None):
     """Build the SiteProblem of the two-site problem at `site` (default
     L/2 - 1).  `sectors` is a list of Index (one per bond, L + 1 entries);
     when None the degeneracy-proportional structure capped at m is used."""
     if ham is None:
-        ham = Hamiltonian(L, L, 0, synthetic(L, seed=1000 + L))
+        ham = Hamiltonian(L, L, 0, synthetic(L, seed=1000 + L, decay=decay), symm=symm)
     if sectors is None:
-        sectors = bounded_sectors(L, ham.phys, ham.right_end, m)
+        sectors = bounded_sectors(L, ham.phys, ham.right_end, m, su2=ham.su2)
     if len(sectors) != L + 1:
         raise ValueError(f"sector structure has {len(sectors)} bonds, expected {L + 1}")
     mps = random_mps_on(sectors, ham.phys, seed=seed)
@@ -54,5 +54,5 @@ def centre_problem(engine, L, m=None, sectors=None, site=None, seed=42, ham=None
         multiply_from_left(mps[i + 1], Rf)
         Lb = engine.left_step(ham.symm, mps[i], Lb, ham.mpo[i])
     W = ham.ts_mpo(c)
-    psi = make_twosite_mps(mps[c], mps[c + 1])
+    psi = make_twosite_mps(mps[c], mps[c + 1], su2=ham.su2)
     return SiteProblem(ham, psi, Lb, R, W, c, mps)
