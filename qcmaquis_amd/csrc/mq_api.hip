@@ -53,7 +53,7 @@ struct DevGemm {
     int ntiles = 0, nsplit = 0;
     void upload(const GemmList &g, hipStream_t s) {
         for (auto &x : g.groups)
-            if ((x.tm != 64 && x.tm != 128) || (x.tn != 64 && x.tn != 128))
+            if ((x.tm != 64 && x.tm != 128) || x.tn < 16 || x.tn > 128 || x.tn % 16 != 0)
                 throw std::runtime_error("GEMM group with an unsupported tile shape");
         for (auto &t : g.tiles)
             if (t.g >= (int)g.groups.size()) throw std::runtime_error("GEMM tile outside the group list");

@@ -80,9 +80,9 @@ __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips
 }
 
 // ------------------------------------------------------------------ grouped GEMM (MFMA)
-// Block tile TM x TN (64 or 128 each, chosen per group), K step 16, 256
-// threads = 4 waves in a 2x2 grid, each wave owning a TM/2 x TN/2 sub-tile
-// computed with 16x16x4 MFMA products (v_mfma_f64_16x16x4 for FP64,
+// Block tile TM x TN (TM 64 or 128, TN a multiple of 16 up to 128, chosen per
+// group to limit padding), K step 16, 256 threads = 4 waves stacked along M,
+// each wave owning a TM/4 x TN sub-tile computed with 16x16x4 MFMA products (v_mfma_f64_16x16x4 for FP64,
 // v_mfma_f32_16x16x4 for FP32 operands).  The MFMA is fed with (B^T, A^T) so
 // that its output lanes run along the rows of the column-major C, which makes
 // the stores 128-byte segments.  Operand layout of both instructions: the
@@ -186,14 +186,15 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
                                           const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
                                           double *lds) {
     typedef Mfma<T> MF;
-    constexpr int MI = TM / 32, NJ = TN / 32;  // 16x16 MFMA blocks per wave
+    // the four waves split the tile along M; each holds TM/4 rows x TN columns
+    constexpr int MI = TM / 64, NJ = TN / 16;  // 16x16 MFMA blocks per wave
     T(*As)[TK][TM + 2] = reinterpret_cast<T(*)[TK][TM + 2]>(lds);
     T(*Bs)[TK][TN + 2] = reinterpret_cast<T(*)[TK][TN + 2]>(reinterpret_cast<T *>(lds) + 2 * TK * (TM + 2));
     const int m0 = t.mt * TM, n0 = t.nt * TN;
     const int kbeg = t.ks * g.kchunk;
     const int kend = min(g.K, kbeg + g.kchunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TN / 2);
+    const int wm = wave * (TM / 4);
     const int lr = lane & 15, lq = lane >> 4;
     typename MF::acc acc[MI][NJ];
 #pragma unroll
@@ -217,7 +218,7 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
             for (int i = 0; i < MI; ++i) av[i] = As[buf][kk + lq][wm + 16 * i + lr];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
-                T bj = Bs[buf][kk + lq][wn + 16 * j + lr];
+                T bj = Bs[buf][kk + lq][16 * j + lr];
 #pragma unroll
                 for (int i = 0; i < MI; ++i) acc[i][j] = MF::run(bj, av[i], acc[i][j]);
             }
@@ -237,7 +238,7 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
             for (int j = 0; j < NJ; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int gn = n0 + wn + 16 * j + MF::dcol(r, lq);
+                    const int gn = n0 + 16 * j + MF::dcol(r, lq);
                     if (gn < g.N) out[gm + gn * ld] = (TO)acc[i][j][r];
                 }
         }
@@ -252,10 +253,27 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
             for (int j = 0; j < NJ; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int gn = n0 + wn + 16 * j + MF::dcol(r, lq);
+                    const int gn = n0 + 16 * j + MF::dcol(r, lq);
                     if (gn < g.N) out[gm + gn * ld] = acc[i][j][r];
                 }
         }
+    }
+}
+
+// Column tile widths are multiples of 16 up to 128.
+template <int TM, class T, class TO>
+__device__ __forceinline__ void gemm_tile_n(const GemmGroup &g, const GemmTile &t, const T *__restrict__ A,
+                                            const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
+                                            double *lds) {
+    switch (g.tn) {
+        case 16: gemm_tile<TM, 16, T, TO>(g, t, A, B, C, W, lds); break;
+        case 32: gemm_tile<TM, 32, T, TO>(g, t, A, B, C, W, lds); break;
+        case 48: gemm_tile<TM, 48, T, TO>(g, t, A, B, C, W, lds); break;
+        case 64: gemm_tile<TM, 64, T, TO>(g, t, A, B, C, W, lds); break;
+        case 80: gemm_tile<TM, 80, T, TO>(g, t, A, B, C, W, lds); break;
+        case 96: gemm_tile<TM, 96, T, TO>(g, t, A, B, C, W, lds); break;
+        case 112: gemm_tile<TM, 112, T, TO>(g, t, A, B, C, W, lds); break;
+        default: gemm_tile<TM, 128, T, TO>(g, t, A, B, C, W, lds); break;
     }
 }
 
@@ -268,17 +286,10 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmGroup *__restrict__ grou
     const GemmTile t = tiles[blockIdx.x];
     if (t.g < 0) return;  // padding tile of the XCD-ordered list
     const GemmGroup g = groups[t.g];
-    if (g.tm == 128) {
-        if (g.tn == 128)
-            gemm_tile<128, 128, T, TO>(g, t, A, B, C, W, lds);
-        else
-            gemm_tile<128, 64, T, TO>(g, t, A, B, C, W, lds);
-    } else {
-        if (g.tn == 128)
-            gemm_tile<64, 128, T, TO>(g, t, A, B, C, W, lds);
-        else
-            gemm_tile<64, 64, T, TO>(g, t, A, B, C, W, lds);
-    }
+    if (g.tm == 128)
+        gemm_tile_n<128, T, TO>(g, t, A, B, C, W, lds);
+    else
+        gemm_tile_n<64, T, TO>(g, t, A, B, C, W, lds);
 }
 
 // C = sum over splits of the partial tiles, in split order (deterministic),

@@ -148,12 +148,24 @@ void parallel_for(int n, F &&f) {
     if (err) std::rethrow_exception(err);
 }
 
+// Column tile width: a multiple of 16 up to 128 minimising the padded columns
+// plus one 16-column block of fixed cost per tile (ties go to the wider tile).
+int column_tile(int N) {
+    if (N <= 128) return std::max(16, ceil_div(N, 16) * 16);
+    int best = 128, cost = ceil_div(N, 128) * (128 + 16);
+    for (int tn = 112; tn >= 64; tn -= 16) {
+        const int c = ceil_div(N, tn) * (tn + 16);
+        if (c < cost) best = tn, cost = c;
+    }
+    return best;
+}
+
 void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     (void)tn;
     g.split = 1;
     g.kchunk = std::max(g.K, 1);
     g.tm = g.M > 64 ? 128 : 64;
-    g.tn = g.N > 64 ? 128 : 64;
+    g.tn = column_tile(g.N);
     (void)tm;
     if (g.M == 0 || g.N == 0) return;
     int mt = ceil_div(g.M, g.tm), nt = ceil_div(g.N, g.tn);

@@ -51,7 +51,7 @@ def gemm(groups, tiles, A, B, C):
     for gi, g in enumerate(groups):
         M, N, K = int(g["M"]), int(g["N"]), int(g["K"])
         tm, tn = int(g["tm"]), int(g["tn"])
-        assert tm in (64, 128) and tn in (64, 128)
+        assert tm in (64, 128) and tn in range(16, 129, 16)
         mt, nt = -(-M // tm), -(-N // tn)
         for ks in range(int(g["split"])):
             for n in range(nt):

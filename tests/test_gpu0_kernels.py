@@ -6,7 +6,9 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(1, 1, 1), (16, 16, 4), (64, 64, 16), (65, 63, 17), (5, 130, 3), (200, 7, 1000), (37, 41, 0),
           (64, 128, 32), (70, 129, 20), (300, 65, 7),
-          (128, 96, 5000), (966, 101, 9000)]
+          (128, 96, 5000), (966, 101, 9000),
+          # one shape per column tile width (16 .. 128 in steps of 16)
+          (90, 17, 33), (150, 47, 30), (130, 81, 50), (129, 112, 31), (100, 200, 20), (64, 987, 98)]
 
 
 @pytest.mark.parametrize("transA,transB", [(False, False), (True, False), (False, True)])
