@@ -20,6 +20,12 @@
  *       dmrg/framework/dmrg/optimize/ietl_jacobi_davidson.h:18-77 with
  *       ietl::jacobi_davidson::calculate_eigenvalue, dmrg/framework/ietl/jacobi.h:364-451
  *     -> mq_jacobi_davidson
+ *   solve_ietl_davidson(SiteProblem&, MPSTensor const&, BaseParameters&)
+ *       dmrg/framework/dmrg/optimize/ietl_davidson.h:121-160 with
+ *       ietl::davidson::calculate_eigenvalue, optimize/davidson.h:64-161, and
+ *       MultDiagonal (ietl_davidson.h:78-118), whose diagonal is
+ *       contraction::SU2::diagonal_hamiltonian, non-abelian/h_diag.hpp:120-152
+ *     -> mq_diag_plan_create + mq_diagonal + mq_davidson
  *
  * All data crosses the boundary as flat integer metadata plus FP64 arrays.
  * Every function returns MQ_OK (0) or a negative error code; a message for
@@ -202,6 +208,32 @@ int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R,
                        const double *const *ortho, int n_ortho,
                        int *iterations);
 
+/* Plan for the diagonal of H_eff (SU2U1 only; other symmetries fail with
+ * MQ_ERR_PLAN, as MultDiagonal throws for them, ietl_davidson.h:60-76).
+ * desc is the sigma plan's descriptor. */
+int mq_diag_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out);
+/* diag (device, vec_len, written) = diagonal of H_eff in the left-paired
+ * layout, as contraction::SU2::diagonal_hamiltonian computes it
+ * (non-abelian/h_diag.hpp:20-152): from the diagonals of the stored
+ * L[b1](lc,lc) and R[b2](rc,rc) blocks and the diagonal entries of the
+ * phys-diagonal operator blocks.  Boundary slices that are not stored
+ * (hermitian partners) contribute nothing.  Asynchronous on the plan's
+ * stream. */
+int mq_diagonal(mq_plan *plan, const double *L, const double *R, double *diag);
+/* Davidson with the diagonal preconditioner (SU2U1 sigma plans): Gram-Schmidt
+ * with one refinement, lowest Ritz pair of the Rayleigh matrix, residual
+ * test |r| <= tol*|theta| or |r| < tol, correction
+ * t = (r - (V.r/(V.V)) V) / (theta - D) with V = u / (theta - D)
+ * (elements with theta == D are left undivided), restart to the two lowest
+ * Ritz vectors when the basis reaches 20, at most max_iter sigma vectors.
+ * diag: device, from mq_diagonal.  x0, u, ortho, prec, iterations as for
+ * mq_jacobi_davidson; the caller orthogonalises ortho beforehand
+ * (ietl_davidson.h:126-129). */
+int mq_davidson(mq_plan *plan, const double *L, const double *R,
+                const double *diag, const double *x0, double *u, double *theta,
+                int max_iter, double tol, int prec,
+                const double *const *ortho, int n_ortho, int *iterations);
+
 /* -- diagnostics (no reference counterpart) ------------------------------ */
 /* C(M x N, ldc) = op(A) op(B) with the library's grouped FP64 GEMM kernel on
  * device arrays (column-major; transA: A stored K x M, transB: B stored
@@ -215,7 +247,7 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
 /* Builds the same task lists the device plans execute, on the host only, so
  * that schedules, FLOP counts and workspace sizes can be inspected (and the
  * task lists checked) on machines without a GPU.  kind: 0 sigma, 1 left
- * step, 2 right step, plus MQ_HOST_FUSE(v) to build with MQ_OPT_FUSE = v
+ * step, 2 right step, 3 diagonal, plus MQ_HOST_FUSE(v) to build with MQ_OPT_FUSE = v
  * (default 1).  Arrays stay owned by the handle. */
 #define MQ_HOST_FUSE(v) (((v) + 1) << 8)
 typedef struct mq_host_plan mq_host_plan;

@@ -293,4 +293,146 @@ int orc_jacobi_davidson(const mq_site_desc *d, const double *L, const double *R,
     }
 }
 
+// Diagonal of H_eff in the flat left-paired layout, non-abelian/h_diag.hpp:120-152.
+int orc_diagonal(const mq_site_desc *d, const double *L, const double *R, double *out) {
+    try {
+        Problem p = make_problem(d);
+        Boundary Lb = to_boundary(d->L, L), Rb = to_boundary(d->R, R);
+        BlockMatrix dg = diagonal_hamiltonian(p, Lb, Rb);
+        mps_to_flat(p.phys, p.left, p.right, dg, out);
+        return 0;
+    } catch (std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Davidson with the diagonal preconditioner: optimize/davidson.h:64-161 driven
+// as solve_ietl_davidson does (optimize/ietl_davidson.h:121-160), with
+// MultDiagonal::precondition (:96-104).  ortho: vectors projected out as
+// SingleSiteVS::project does (the caller applies the Gram-Schmidt of
+// ietl_davidson.h:126-129 beforehand).  Eigenvectors of the Rayleigh matrix:
+// ascending eigenvalues, each column with its largest component positive.
+int orc_davidson(const mq_site_desc *d, const double *L, const double *R, const double *x0, double *u_out,
+                 double *theta_out, int max_iter, double tol, const double *const *ortho, int n_ortho,
+                 int *iters_out, double *flops) {
+    try {
+        Problem p = make_problem(d);
+        Boundary Lb = to_boundary(d->L, L), Rb = to_boundary(d->R, R);
+        int64_t n = mps_len(p.phys, p.left, p.right);
+        std::vector<double> D(n);
+        mps_to_flat(p.phys, p.left, p.right, diagonal_hamiltonian(p, Lb, Rb), D.data());
+        Counter c;
+        auto mult = [&](const std::vector<double> &x, std::vector<double> &y) {
+            BlockMatrix ket = mps_from_flat(p.phys, p.left, p.right, x.data());
+            g_counter = flops ? &c : nullptr;
+            BlockMatrix s = site_hamil2(p, ket, Lb, Rb, -1);
+            g_counter = nullptr;
+            y.assign(n, 0.0);
+            mps_to_flat(p.phys, p.left, p.right, s, y.data());
+        };
+        std::vector<std::vector<double>> O;
+        for (int i = 0; i < n_ortho; ++i) O.emplace_back(ortho[i], ortho[i] + n);
+        auto project = [&](std::vector<double> &t) {
+            for (auto &o : O) {
+                double f = dotv(o, t) / dotv(o, o);
+                for (int64_t k = 0; k < n; ++k) t[k] -= f * o[k];
+            }
+        };
+        auto norm = [&](const std::vector<double> &v) { return std::sqrt(dotv(v, v)); };
+        // MultDiagonal::mult_diag: x /= (theta - D) where that difference is nonzero
+        auto mult_diag = [&](double theta, std::vector<double> &x) {
+            for (int64_t k = 0; k < n; ++k)
+                if (std::fabs(theta - D[k]) != 0.0) x[k] /= (theta - D[k]);
+        };
+        std::vector<double> t(x0, x0 + n), u, r;
+        project(t);
+        std::vector<std::vector<double>> V, VA;
+        double theta = 0;
+        int it = 0;
+        const double kappa = 0.25;
+        while (true) {
+            double tau = norm(t);
+            for (size_t i = 0; i < V.size(); ++i) {
+                double f = dotv(V[i], t);
+                for (int64_t k = 0; k < n; ++k) t[k] -= f * V[i][k];
+            }
+            if (norm(t) < kappa * tau)
+                for (size_t i = 0; i < V.size(); ++i) {
+                    double f = dotv(V[i], t);
+                    for (int64_t k = 0; k < n; ++k) t[k] -= f * V[i][k];
+                }
+            project(t);
+            double nt = norm(t);
+            for (auto &v : t) v /= nt;
+            V.push_back(t);
+            VA.emplace_back();
+            mult(V.back(), VA.back());
+            const int dim = (int)V.size();
+            std::vector<double> M((size_t)dim * dim);
+            for (int i = 0; i < dim; ++i)
+                for (int j = i; j < dim; ++j) {
+                    M[(size_t)j * dim + i] = dotv(V[i], VA[j]);
+                    M[(size_t)i * dim + j] = M[(size_t)j * dim + i];
+                }
+            std::vector<double> w, Z;
+            sym_eig(dim, M, w, Z);
+            std::vector<int> order(dim);
+            for (int i = 0; i < dim; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return w[x] < w[y]; });
+            // Mevecs(i, j): component i of the j-th lowest eigenvector
+            std::vector<double> E((size_t)dim * dim);
+            for (int j = 0; j < dim; ++j) {
+                const double *z = Z.data() + (size_t)order[j] * dim;
+                int big = 0;
+                for (int i = 1; i < dim; ++i)
+                    if (std::fabs(z[i]) > std::fabs(z[big])) big = i;
+                const double sg = z[big] < 0 ? -1.0 : 1.0;
+                for (int i = 0; i < dim; ++i) E[(size_t)j * dim + i] = sg * z[i];
+            }
+            std::vector<std::vector<double>> Vp = V, VAp = VA;
+            for (int j = 0; j < dim; ++j) {
+                std::fill(V[j].begin(), V[j].end(), 0.0);
+                std::fill(VA[j].begin(), VA[j].end(), 0.0);
+                for (int i = 0; i < dim; ++i) {
+                    const double e = E[(size_t)j * dim + i];
+                    for (int64_t k = 0; k < n; ++k) {
+                        V[j][k] += Vp[i][k] * e;
+                        VA[j][k] += VAp[i][k] * e;
+                    }
+                }
+            }
+            theta = w[order[0]];
+            r.assign(n, 0.0);
+            for (int64_t k = 0; k < n; ++k) r[k] = VA[0][k] - V[0][k] * theta;
+            u = V[0];
+            ++it;
+            const double rn = norm(r);
+            if (rn <= tol * std::fabs(theta) || rn < tol || it >= max_iter) break;
+            // MultDiagonal::precondition
+            std::vector<double> Vcpy = V[0];
+            mult_diag(theta, Vcpy);
+            const double a = dotv(Vcpy, r), b = dotv(V[0], Vcpy);
+            for (int64_t k = 0; k < n; ++k) r[k] -= a / b * V[0][k];
+            mult_diag(theta, r);
+            std::swap(t, r);
+            nt = norm(t);
+            for (auto &v : t) v /= nt;
+            if (V.size() >= 20) {
+                V.resize(2);
+                VA.resize(2);
+            }
+        }
+        std::copy(u.begin(), u.end(), u_out);
+        *theta_out = theta;
+        if (iters_out) *iters_out = it;
+        if (flops) *flops = c.flops;
+        return 0;
+    } catch (std::exception &e) {
+        g_counter = nullptr;
+        g_err = e.what();
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -868,4 +868,73 @@ Boundary overlap_mpo_right_step(const Problem &p, const BlockMatrix &ket_lp, con
     return ret;
 }
 
+// non-abelian/h_diag.hpp:20-117 (lbtm_diag_kernel) and :120-152
+// (diagonal_hamiltonian): the diagonal of H_eff built from the diagonals of
+// the stored boundary blocks L[b1](lc,lc), R[b2](rc,rc) and the diagonal
+// entries (row == col) of the phys-diagonal operator blocks.  Boundary slices
+// that are not stored (hermitian partners) contribute nothing, as in the
+// reference.  The result holds one (rc, rc) block per right charge reached by
+// some R[b2].
+BlockMatrix diagonal_hamiltonian(const Problem &p, const Boundary &L, const Boundary &R) {
+    if (p.symm != MQ_SYMM_SU2U1) throw std::runtime_error("Davidson only implemented for spin-adapted Hamiltonians");
+    const MPO &mpo = p.mpo;
+    Index right_i = p.right, out_left_i = product(p.phys, p.left);
+    common_subset(out_left_i, right_i);
+    ProductBasis left_pb(p.phys, p.left);
+    BlockMatrix ret;
+    for (int b2 = 0; b2 < (int)R.size(); ++b2) {
+        BlockMatrix lb2;
+        for (int b1 : mpo.col_rows[b2]) {
+            const std::vector<Term> &terms = mpo.at.at({b1, b2});
+            for (const Term &term : terms) {
+                const Op &W = mpo.ops[term.op];
+                const int a = mpo.left_spin[b1], k = W.spin, ap = mpo.right_spin[b2];
+                for (size_t block = 0; block < right_i.size(); ++block) {
+                    const Charge in_charge = right_i[block].first;
+                    const size_t o = lb2.find_or_insert(in_charge, in_charge, out_left_i[block].second,
+                                                        right_i[block].second);
+                    for (size_t s = 0; s < p.phys.size(); ++s) {
+                        const Charge phys_charge = p.phys[s].first;
+                        const size_t l = p.left.position(fuse(in_charge, -phys_charge));
+                        if (l == p.left.size()) continue;
+                        const Charge lc = p.left[l].first;
+                        const size_t lb = L[b1].find(lc, lc);
+                        if (lb == L[b1].n()) continue;
+                        const int ls = p.left[l].second;
+                        std::vector<double> left_diagonal(ls);
+                        for (int i = 0; i < ls; ++i) left_diagonal[i] = L[b1].b[lb].m(i, i);
+                        const int left_offset = left_pb(phys_charge, lc);
+                        for (const OpBlock &wb : W.blocks) {
+                            if (phys_charge != wb.in || wb.in != wb.out) continue;
+                            const int i = spin(lc), ip = spin(in_charge), j = spin(lc), jp = spin(in_charge);
+                            const int two_sp = std::abs(i - ip), two_s = std::abs(j - jp);
+                            const double prefactor = std::sqrt((ip + 1.) * (j + 1.) / ((i + 1.) * (jp + 1.))) * term.scale;
+                            const double couplings[2] = {
+                                prefactor * mod_coupling(j, two_s, jp, a, k, ap, i, two_sp, ip),
+                                prefactor * mod_coupling(j, 2, jp, a, k, ap, i, 2, ip)};
+                            for (const Entry &e : wb.e) {
+                                if (e.row != e.col) continue;
+                                const double alfa_t = e.coeff * couplings[e.rspin == 2 ? 1 : 0];
+                                Matrix &out = lb2.b[o].m;
+                                for (int col = 0; col < right_i[block].second; ++col)
+                                    for (int r = 0; r < ls; ++r)
+                                        out(left_offset + e.row * ls + r, col) += left_diagonal[r] * alfa_t;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        for (Block &bk : lb2.b) {
+            const size_t rb = R[b2].find(bk.rc, bk.rc);
+            if (rb == R[b2].n()) continue;
+            const Matrix &Rm = R[b2].b[rb].m;
+            for (int c = 0; c < bk.m.c; ++c)
+                for (int r = 0; r < bk.m.r; ++r) bk.m(r, c) *= Rm(c, c);
+            ret.match_and_add(bk);
+        }
+    }
+    return ret;
+}
+
 }  // namespace orc

@@ -268,6 +268,8 @@ BlockMatrix site_hamil2(const Problem &p, const BlockMatrix &ket_lp, const Bound
 // boundary steps, common/move_boundary.hpp:130-229
 Boundary overlap_mpo_left_step(const Problem &p, const BlockMatrix &ket_lp, const Boundary &L);
 Boundary overlap_mpo_right_step(const Problem &p, const BlockMatrix &ket_lp, const Boundary &R);
+// diagonal of H_eff for the Davidson preconditioner, non-abelian/h_diag.hpp:20-152 (SU2 only)
+BlockMatrix diagonal_hamiltonian(const Problem &p, const Boundary &L, const Boundary &R);
 
 // SU(2) coupling coefficients, block_matrix/symmetry/gsl_coupling.h:166-205
 double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i);

@@ -47,6 +47,8 @@ def lib():
         L.orc_result_free.argtypes = [C.c_void_p]
         L.orc_jacobi_davidson.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p, C.c_int, C.c_double,
                                           C.POINTER(f64p), C.c_int, C.POINTER(C.c_int), f64p]
+        L.orc_davidson.argtypes = L.orc_jacobi_davidson.argtypes
+        L.orc_diagonal.argtypes = [C.c_void_p, f64p, f64p, f64p]
         L.orc_set_num_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -133,19 +135,33 @@ class OracleEngine:
         self.flops += fl.value
         return y, fl.value
 
-    def jd(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=()):
+    def _solve(self, fn, symm, psi, L, R, W, max_iter, tol, ortho, x0):
         sd = SiteDesc(symm, psi.phys, psi.left, psi.right, W, L.layout, R.layout)
-        x0 = psi.to_flat()
+        x0 = psi.to_flat() if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         u = np.zeros_like(x0)
         th = C.c_double()
         its = C.c_int()
         fl = C.c_double()
         orth = [np.ascontiguousarray(o, dtype=np.float64) for o in ortho]
         arr = (f64p * max(1, len(orth)))(*[_fp(o) for o in orth])
-        _check(lib().orc_jacobi_davidson(sd.ptr(), _fp(L.data), _fp(R.data), _fp(x0), _fp(u), C.byref(th),
-                                         max_iter, tol, arr, len(orth), C.byref(its), C.byref(fl)))
+        _check(fn(sd.ptr(), _fp(L.data), _fp(R.data), _fp(x0), _fp(u), C.byref(th),
+                  max_iter, tol, arr, len(orth), C.byref(its), C.byref(fl)))
         self.flops += fl.value
         return th.value, u, its.value
+
+    def jd(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=(), x0=None):
+        return self._solve(lib().orc_jacobi_davidson, symm, psi, L, R, W, max_iter, tol, ortho, x0)
+
+    def davidson(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=(), x0=None):
+        """Davidson with the diagonal preconditioner (SU2 only), optimize/davidson.h:64-161."""
+        return self._solve(lib().orc_davidson, symm, psi, L, R, W, max_iter, tol, ortho, x0)
+
+    def diagonal(self, symm, psi, L, R, W):
+        """Diagonal of H_eff, non-abelian/h_diag.hpp:120-152, flat left-paired layout."""
+        sd = SiteDesc(symm, psi.phys, psi.left, psi.right, W, L.layout, R.layout)
+        out = np.zeros(psi.to_flat().size)
+        _check(lib().orc_diagonal(sd.ptr(), _fp(L.data), _fp(R.data), _fp(out)))
+        return out
 
 
 def wigner9j(*a):

@@ -162,7 +162,8 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         const int tm = 64, tn = 64, kc = 2048;
         if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse);
         else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse);
-        else p->hp = build_rstep_plan(*desc, tm, tn, kc);
+        else if (kind == PLAN_RSTEP) p->hp = build_rstep_plan(*desc, tm, tn, kc);
+        else p->hp = build_diag_plan(*desc, tm, tn, kc);
         hipStream_t s = ctx->stream;
         HostPlan &h = p->hp;
         p->arrL.upload(h.arrL, s);
@@ -182,6 +183,8 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         p->c.alloc(h.n_c);
         p->tmp.alloc(h.n_tmp);
         p->ws.alloc(std::max(h.g1.ws, h.g2.ws));
+        // diagonal plans: T columns that no strip writes stay zero for the plan's life
+        if (kind == PLAN_DIAG && h.n_t) check(hipMemsetAsync(p->t.p, 0, sizeof(double) * h.n_t, s), "clear T");
         check(hipStreamSynchronize(s), "plan upload");
         *out = p.release();
         return MQ_OK;
@@ -198,6 +201,9 @@ int mq_lstep_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out) {
 }
 int mq_rstep_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out) {
     return make_plan(ctx, desc, out, PLAN_RSTEP);
+}
+int mq_diag_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out) {
+    return make_plan(ctx, desc, out, PLAN_DIAG);
 }
 
 int mq_plan_destroy(mq_plan *plan) {
@@ -466,6 +472,29 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
     }
 }
 
+int mq_diagonal(mq_plan *plan, const double *L, const double *R, double *diag) {
+    if (!plan || !L || !R || !diag) return MQ_ERR_ARG;
+    HostPlan &h = plan->hp;
+    if (h.kind != PLAN_DIAG) {
+        plan->ctx->err = "mq_diagonal: plan is not a diagonal plan";
+        return MQ_ERR_ARG;
+    }
+    hipStream_t s = plan->ctx->stream;
+    try {
+        check(launch_copy2d(plan->arrL.p, (int)h.arrL.size(), L, plan->la.p, s), "L diagonals");
+        check(launch_copy2d(plan->arrR.p, (int)h.arrR.size(), R, plan->rb.p, s), "R diagonals");
+        check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->la.p, plan->t.p,
+                            ST_MISC, s),
+              "diagonal W apply");
+        check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p, plan->g2.nsplit,
+                          plan->t.p, plan->rb.p, diag, plan->ws.p, ST_MISC, s),
+              "diagonal gemm");
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ Jacobi-Davidson
@@ -727,6 +756,186 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
     }
 }
 
+// ------------------------------------------------------------------ Davidson (diagonal preconditioner)
+// optimize/davidson.h:64-161 with MultDiagonal::precondition (ietl_davidson.h:96-104).
+// The reference rotates its whole basis into Ritz vectors every iteration
+// (davidson.h:107-124); here the basis V, VA stays as built and the Ritz
+// vectors are formed from the eigenvectors of its Rayleigh matrix (the same
+// subspace and Ritz pairs), except at the restart, where the two lowest Ritz
+// vectors and their images become the new basis (davidson.h:152-156).
+extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, const double *diag, const double *x0,
+                           double *u_out, double *theta_out, int max_iter, double tol, int prec,
+                           const double *const *ortho, int n_ortho, int *iterations) {
+    constexpr int kMaxDim = 20;  // basis size that triggers the restart
+    if (!plan || !diag || !x0 || !u_out || !theta_out || max_iter < 1 || max_iter > 100000 || n_ortho < 0 ||
+        n_ortho > 60)
+        return MQ_ERR_ARG;
+    if (plan->hp.kind != PLAN_SIGMA) {
+        plan->ctx->err = "mq_davidson: needs a sigma plan";
+        return MQ_ERR_ARG;
+    }
+    if (plan->hp.symm != MQ_SYMM_SU2U1) {
+        plan->ctx->err = "mq_davidson: Davidson only implemented for spin-adapted Hamiltonians";
+        return MQ_ERR_ARG;
+    }
+    if (const char *e = prec_error(prec)) {
+        plan->ctx->err = std::string("mq_davidson: ") + e;
+        return MQ_ERR_ARG;
+    }
+    try {
+        JD jd;
+        jd.p = plan;
+        jd.s = plan->ctx->stream;
+        jd.n = plan->hp.vec_len;
+        jd.nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kJdBlocks, (jd.n + 255) / 256));
+        jd.init();
+        const int64_t n = jd.n;
+        const int need = 2 * kMaxDim + 5;
+        if (plan->jd_cap < need) {
+            plan->jd_vecs.alloc((size_t)need * n);
+            plan->jd_cap = need;
+        }
+        auto V = [&](int i) { return jd.vec(i); };
+        auto VA = [&](int i) { return jd.vec(kMaxDim + i); };
+        double *u = jd.vec(2 * kMaxDim), *r = jd.vec(2 * kMaxDim + 1), *vc = jd.vec(2 * kMaxDim + 2);
+        double *w0 = jd.vec(2 * kMaxDim + 3), *w1 = jd.vec(2 * kMaxDim + 4);
+        std::vector<double> oo(n_ortho);
+        for (int k = 0; k < n_ortho; ++k) oo[k] = jd.dot(ortho[k], ortho[k]);
+        auto project = [&](double *t) {  // SingleSiteVS::project (ietl_lanczos_solver.h:90-95)
+            if (n_ortho == 0) return;
+            std::vector<const double *> xs, ys;
+            for (int k = 0; k < n_ortho; ++k) {
+                xs.push_back(ortho[k]);
+                ys.push_back(t);
+            }
+            std::vector<double> d = jd.dots(xs, ys), c(n_ortho);
+            for (int k = 0; k < n_ortho; ++k) c[k] = -d[k] / oo[k];
+            jd.lincomb(std::vector<const double *>(ortho, ortho + n_ortho), c, 1.0, t);
+        };
+        // Rayleigh matrix of the current basis, M[j * kMaxDim + i] = V_i . VA_j (i <= j)
+        std::vector<double> M((size_t)kMaxDim * kMaxDim, 0.0);
+        // ascending eigenpairs of M(0:dim), each eigenvector with its largest component positive
+        auto ritz = [&](int dim, std::vector<double> &w, std::vector<double> &E) {
+            std::vector<double> A((size_t)dim * dim), Z, wr;
+            for (int j = 0; j < dim; ++j)
+                for (int i = 0; i <= j; ++i) {
+                    const double v = M[(size_t)j * kMaxDim + i];
+                    A[(size_t)j * dim + i] = v;
+                    A[(size_t)i * dim + j] = v;
+                }
+            sym_eig(dim, A, wr, Z);
+            std::vector<int> order(dim);
+            for (int i = 0; i < dim; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return wr[x] < wr[y]; });
+            w.resize(dim);
+            E.assign((size_t)dim * dim, 0.0);
+            for (int j = 0; j < dim; ++j) {
+                const double *z = Z.data() + (size_t)order[j] * dim;
+                int big = 0;
+                for (int i = 1; i < dim; ++i)
+                    if (std::fabs(z[i]) > std::fabs(z[big])) big = i;
+                const double sg = z[big] < 0 ? -1.0 : 1.0;
+                for (int i = 0; i < dim; ++i) E[(size_t)j * dim + i] = sg * z[i];
+                w[j] = wr[order[j]];
+            }
+        };
+        check(hipMemcpyAsync(V(0), x0, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "x0");
+        project(V(0));
+        int dim = 0, it = 0;
+        double theta = 0;
+        while (true) {
+            // Gram-Schmidt with one refinement when the norm drops below kappa = 0.25 of
+            // its value (davidson.h:93-99), projection, normalisation
+            double *t = V(dim);
+            std::vector<const double *> xs{t}, ys{t};
+            for (int i = 0; i < dim; ++i) {
+                xs.push_back(V(i));
+                ys.push_back(t);
+            }
+            std::vector<double> d = jd.dots(xs, ys);
+            const double tau = std::sqrt(d[0]);
+            double nt2 = d[0];
+            for (int pass = 0; pass < 2 && dim > 0; ++pass) {
+                std::vector<const double *> vv;
+                std::vector<double> c;
+                for (int i = 0; i < dim; ++i) {
+                    vv.push_back(V(i));
+                    c.push_back(-d[1 + i]);
+                }
+                jd.lincomb(vv, c, 1.0, t);
+                d = jd.dots(xs, ys);
+                nt2 = d[0];
+                if (std::sqrt(nt2) >= 0.25 * tau) break;
+            }
+            if (n_ortho > 0) {
+                project(t);
+                nt2 = jd.dot(t, t);
+            }
+            jd.lincomb({t}, {1.0 / std::sqrt(nt2) - 1.0}, 1.0, t);
+            sigma_launch(plan, L, R, t, VA(dim), prec);
+            ++dim;
+            // new Rayleigh column (davidson.h:104-106)
+            xs.clear();
+            ys.clear();
+            for (int i = 0; i < dim; ++i) {
+                xs.push_back(V(i));
+                ys.push_back(VA(dim - 1));
+            }
+            std::vector<double> col = jd.dots(xs, ys);
+            for (int i = 0; i < dim; ++i) M[(size_t)(dim - 1) * kMaxDim + i] = col[i];
+            std::vector<double> w, E;
+            ritz(dim, w, E);
+            theta = w[0];
+            std::vector<const double *> vv, va;
+            for (int j = 0; j < dim; ++j) {
+                vv.push_back(V(j));
+                va.push_back(VA(j));
+            }
+            std::vector<double> y0(E.begin(), E.begin() + dim);
+            jd.lincomb(vv, y0, 0.0, u);
+            jd.lincomb(va, y0, 0.0, r);
+            jd.lincomb({u}, {-theta}, 1.0, r);  // r = VA y0 - theta V y0
+            // Vcpy = u / (theta - D) for the preconditioner, formed before the test
+            check(launch_diag_scale(u, u, 0.0, diag, theta, vc, n, jd.s), "precondition");
+            ++it;
+            std::vector<double> rr = jd.dots({r, vc, u}, {r, r, vc});
+            const double rn = std::sqrt(rr[0]);
+            if (rn <= tol * std::fabs(theta) || rn < tol || it >= max_iter) break;
+            if (dim >= kMaxDim) {
+                // restart with the two lowest Ritz vectors and their images
+                std::vector<double> y1(E.begin() + dim, E.begin() + 2 * dim);
+                jd.lincomb(vv, y1, 0.0, w0);
+                jd.lincomb(va, y0, 0.0, w1);
+                check(hipMemcpyAsync(V(0), u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "restart");
+                check(hipMemcpyAsync(V(1), w0, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "restart");
+                jd.lincomb(va, y1, 0.0, w0);
+                check(hipMemcpyAsync(VA(0), w1, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "restart");
+                check(hipMemcpyAsync(VA(1), w0, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "restart");
+                dim = 2;
+                std::vector<double> m = jd.dots({V(0), V(0), V(1)}, {VA(0), VA(1), VA(1)});
+                M[0] = m[0];
+                M[(size_t)kMaxDim] = m[1];
+                M[(size_t)kMaxDim + 1] = m[2];
+            }
+            // t = (r - (Vcpy.r)/(u.Vcpy) u) / (theta - D), normalised by the next pass
+            check(launch_diag_scale(r, u, rr[1] / rr[2], diag, theta, V(dim), n, jd.s), "precondition");
+        }
+        if (prec != MQ_PREC_FP64) {
+            // FP64 defect correction, as in the Jacobi-Davidson driver
+            sigma_launch(plan, L, R, u, r, MQ_PREC_FP64);
+            std::vector<double> d = jd.dots({u, u}, {r, u});
+            theta = d[0] / d[1];
+        }
+        check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
+        jd.sync();
+        *theta_out = theta;
+        if (iterations) *iterations = it;
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
 // ------------------------------------------------------------------ host-side introspection
 struct mq_host_plan {
     HostPlan hp;
@@ -747,6 +956,7 @@ int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out) 
         if (k == PLAN_SIGMA) h->hp = build_sigma_plan(*desc, tm, tn, kc, fuse);
         else if (k == PLAN_LSTEP) h->hp = build_lstep_plan(*desc, tm, tn, kc, fuse);
         else if (k == PLAN_RSTEP) h->hp = build_rstep_plan(*desc, tm, tn, kc);
+        else if (k == PLAN_DIAG) h->hp = build_diag_plan(*desc, tm, tn, kc);
         else return MQ_ERR_ARG;
         *out = h.release();
         return MQ_OK;

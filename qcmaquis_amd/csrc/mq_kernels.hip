@@ -364,6 +364,7 @@ static hipError_t wapply_launch(const Strip *strips, int n, const Src *srcs, con
     switch (stage) {
         case ST_SIGMA: hipLaunchKernelGGL((k_wapply<ST_SIGMA, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
         case ST_FUSE: hipLaunchKernelGGL((k_wapply<ST_FUSE, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
+        case ST_MISC: hipLaunchKernelGGL((k_wapply<ST_MISC, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
         default: hipLaunchKernelGGL((k_wapply<ST_STEP, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
     }
     return hipGetLastError();
@@ -416,6 +417,24 @@ hipError_t launch_multidot(const double *const *xs, const double *const *ys, int
     hipLaunchKernelGGL(k_multidot, dim3(nblocks, npairs), dim3(256), 0, s, xs, ys, n, partial);
     return hipGetLastError();
 }
+__global__ void __launch_bounds__(256) k_diag_scale(const double *x, const double *u, double f,
+                                                    const double *__restrict__ D, double theta, double *out,
+                                                    int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double v = x[e] - f * u[e];
+        const double d = theta - D[e];
+        out[e] = d != 0.0 ? v / d : v;
+    }
+}
+
+hipError_t launch_diag_scale(const double *x, const double *u, double f, const double *D, double theta, double *out,
+                             int64_t n, hipStream_t s) {
+    int nb = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_diag_scale, dim3(nb), dim3(256), 0, s, x, u, f, D, theta, out, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_lincomb(const double *const *vs, const double *coef, int nv, int64_t n, double beta, double *out,
                           hipStream_t s) {
     int nb = (int)std::min<int64_t>((n + 255) / 256, 4096);

@@ -29,6 +29,10 @@ hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntile
                        const float *A, const float *B, double *C, float *W, int stage, hipStream_t s);
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,
                            double *partial, hipStream_t s);
+// out = (x - f u) / (theta - D), elementwise, leaving the elements where theta == D
+// undivided (MultDiagonal::mult_diag); out may alias x
+hipError_t launch_diag_scale(const double *x, const double *u, double f, const double *D, double theta, double *out,
+                             int64_t n, hipStream_t s);
 hipError_t launch_lincomb(const double *const *vs, const double *coef, int nv, int64_t n, double beta, double *out,
                           hipStream_t s);
 

@@ -1382,4 +1382,168 @@ double reference_sigma_flops(const mq_site_desc &d) {
     return f;
 }
 
+// ============================================================ diagonal of H_eff
+// non-abelian/h_diag.hpp:20-152 (the Davidson preconditioner).  For a row slice
+// (right charge rc, phys p, left charge lc = rc - p, phys state ss) of the
+// left-paired vector and column c:
+//   diag = sum_b2 R[b2](rc,rc)[c,c] * sum_b1 alpha(b1,b2) * L[b1](lc,lc)[l,l]
+// with alpha summed over the operator terms of (b1,b2), their phys-diagonal
+// blocks and the entries with row == col == ss:
+//   alpha += coeff * scale * coupling(j, |j-jp|, jp, a, k, ap, j, |j-jp|, jp)[rspin == 2 ? 3 : 0]
+// (j = spin(lc), jp = spin(rc)).  Only stored boundary blocks contribute (hermitian
+// partners are not stored).  Evaluation: LA = diagonals of the L blocks, RB_rc =
+// (K_rc x cols) diagonals of the R blocks over the b2 of a non-empty MPO column,
+// T_slice (l x K_rc) from W-application strips over LA, diag_slice = T_slice * RB_rc.
+HostPlan build_diag_plan(const mq_site_desc &d, int tm, int tn, int kc) {
+    Site s(d);
+    if (!s.su2) throw std::runtime_error("Davidson only implemented for spin-adapted Hamiltonians");
+    BLayout L = to_layout(d.L), R = to_layout(d.R);
+    if (L.aux != s.W.rows || R.aux != s.W.cols) throw std::runtime_error("boundary aux_dim does not match the MPO");
+    HostPlan P;
+    P.kind = PLAN_DIAG;
+    P.symm = s.symm;
+    P.vec_len = s.X.total;
+    // diagonals of L[b1](lc,lc)
+    std::vector<int64_t> dl((size_t)s.W.rows * s.nl, -1);
+    int64_t ndl = 0;
+    for (int b1 = 0; b1 < s.W.rows; ++b1)
+        for (const BBlk &bk : L.b[b1]) {
+            if (!(bk.lc == bk.rc)) continue;
+            const int l = s.li.get(bk.lc);
+            if (l < 0) continue;
+            if (bk.rows != s.lsize(l) || bk.cols != bk.rows)
+                throw std::runtime_error("left boundary block does not match the left index");
+            dl[(size_t)b1 * s.nl + l] = ndl;
+            P.arrL.push_back({ndl, bk.off, 1, bk.rows, 1, bk.rows + 1, 0, 1.0});
+            ndl += bk.rows;
+        }
+    // diagonals of R[b2](rc,rc) for the b2 of non-empty MPO columns, per right charge
+    std::vector<std::vector<int>> kb2(s.nr);
+    std::vector<int64_t> dr_off(s.nr, 0);
+    int64_t ndr = 0;
+    for (int r = 0; r < s.nr; ++r) {
+        if (s.x_of_r[r] < 0) continue;
+        const Q rc = s.right.q[r];
+        const int cs = s.rsize(r);
+        std::vector<const BBlk *> blks;
+        for (int b2 = 0; b2 < s.W.cols; ++b2) {
+            if (s.W.col_b1[b2].empty()) continue;
+            const BBlk *bk = R.find(b2, rc, rc);
+            if (!bk) continue;
+            if (bk->rows != cs || bk->cols != cs)
+                throw std::runtime_error("right boundary block does not match the right index");
+            kb2[r].push_back(b2);
+            blks.push_back(bk);
+        }
+        const int K = (int)kb2[r].size();
+        dr_off[r] = ndr;
+        for (int k = 0; k < K; ++k) P.arrR.push_back({ndr + k, blks[k]->off, 1, cs, K, blks[k]->rows + 1, 0, 1.0});
+        ndr += (int64_t)K * cs;
+    }
+    P.n_la = ndl;
+    P.n_rb = ndr;
+    // row slices (r, p, ss) of every vector block; T offsets in that order
+    struct Slice {
+        int p, l, ss;
+        int64_t t;
+    };
+    std::vector<std::vector<Slice>> slices(s.nr);
+    std::vector<std::vector<int>> slice_of(s.nr);  // [p * maxss + ss] -> slice index
+    int maxss = 1;
+    for (int p = 0; p < s.np; ++p) maxss = std::max(maxss, s.phys.s[p]);
+    int64_t nt = 0;
+    for (int r = 0; r < s.nr; ++r) {
+        if (s.x_of_r[r] < 0) continue;
+        slice_of[r].assign((size_t)s.np * maxss, -1);
+        const int K = (int)kb2[r].size();
+        for (int p = 0; p < s.np; ++p) {
+            const int l = s.rmp_l[(size_t)r * s.np + p];
+            if (l < 0) continue;
+            for (int ss = 0; ss < s.phys.s[p]; ++ss) {
+                slice_of[r][(size_t)p * maxss + ss] = (int)slices[r].size();
+                slices[r].push_back({p, l, ss, nt});
+                nt += (int64_t)s.lsize(l) * K;
+            }
+        }
+    }
+    P.n_t = nt;
+    // strips per right charge (host threads), concatenated in charge order
+    std::vector<StripList> part(s.nr);
+    parallel_for(s.nr, [&](int r) {
+        if (s.x_of_r[r] < 0) return;
+        const Q rc = s.right.q[r];
+        const int jp = spin2(rc);
+        StripList &out = part[r];
+        std::vector<std::map<int, double>> acc(slices[r].size());
+        for (int k = 0; k < (int)kb2[r].size(); ++k) {
+            const int b2 = kb2[r][k];
+            const int ap = s.W.rspin[b2];
+            for (auto &a_ : acc) a_.clear();
+            for (int b1 : s.W.col_b1[b2]) {
+                const int a = s.W.lspin[b1];
+                for (const TermD &term : s.W.at.at({b1, b2})) {
+                    const int opk = s.W.ops[term.op].spin;
+                    for (const Site::OpB &ob : s.opb[term.op]) {
+                        if (ob.pin != ob.pout) continue;
+                        const int p = ob.pin;
+                        const int l = s.rmp_l[(size_t)r * s.np + p];
+                        if (l < 0 || dl[(size_t)b1 * s.nl + l] < 0) continue;
+                        const int j = spin2(s.left.q[l]), ts = std::abs(j - jp);
+                        const std::array<double, 4> &cp = coupling(j, ts, jp, a, opk, ap, j, ts, jp);
+                        for (const Ent &e : ob.b->e) {
+                            if (e.row != e.col || e.row < 0 || e.row >= s.phys.s[p]) continue;
+                            const int si = slice_of[r][(size_t)p * maxss + e.row];
+                            if (si < 0) continue;
+                            acc[si][b1] += e.coeff * term.scale * cp[e.rspin == 2 ? 3 : 0];
+                        }
+                    }
+                }
+            }
+            for (size_t si = 0; si < acc.size(); ++si) {
+                if (acc[si].empty()) continue;
+                const Slice &sl = slices[r][si];
+                const int ls = s.lsize(sl.l);
+                Strip st{sl.t + (int64_t)k * ls, ls, 1, ls, (int)out.srcs.size(), 0};
+                for (auto &kv : acc[si]) out.srcs.push_back({dl[(size_t)kv.first * s.nl + sl.l], ls, kv.second});
+                st.s1 = (int)out.srcs.size();
+                out.strips.push_back(st);
+            }
+        }
+    });
+    double src_el = 0;
+    for (int r = 0; r < s.nr; ++r) {
+        const int base = (int)P.wapply.srcs.size();
+        for (Strip st : part[r].strips) {
+            st.s0 += base;
+            st.s1 += base;
+            src_el += (double)st.rows * (st.s1 - st.s0);
+            P.wapply.strips.push_back(st);
+        }
+        P.wapply.srcs.insert(P.wapply.srcs.end(), part[r].srcs.begin(), part[r].srcs.end());
+    }
+    // diag slice = T_slice * RB_rc (K = 0 writes zeros)
+    for (int r = 0; r < s.nr; ++r) {
+        if (s.x_of_r[r] < 0) continue;
+        const VBlk &xb = s.X.blk[s.x_of_r[r]];
+        const int K = (int)kb2[r].size();
+        for (const Slice &sl : slices[r]) {
+            const int ls = s.lsize(sl.l);
+            GemmGroup g{};
+            g.a = sl.t;
+            g.b = dr_off[r];
+            g.c = xb.off + s.pbl[(size_t)sl.p * s.nl + sl.l] + (int64_t)sl.ss * ls;
+            g.M = ls;
+            g.N = xb.cols;
+            g.K = K;
+            g.lda = ls;
+            g.ldb = std::max(K, 1);
+            g.ldc = xb.rows;
+            add_gemm(P.g2, g, tm, tn, kc);
+        }
+    }
+    order_tiles(P.g2);
+    P.exec_flops = P.g2.flops() + 2.0 * src_el;
+    return P;
+}
+
 }  // namespace mq

@@ -70,7 +70,7 @@ struct StripList {
     std::vector<Src> srcs;
 };
 
-enum PlanKind { PLAN_SIGMA = 0, PLAN_LSTEP = 1, PLAN_RSTEP = 2 };
+enum PlanKind { PLAN_SIGMA = 0, PLAN_LSTEP = 1, PLAN_RSTEP = 2, PLAN_DIAG = 3 };
 
 // Host-side plan; the API layer uploads the task arrays.
 struct HostPlan {
@@ -102,6 +102,10 @@ struct HostPlan {
 HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, int fuse = 1);
 HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, int fuse = 1);
 HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
+// Diagonal of H_eff (SU2 only): arrL extracts the diagonals of the L[b1](lc,lc)
+// blocks into LA, arrR those of R[b2](rc,rc) into RB, the W-application strips
+// combine LA columns into T, and g2 writes the diagonal = T * RB per row slice.
+HostPlan build_diag_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
 
 // One GEMM group split into tiles (and K chunks) as the plans do.
 void tile_gemm(GemmList &gl, GemmGroup g, int tile_m, int tile_n, int kchunk_target);

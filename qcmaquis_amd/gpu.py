@@ -39,7 +39,7 @@ def lib():
         L.mq_ctx_stream.restype = vp
         L.mq_ctx_synchronize.argtypes = [vp]
         L.mq_ctx_set_option.argtypes = [vp, C.c_int, C.c_int]
-        for f in ("mq_sigma_plan_create", "mq_lstep_plan_create", "mq_rstep_plan_create"):
+        for f in ("mq_sigma_plan_create", "mq_lstep_plan_create", "mq_rstep_plan_create", "mq_diag_plan_create"):
             getattr(L, f).argtypes = [vp, vp, C.POINTER(vp)]
         L.mq_plan_destroy.argtypes = [vp]
         L.mq_plan_vec_len.argtypes = [vp]
@@ -58,6 +58,9 @@ def lib():
         L.mq_boundary_step.argtypes = [vp, vp, vp, vp]
         L.mq_jacobi_davidson.argtypes = [vp, vp, vp, vp, vp, f64p, C.c_int, C.c_double, C.c_int,
                                          C.POINTER(vp), C.c_int, C.POINTER(C.c_int)]
+        L.mq_diagonal.argtypes = [vp, vp, vp, vp]
+        L.mq_davidson.argtypes = [vp, vp, vp, vp, vp, vp, f64p, C.c_int, C.c_double, C.c_int,
+                                  C.POINTER(vp), C.c_int, C.POINTER(C.c_int)]
         L.mq_host_plan_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
         L.mq_host_plan_part.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.mq_host_plan_sizes.argtypes = [vp, i64p]
@@ -83,7 +86,7 @@ SRC = np.dtype([("off", "<i8"), ("ld", "<i4"), ("coef", "<f8")], align=True)
 PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D, STRIP, SRC]
 PART_NAMES = ["arrL", "arrR", "reshape", "g1_groups", "g1_tiles", "strips", "srcs", "g2_groups", "g2_tiles",
               "scatter", "comb_strips", "comb_srcs"]
-KIND = {"sigma": 0, "lstep": 1, "rstep": 2}
+KIND = {"sigma": 0, "lstep": 1, "rstep": 2, "diag": 3}
 
 
 class HostPlan:
@@ -113,7 +116,7 @@ class HostPlan:
             L.mq_host_plan_flops(h, C.byref(a), C.byref(b), C.byref(c))
             self.ref_flops, self.exec_flops, self.min_bytes = a.value, b.value, c.value
             self.out_layout = None
-            if kind != "sigma":
+            if kind in ("lstep", "rstep"):
                 d = mq_boundary_desc()
                 L.mq_host_plan_out_desc(h, C.byref(d))
                 self.out_layout = BoundaryLayout.from_c(d)
@@ -207,7 +210,7 @@ class GpuEngine:
     def _plan(self, kind, site):
         p = C.c_void_p()
         fn = {"sigma": lib().mq_sigma_plan_create, "lstep": lib().mq_lstep_plan_create,
-              "rstep": lib().mq_rstep_plan_create}[kind]
+              "rstep": lib().mq_rstep_plan_create, "diag": lib().mq_diag_plan_create}[kind]
         self.check(fn(self.ctx, site.ptr(), C.byref(p)), f"{kind} plan")
         return p
 
@@ -252,12 +255,32 @@ class GpuEngine:
         finally:
             sp.close()
 
-    def jd(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=()):
+    def jd(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=(), x0=None):
         sp = self.sigma_plan(symm, psi, L, R, W)
         try:
-            theta, u, its = sp.jd(self.to_dev(psi.to_flat()), max_iter, tol, ortho)
+            theta, u, its = sp.jd(self.to_dev(psi.to_flat() if x0 is None else x0), max_iter, tol, ortho)
             self.flops += its * sp.ref_flops
             return theta, u.cpu().numpy(), its
+        finally:
+            sp.close()
+
+    def davidson(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=(), x0=None):
+        """Davidson with the diagonal preconditioner (SU2U1), solve_ietl_davidson."""
+        sp = self.sigma_plan(symm, psi, L, R, W)
+        try:
+            theta, u, its = sp.davidson(self.to_dev(psi.to_flat() if x0 is None else x0), max_iter, tol, ortho)
+            self.flops += its * sp.ref_flops
+            return theta, u.cpu().numpy(), its
+        finally:
+            sp.close()
+
+    def diagonal(self, symm, psi, L, R, W):
+        """Diagonal of H_eff (SU2U1), contraction::SU2::diagonal_hamiltonian."""
+        sp = self.sigma_plan(symm, psi, L, R, W)
+        try:
+            d = sp.diagonal()
+            self.sync()
+            return d.cpu().numpy()
         finally:
             sp.close()
 
@@ -277,6 +300,9 @@ class SigmaPlan:
         self.workspace_bytes = lib().mq_plan_workspace_bytes(self.p)
 
     def close(self):
+        if getattr(self, "_dp", None):
+            lib().mq_plan_destroy(self._dp)
+            self._dp = None
         if self.p:
             lib().mq_plan_destroy(self.p)
             self.p = None
@@ -309,6 +335,32 @@ class SigmaPlan:
         self.eng.check(lib().mq_sigma_timed(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y), prec,
                                             ms, work), "mq_sigma_timed")
         return list(ms), list(work)
+
+    def diagonal(self):
+        """Diagonal of H_eff as a device vector (builds a diagonal plan once)."""
+        t = self.eng.torch
+        if getattr(self, "_dp", None) is None:
+            self._dp = self.eng._plan("diag", self.site)
+        d = t.empty(self.n, dtype=t.float64, device=self.eng.device)
+        t.cuda.synchronize()
+        self.eng.check(lib().mq_diagonal(self._dp, _ptr(self.L.data), _ptr(self.R.data), _ptr(d)), "mq_diagonal")
+        return d
+
+    def davidson(self, x0, max_iter=10, tol=1e-8, ortho=(), prec=None, diag=None):
+        t = self.eng.torch
+        prec = self.eng.prec if prec is None else prec
+        if diag is None:
+            diag = self.diagonal()
+        u = t.empty(self.n, dtype=t.float64, device=self.eng.device)
+        th = C.c_double()
+        its = C.c_int()
+        od = [self.eng.to_dev(o) for o in ortho]
+        arr = (C.c_void_p * max(1, len(od)))(*[o.data_ptr() for o in od])
+        t.cuda.synchronize()
+        self.eng.check(lib().mq_davidson(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(diag), _ptr(x0),
+                                         _ptr(u), C.byref(th), max_iter, tol, prec, arr, len(od), C.byref(its)),
+                       "mq_davidson")
+        return th.value, u, its.value
 
     def jd(self, x0, max_iter=10, tol=1e-8, ortho=(), prec=None):
         t = self.eng.torch

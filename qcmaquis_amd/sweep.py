@@ -27,9 +27,10 @@ from .twou1 import twou1_terms
 class Hamiltonian:
     """Quantum-chemistry Hamiltonian as an MPO (harness input)."""
 
-    def __init__(self, L, nelec, spin, integrals, symm="su2u1"):
+    def __init__(self, L, nelec, spin, integrals, symm="su2u1", herm=True):
         """spin is 2S.  symm "su2u1" (models/chem/su2u1) or "2u1" (models/chem/2u1,
-        spin-orbital basis with N_alpha - N_beta = 2S)."""
+        spin-orbital basis with N_alpha - N_beta = 2S).  herm=False builds the SU2
+        MPO without hermitian-partner bonds (every boundary slice is stored)."""
         self.L, self.nelec, self.spin = L, nelec, spin
         table = OpTable()
         if symm == "su2u1":
@@ -37,7 +38,7 @@ class Hamiltonian:
             self.su2 = True
             ops = SU2Ops(table)
             self.terms = su2_terms(to_aligned(integrals), L, nelec, ops)
-            mk = TaggedMPOMaker(L, table, ops.ident, ops.fill, ops.ident_full)
+            mk = TaggedMPOMaker(L, table, ops.ident, ops.fill, ops.ident_full, herm=herm)
             self.phys = su2_phys_index()
             self.right_end = (nelec, spin)
         elif symm == "2u1":

@@ -105,3 +105,18 @@ def run_step(hp, kind, Bdata, A):
         gemm(P["g2_groups"], P["g2_tiles"], Cb, prp, tmp)
     copy2d(P["scatter"], tmp, out)
     return out[:s["out"]]
+
+
+def run_diag(hp, Ldata, Rdata):
+    """Diagonal plan: L/R diagonals, strips into T (zero where no strip writes),
+    then diag slice = T_slice * RB_rc; every element of the result must be written."""
+    s = hp.sizes
+    P = hp.parts
+    la, rb, T = np.zeros(max(s["la"], 1)), np.zeros(max(s["rb"], 1)), np.zeros(max(s["t"], 1))
+    out = np.full(s["vec"], np.nan)
+    copy2d(P["arrL"], Ldata, la)
+    copy2d(P["arrR"], Rdata, rb)
+    wapply(P["strips"], P["srcs"], la, T)
+    gemm(P["g2_groups"], P["g2_tiles"], T, rb, out)
+    assert not np.isnan(out).any(), "diagonal element not written"
+    return out

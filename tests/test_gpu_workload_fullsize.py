@@ -82,6 +82,42 @@ def test_fullsize_jacobi_davidson_vs_oracle(centre, gpu_engine, oracle_engine):
     assert abs(t1 - t0) <= 1e-10 * abs(t0)
 
 
+def test_fullsize_diagonal_and_davidson(centre, gpu_engine, oracle_engine):
+    """Diagonal of H_eff equal to the oracle's; the diagonal-preconditioned
+    Davidson (with restarts) converges to the eigenvalue of a converged
+    Jacobi-Davidson chain, with its residual within the tolerance and its
+    eigenvalue equal to the FP64 Rayleigh quotient of its vector; the mixed
+    form within 1e-8."""
+    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    p = centre
+    D0 = oracle_engine.diagonal(p.ham.symm, p.psi, _host(p.L), _host(p.R), p.W)
+    sp = gpu_engine.sigma_plan(p.ham.symm, p.psi, p.L, p.R, p.W)
+    try:
+        diag = sp.diagonal()
+        D1 = diag.cpu().numpy()
+        print(f"diagonal rel err {_rel(D1, D0):.2e}", flush=True)
+        assert _rel(D1, D0) <= 1e-13
+        x = gpu_engine.to_dev(p.psi.to_flat())
+        t, u, its = sp.davidson(x, 300, 1e-9, prec=MQ_PREC_FP64, diag=diag)
+        y = sp.apply(u, prec=MQ_PREC_FP64)
+        uu, uy = float(u @ u), float(u @ y)
+        rn = float((y - t * u).norm())
+        tj, uj, _ = sp.jd(x, 20, 1e-10, prec=MQ_PREC_FP64)
+        for _ in range(4):
+            tj, uj, _ = sp.jd(uj, 20, 1e-10, prec=MQ_PREC_FP64)
+        t32, _, its32 = sp.davidson(x, 300, 1e-9, prec=MQ_PREC_FP32, diag=diag)
+    finally:
+        sp.close()
+    print(f"davidson theta {t:.12f} ({its} it), jd {tj:.12f}, mixed {t32:.12f} ({its32} it), |r| {rn:.2e}",
+          flush=True)
+    assert its < 300
+    assert abs(uu - 1) <= 1e-12
+    assert abs(uy / uu - t) <= 1e-10 * abs(t)
+    assert rn <= 1.01e-9 * abs(t)
+    assert abs(t - tj) <= 1e-8
+    assert abs(t32 - t) <= 1e-8
+
+
 @pytest.fixture(scope="module")
 def centre_2u1(gpu_engine):
     from qcmaquis_amd.workload import centre_problem, load_sectors
