@@ -148,8 +148,11 @@ const char *mq_last_error(const mq_ctx *ctx);
  * to plans created afterwards.  MQ_OPT_FUSE: how MPO columns with many terms
  * are evaluated -- 0 term by term, 1 (default) columns whose operator entries
  * repeat at few positions are summed on the left boundary first, 2 every
- * column is summed that way (testing aid).  Results agree to round-off. */
-enum mq_option { MQ_OPT_FUSE = 0 };
+ * column is summed that way (testing aid).  Under mode 1 a column is summed
+ * that way when its entries number at least MQ_OPT_FUSE_RATIO (1..255,
+ * default 4) times its distinct positions and at least MQ_OPT_FUSE_MIN
+ * (1..4095, default 64).  Results agree to round-off. */
+enum mq_option { MQ_OPT_FUSE = 0, MQ_OPT_FUSE_RATIO = 1, MQ_OPT_FUSE_MIN = 2 };
 int mq_ctx_set_option(mq_ctx *ctx, int option, int value);
 /* HIP stream (hipStream_t) the context launches on. */
 void *mq_ctx_stream(mq_ctx *ctx);
@@ -247,9 +250,12 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
 /* Builds the same task lists the device plans execute, on the host only, so
  * that schedules, FLOP counts and workspace sizes can be inspected (and the
  * task lists checked) on machines without a GPU.  kind: 0 sigma, 1 left
- * step, 2 right step, 3 diagonal, plus MQ_HOST_FUSE(v) to build with MQ_OPT_FUSE = v
- * (default 1).  Arrays stay owned by the handle. */
+ * step, 2 right step, 3 diagonal, plus MQ_HOST_FUSE(v), MQ_HOST_FUSE_RATIO(r)
+ * and MQ_HOST_FUSE_MIN(m) to build with those MQ_OPT_FUSE* values (0 keeps a
+ * default).  Arrays stay owned by the handle. */
 #define MQ_HOST_FUSE(v) (((v) + 1) << 8)
+#define MQ_HOST_FUSE_RATIO(r) ((r) << 10) /* 1..255 */
+#define MQ_HOST_FUSE_MIN(m) ((m) << 18)   /* 1..4095 */
 typedef struct mq_host_plan mq_host_plan;
 enum mq_plan_part {
     MQ_PART_ARR_L = 0, MQ_PART_ARR_R = 1, MQ_PART_RESHAPE = 2, MQ_PART_G1_GROUPS = 3,

@@ -14,7 +14,7 @@ struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    int fuse = 1;  // MQ_OPT_FUSE
+    FuseRule fuse;  // MQ_OPT_FUSE, MQ_OPT_FUSE_RATIO, MQ_OPT_FUSE_MIN
 };
 
 namespace {
@@ -134,7 +134,15 @@ const char *mq_last_error(const mq_ctx *ctx) { return ctx ? ctx->err.c_str() : "
 int mq_ctx_set_option(mq_ctx *ctx, int option, int value) {
     if (!ctx) return MQ_ERR_ARG;
     if (option == MQ_OPT_FUSE && value >= 0 && value <= 2) {
-        ctx->fuse = value;
+        ctx->fuse.mode = value;
+        return MQ_OK;
+    }
+    if (option == MQ_OPT_FUSE_RATIO && value >= 1 && value <= 255) {
+        ctx->fuse.ratio = value;
+        return MQ_OK;
+    }
+    if (option == MQ_OPT_FUSE_MIN && value >= 1 && value <= 4095) {
+        ctx->fuse.min_entries = value;
         return MQ_OK;
     }
     ctx->err = "mq_ctx_set_option: unknown option or value";
@@ -952,7 +960,11 @@ int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out) 
     try {
         std::unique_ptr<mq_host_plan> h(new mq_host_plan());
         const int tm = 64, tn = 64, kc = 2048;
-        const int fm = (kind >> 8) & 3, fuse = fm ? fm - 1 : 1, k = kind & 255;
+        const int fm = (kind >> 8) & 3, fr = (kind >> 10) & 255, fmin = (kind >> 18) & 4095, k = kind & 255;
+        FuseRule fuse;
+        if (fm) fuse.mode = fm - 1;
+        if (fr) fuse.ratio = fr;
+        if (fmin) fuse.min_entries = fmin;
         if (k == PLAN_SIGMA) h->hp = build_sigma_plan(*desc, tm, tn, kc, fuse);
         else if (k == PLAN_LSTEP) h->hp = build_lstep_plan(*desc, tm, tn, kc, fuse);
         else if (k == PLAN_RSTEP) h->hp = build_rstep_plan(*desc, tm, tn, kc);

@@ -93,7 +93,14 @@ __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips
 // C's row.  Operands are staged through a double-buffered LDS tile with zero
 // padding at the edges, so any M, N, K (including K = 0) is handled; the next
 // K tile is fetched into registers while the current one is multiplied.
-constexpr int TK = 16, TMAX = 128;
+constexpr int TMAX = 128;
+// K step of a tile: 16 for FP64 operands, KSTEP_F32 for FP32 ones (the LDS tile
+// holds the same bytes either way).
+constexpr int KSTEP_F32 = 16;
+template <class T>
+constexpr int kstep() {
+    return sizeof(T) == 4 ? KSTEP_F32 : 16;
+}
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
@@ -118,14 +125,15 @@ struct Mfma<float> {
 
 template <int TM, int TN, class T>
 struct GemmStage {
-    T a[TM / 16], b[TN / 16];
+    T a[TM * kstep<T>() / 256], b[TN * kstep<T>() / 256];
 };
 
 template <int TM, int TN, class T>
 __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restrict__ A, const T *__restrict__ B,
                                            int m0, int n0, int k0, int kend, GemmStage<TM, TN, T> &st) {
+    constexpr int TK = kstep<T>();
 #pragma unroll
-    for (int r = 0; r < TM / 16; ++r) {
+    for (int r = 0; r < TM * TK / 256; ++r) {
         const int e = threadIdx.x + 256 * r;
         int i, k;
         if (!g.transA) {
@@ -141,7 +149,7 @@ __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restri
         st.a[r] = v;
     }
 #pragma unroll
-    for (int r = 0; r < TN / 16; ++r) {
+    for (int r = 0; r < TN * TK / 256; ++r) {
         const int e = threadIdx.x + 256 * r;
         int j, k;
         if (!g.transB) {
@@ -161,8 +169,9 @@ __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restri
 template <int TM, int TN, class T>
 __device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<TM, TN, T> &st, T (*As)[TM + 2],
                                            T (*Bs)[TN + 2]) {
+    constexpr int TK = kstep<T>();
 #pragma unroll
-    for (int r = 0; r < TM / 16; ++r) {
+    for (int r = 0; r < TM * TK / 256; ++r) {
         const int e = threadIdx.x + 256 * r;
         if (!g.transA)
             As[e / TM][e % TM] = st.a[r];
@@ -170,7 +179,7 @@ __device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<T
             As[e % TK][e / TK] = st.a[r];
     }
 #pragma unroll
-    for (int r = 0; r < TN / 16; ++r) {
+    for (int r = 0; r < TN * TK / 256; ++r) {
         const int e = threadIdx.x + 256 * r;
         if (!g.transB)
             Bs[e % TK][e / TK] = st.b[r];
@@ -186,6 +195,7 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
                                           const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
                                           double *lds) {
     typedef Mfma<T> MF;
+    constexpr int TK = kstep<T>();
     // the four waves split the tile along M; each holds TM/4 rows x TN columns
     constexpr int MI = TM / 64, NJ = TN / 16;  // 16x16 MFMA blocks per wave
     T(*As)[TK][TM + 2] = reinterpret_cast<T(*)[TK][TM + 2]>(lds);
@@ -282,7 +292,7 @@ template <int STAGE, class T, class TO>
 __global__ void __launch_bounds__(256) k_gemm(const GemmGroup *__restrict__ groups, const GemmTile *__restrict__ tiles,
                                               const T *__restrict__ A, const T *__restrict__ B, TO *__restrict__ C,
                                               T *__restrict__ W) {
-    __shared__ double lds[(2 * TK * (TMAX + 2) * 2 * sizeof(T) + sizeof(double) - 1) / sizeof(double)];
+    __shared__ double lds[(2 * kstep<T>() * (TMAX + 2) * 2 * sizeof(T) + sizeof(double) - 1) / sizeof(double)];
     const GemmTile t = tiles[blockIdx.x];
     if (t.g < 0) return;  // padding tile of the XCD-ordered list
     const GemmGroup g = groups[t.g];

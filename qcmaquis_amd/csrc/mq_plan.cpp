@@ -308,14 +308,10 @@ struct ColProg {
     std::vector<char> direct, member;     // [b1] row has direct terms / feeds a fused group
 };
 
-// a column is fused when its entries are at least kFuseRatio times its
-// distinct positions and number at least kFuseMinEntries
-constexpr int kFuseRatio = 4;
-constexpr int kFuseMinEntries = 64;
-
-// fuse: 0 never, 1 by the thresholds above, 2 every non-empty column (used by tests
-// to exercise fused groups on small systems)
-ColProg column_program(const Site &s, bool skip_rskip_cols, int fuse) {
+// fuse.mode: 0 never, 1 when a column's entries are at least fuse.ratio times its
+// distinct positions and number at least fuse.min_entries, 2 every non-empty
+// column (used by tests to exercise fused groups on small systems)
+ColProg column_program(const Site &s, bool skip_rskip_cols, FuseRule fuse) {
     ColProg cp;
     cp.col.resize(s.W.cols);
     cp.fcol.resize(s.W.cols);
@@ -349,8 +345,9 @@ ColProg column_program(const Site &s, bool skip_rskip_cols, int fuse) {
         size_t ngroups = 0;
         for (size_t i = 0; i < ents.size(); ++i)
             if (i == 0 || ents[i].key != ents[i - 1].key) ++ngroups;
-        bool fused = (fuse == 2 && !ents.empty()) ||
-                     (fuse == 1 && ents.size() >= (size_t)kFuseMinEntries && ents.size() >= kFuseRatio * ngroups);
+        bool fused = (fuse.mode == 2 && !ents.empty()) ||
+                     (fuse.mode == 1 && !ents.empty() && ents.size() >= (size_t)fuse.min_entries &&
+                      ents.size() >= (size_t)fuse.ratio * ngroups);
         if (!fused) {
             for (int b1 : s.W.col_b1[b2]) {
                 for (auto &term : s.W.at.at({b1, b2})) cp.col[b2].push_back({b1, term.op, term.scale, s.W.lspin[b1]});
@@ -830,7 +827,7 @@ void order_tiles(GemmList &gl) {
 }
 
 // ============================================================ sigma
-HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, int fuse) {
+HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse) {
     Site s(d);
     BLayout L = to_layout(d.L), R = to_layout(d.R);
     if (L.aux != s.W.rows || R.aux != s.W.cols) throw std::runtime_error("boundary aux_dim does not match the MPO");
@@ -917,7 +914,7 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, int fus
 }
 
 // ============================================================ left step
-HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, int fuse) {
+HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse) {
     Site s(d);
     BLayout L = to_layout(d.L);
     if (L.aux != s.W.rows) throw std::runtime_error("left boundary aux_dim does not match the MPO");

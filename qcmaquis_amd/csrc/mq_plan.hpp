@@ -98,9 +98,16 @@ struct HostPlan {
     double exec_flops = 0;
 };
 
-// fuse: 0 no fused MPO columns, 1 fuse the high fan-in columns, 2 fuse every column
-HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, int fuse = 1);
-HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, int fuse = 1);
+// Which MPO columns are summed on the left boundary first ("fused"): mode 0 none,
+// 1 the columns whose operator entries number at least `ratio` times their
+// distinct positions and at least `min_entries`, 2 every non-empty column.
+struct FuseRule {
+    int mode = 1;
+    int ratio = 4;
+    int min_entries = 64;
+};
+HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {});
+HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {});
 HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
 // Diagonal of H_eff (SU2 only): arrL extracts the diagonals of the L[b1](lc,lc)
 // blocks into LA, arrR those of R[b2](rc,rc) into RB, the W-application strips

@@ -92,10 +92,11 @@ KIND = {"sigma": 0, "lstep": 1, "rstep": 2, "diag": 3}
 class HostPlan:
     """Task lists of a plan, built on the host (no GPU needed)."""
 
-    def __init__(self, site: SiteDesc, kind: str, fuse=1):
+    def __init__(self, site: SiteDesc, kind: str, fuse=1, fuse_ratio=0, fuse_min=0):
         L = lib()
         h = C.c_void_p()
-        if L.mq_host_plan_create(site.ptr(), KIND[kind] | ((fuse + 1) << 8), C.byref(h)) != 0:
+        code = KIND[kind] | ((fuse + 1) << 8) | (fuse_ratio << 10) | (fuse_min << 18)
+        if L.mq_host_plan_create(site.ptr(), code, C.byref(h)) != 0:
             raise MQError(L.mq_host_last_error().decode())
         try:
             self.parts = {}
@@ -155,10 +156,15 @@ class GpuEngine:
         self.ctx = C.c_void_p()
         if lib().mq_ctx_create(device, C.byref(self.ctx)) != 0:
             raise MQError("mq_ctx_create failed")
-        self.check(lib().mq_ctx_set_option(self.ctx, 0, fuse), "mq_ctx_set_option")
+        self.set_fuse(fuse)
         self.prec = prec
         self.flops = 0.0
         self.stats = []
+
+    def set_fuse(self, mode=1, ratio=4, min_entries=64):
+        """Fused-column rule of the plans built afterwards (MQ_OPT_FUSE*)."""
+        for opt, v in ((0, mode), (1, ratio), (2, min_entries)):
+            self.check(lib().mq_ctx_set_option(self.ctx, opt, v), "mq_ctx_set_option")
 
     def close(self):
         if self.ctx:

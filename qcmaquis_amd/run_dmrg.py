@@ -69,11 +69,11 @@ def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print):
         eng.eng.close()
 
 
-def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print):
+def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd"):
     from .gpu import GpuEngine
     eng = GpuEngine(0, prec=prec)
     try:
-        d = TwoSiteDMRG(ham, eng, ramp[0], jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed)
+        d = TwoSiteDMRG(ham, eng, ramp[0], jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed, eigensolver=eigensolver)
         out = []
         t0 = time.perf_counter()
         for s, m in enumerate(ramp):
@@ -98,6 +98,8 @@ def main(argv=None):
     ap.add_argument("--jd-maxiter", type=int, default=8)
     ap.add_argument("--jd-tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--eigensolver", default="jd", choices=["jd", "davidson"],
+                    help="IETL_JCD (default) or IETL_DAVIDSON (SU2 only)")
     ap.add_argument("--out")
     a = ap.parse_args(argv)
     if a.fcidump:
@@ -115,8 +117,8 @@ def main(argv=None):
         return rec
     precs = {"fp64": [("fp64", MQ_PREC_FP64)], "mixed": [("mixed", MQ_PREC_FP32)],
              "both": [("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)]}[a.prec]
-    res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed) for name, p in precs}
-    rec = dict(L=ham.L, ramp=ramp, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, results=res)
+    res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver) for name, p in precs}
+    rec = dict(L=ham.L, ramp=ramp, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, eigensolver=a.eigensolver, results=res)
     if "fp64" in res and "mixed" in res:
         rec["delta_E"] = [m["energy"] - f["energy"] for f, m in zip(res["fp64"], res["mixed"])]
         for s, dE in enumerate(rec["delta_E"]):

@@ -76,7 +76,13 @@ def micro_site(L, i):
 
 class TwoSiteDMRG:
     def __init__(self, ham: Hamiltonian, engine, max_bond_dimension, init_bond_dimension=5, seed=42,
-                 cutoff=1e-16, jd_tol=1e-8, jd_maxiter=10, mps=None, log=None):
+                 cutoff=1e-16, jd_tol=1e-8, jd_maxiter=10, mps=None, log=None, eigensolver="jd"):
+        """eigensolver: "jd" (Jacobi-Davidson, IETL_JCD) or "davidson" (diagonal-preconditioned
+        Davidson, IETL_DAVIDSON, SU2 only), as parms["eigensolver"] selects in
+        optimize/ts_optimize.hpp:139-147; both take ietl_jcd_maxiter / ietl_jcd_tol."""
+        if eigensolver not in ("jd", "davidson"):
+            raise ValueError(f"unknown eigensolver {eigensolver!r}")
+        self.solve = engine.jd if eigensolver == "jd" else engine.davidson
         self.ham = ham
         self.engine = engine
         self.m = max_bond_dimension
@@ -110,7 +116,7 @@ class TwoSiteDMRG:
         change = (i == L - 2) or (i == n_microiterations(L) - 1)
         t0 = time.perf_counter()
         psi, Lb, Rb, W = self.site_problem(site)
-        theta, u, its = self.engine.jd(ham.symm, psi, Lb, Rb, W, self.maxiter, self.tol)
+        theta, u, its = self.solve(ham.symm, psi, Lb, Rb, W, self.maxiter, self.tol)
         t1 = time.perf_counter()
         E = theta + ham.ecore
         ut = MPSTensor.from_flat(psi.phys, psi.left, psi.right, u)
