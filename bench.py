@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-delta-e", action="store_true", help="skip the converged-solve energy comparison")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the default schedule instead of tuning it per form on the workload")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="first run a plain device copy of this many GiB (reference for HBM counters)")
     a = ap.parse_args()
@@ -91,6 +93,9 @@ def main():
     import numpy as np
     import torch
     from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    from dataclasses import asdict
+
+    from qcmaquis_amd.autotune import DEFAULT as DEFAULT_SCHEDULE, table_json, tune
     from qcmaquis_amd.gpu import GpuEngine
     from qcmaquis_amd.workload import centre_problem, load_sectors
 
@@ -119,15 +124,32 @@ def main():
     sp_ = centre_problem(eng, a.L, m=a.m, sectors=sectors, symm=a.symm, decay=a.decay)
     ham, psi, Lb, Rb, W, site = sp_.ham, sp_.psi, sp_.L, sp_.R, sp_.W, sp_.site
     log(f"setup {time.perf_counter() - t0:.1f}s: site {site}, psi {psi.to_flat().size}, W {W.row_dim}x{W.col_dim}")
-    t0 = time.perf_counter()
-    sp = eng.sigma_plan(ham.symm, psi, Lb, Rb, W)
-    t_plan = time.perf_counter() - t0
-    log(f"plan {t_plan:.2f}s: F {sp.ref_flops / 1e9:.3f} GF, executed {sp.exec_flops / 1e9:.3f} GF, "
-        f"workspace {sp.workspace_bytes / 1e9:.2f} GB")
+    # schedule per form (fused-column rule, split-K chunk): tuned on this problem
+    tuned = {}
+    for name, prec in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)):
+        if a.no_autotune:
+            tuned[name] = (DEFAULT_SCHEDULE, [])
+        else:
+            t0 = time.perf_counter()
+            tuned[name] = tune(eng, ham.symm, psi, Lb, Rb, W, prec=prec)
+            log(f"autotune {name}: {tuned[name][0].label()} ({tuned[name][1][0][1]:.3f} ms; default "
+                f"{[r[1] for r in tuned[name][1] if r[0] == DEFAULT_SCHEDULE][0]:.3f} ms) "
+                f"in {time.perf_counter() - t0:.1f}s")
+    plans = {}
+    t_plan = 0.0
+    for name in ("fp64", "mixed"):
+        eng.set_schedule(tuned[name][0])
+        t0 = time.perf_counter()
+        plans[name] = eng.sigma_plan(ham.symm, psi, Lb, Rb, W)
+        t_plan = max(t_plan, time.perf_counter() - t0)
+        log(f"{name} plan {time.perf_counter() - t0:.2f}s: F {plans[name].ref_flops / 1e9:.3f} GF, executed "
+            f"{plans[name].exec_flops / 1e9:.3f} GF, workspace {plans[name].workspace_bytes / 1e9:.2f} GB")
+    sp = plans["fp64"]
     x = eng.to_dev(psi.to_flat())
     y = torch.empty_like(x)
 
     def measure(prec):
+        sp = plans["fp64" if prec == MQ_PREC_FP64 else "mixed"]
         torch.cuda.synchronize()
         for _ in range(a.warmup):
             sp.apply(x, y, prec)
@@ -168,7 +190,7 @@ def main():
             _, u, _ = sp.jd(u, 20, 1e-12, prec=MQ_PREC_FP64)
         conv = {}
         for name, prec in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)):
-            th, _, its = sp.jd(u, 10, 1e-10, prec=prec)
+            th, _, its = plans[name].jd(u, 10, 1e-10, prec=prec)
             conv[name] = dict(theta=th + ham.ecore, iterations=its)
         dE = abs(conv["mixed"]["theta"] - conv["fp64"]["theta"])
         log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it) mixed "
@@ -242,8 +264,10 @@ def main():
         "ms_per_davidson_iter": ch["ms_jd"],
         "davidson_iters": ch["its"],
         "flops_per_sigma": sp.ref_flops,
-        "executed_flops_per_sigma": sp.exec_flops,
+        "executed_flops_per_sigma": plans[prec_name].exec_flops,
         "plan_build_s": t_plan,
+        "schedule": {k: asdict(v[0]) for k, v in tuned.items()},
+        "autotune": None if a.no_autotune else {k: table_json(v[1][:5]) for k, v in tuned.items()},
         "fp64": summary(r64),
         "mixed": summary(r32),
         "mixed_speedup_vs_fp64": r64["dt"] / r32["dt"],
@@ -255,7 +279,8 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    sp.close()
+    for p_ in plans.values():
+        p_.close()
     eng.close()
 
 

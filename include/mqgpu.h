@@ -151,8 +151,12 @@ const char *mq_last_error(const mq_ctx *ctx);
  * column is summed that way (testing aid).  Under mode 1 a column is summed
  * that way when its entries number at least MQ_OPT_FUSE_RATIO (1..255,
  * default 4) times its distinct positions and at least MQ_OPT_FUSE_MIN
- * (1..4095, default 64).  Results agree to round-off. */
-enum mq_option { MQ_OPT_FUSE = 0, MQ_OPT_FUSE_RATIO = 1, MQ_OPT_FUSE_MIN = 2 };
+ * (1..4095, default 64).  MQ_OPT_KCHUNK: target K chunk of split-K GEMM
+ * groups (256..65536, a multiple of 16, default 2048).  Results agree to
+ * round-off.  MQ_OPT_GRAPH (0 default / 1): the Davidson drivers run their
+ * sigma vectors through mq_sigma_graph's captured graph (same results). */
+enum mq_option { MQ_OPT_FUSE = 0, MQ_OPT_FUSE_RATIO = 1, MQ_OPT_FUSE_MIN = 2, MQ_OPT_KCHUNK = 3,
+                 MQ_OPT_GRAPH = 4 };
 int mq_ctx_set_option(mq_ctx *ctx, int option, int value);
 /* HIP stream (hipStream_t) the context launches on. */
 void *mq_ctx_stream(mq_ctx *ctx);
@@ -177,6 +181,13 @@ double mq_plan_exec_flops(const mq_plan *plan);
  * given (L, R) pointer pair and reused while the pair stays the same. */
 int mq_sigma(mq_plan *plan, const double *L, const double *R,
              const double *x, double *y, int prec);
+/* y = H_eff x like mq_sigma, through a HIP graph of the sigma stages captured
+ * on the plan's own staging vectors (x is copied in, y copied out); the graph
+ * is captured on the first call and again when L, R or prec change, and
+ * replayed otherwise (no reference counterpart: for callers that replay the
+ * same problem many times). */
+int mq_sigma_graph(mq_plan *plan, const double *L, const double *R,
+                   const double *x, double *y, int prec);
 /* Drop the rearranged boundaries (call after changing L or R in place). */
 int mq_plan_rebind(mq_plan *plan);
 /* Instrumented mq_sigma (no reference counterpart): HIP events around each

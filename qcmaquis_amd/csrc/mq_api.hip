@@ -15,6 +15,8 @@ struct mq_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     FuseRule fuse;  // MQ_OPT_FUSE, MQ_OPT_FUSE_RATIO, MQ_OPT_FUSE_MIN
+    int kchunk = 2048;  // MQ_OPT_KCHUNK
+    int graph = 0;      // MQ_OPT_GRAPH
 };
 
 namespace {
@@ -93,7 +95,13 @@ struct mq_plan {
     // pinned host staging of the Davidson coefficients / pointers / partial sums
     double *h_coef = nullptr, *h_partial = nullptr;
     const double **h_ptrs = nullptr;
+    // captured sigma stages on fixed staging vectors (mq_sigma_graph)
+    hipGraphExec_t gexec = nullptr;
+    int gprec = -1;
+    const double *gL = nullptr, *gR = nullptr;
+    DevArray<double> gx, gy;
     ~mq_plan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (h_coef) (void)hipHostFree(h_coef);
         if (h_partial) (void)hipHostFree(h_partial);
         if (h_ptrs) (void)hipHostFree((void *)h_ptrs);
@@ -145,6 +153,14 @@ int mq_ctx_set_option(mq_ctx *ctx, int option, int value) {
         ctx->fuse.min_entries = value;
         return MQ_OK;
     }
+    if (option == MQ_OPT_KCHUNK && value >= 256 && value <= 65536 && value % 16 == 0) {
+        ctx->kchunk = value;
+        return MQ_OK;
+    }
+    if (option == MQ_OPT_GRAPH && (value == 0 || value == 1)) {
+        ctx->graph = value;
+        return MQ_OK;
+    }
     ctx->err = "mq_ctx_set_option: unknown option or value";
     return MQ_ERR_ARG;
 }
@@ -167,7 +183,7 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         check(hipSetDevice(ctx->device), "hipSetDevice");
         std::unique_ptr<mq_plan> p(new mq_plan());
         p->ctx = ctx;
-        const int tm = 64, tn = 64, kc = 2048;
+        const int tm = 64, tn = 64, kc = ctx->kchunk;
         if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse);
         else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse);
         else if (kind == PLAN_RSTEP) p->hp = build_rstep_plan(*desc, tm, tn, kc);
@@ -318,6 +334,50 @@ static void sigma_launch(mq_plan *p, const double *L, const double *R, const dou
     sigma_stages(p, x, y, prec, nullptr);
 }
 
+// The sigma stages captured once into a HIP graph on the plan's staging vectors
+// (gx -> gy) and replayed: the graph holds kernel launches only (no memset or
+// host work), and is captured again when the boundaries or the precision change.
+static void sigma_graph_launch(mq_plan *p, const double *L, const double *R, const double *x, double *y,
+                               int prec) {
+    hipStream_t s = p->ctx->stream;
+    bind(p, L, R, s);
+    if (prec != MQ_PREC_FP64) bind32(p, s);
+    const int64_t n = p->hp.vec_len;
+    if (!p->gx.p && n) {
+        p->gx.alloc(n);
+        p->gy.alloc(n);
+    }
+    if (!p->gexec || p->gL != L || p->gR != R || p->gprec != prec) {
+        if (p->gexec) check(hipGraphExecDestroy(p->gexec), "graph destroy");
+        p->gexec = nullptr;
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            sigma_stages(p, p->gx.p, p->gy.p, prec, nullptr);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        check(hipStreamEndCapture(s, &graph), "end capture");
+        hipError_t e = hipGraphInstantiate(&p->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check(e, "graph instantiate");
+        p->gL = L;
+        p->gR = R;
+        p->gprec = prec;
+    }
+    check(hipMemcpyAsync(p->gx.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "graph input");
+    check(hipGraphLaunch(p->gexec, s), "graph launch");
+    check(hipMemcpyAsync(y, p->gy.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "graph output");
+}
+
+// sigma of the Davidson drivers: eager launches, or the captured graph (MQ_OPT_GRAPH)
+static void solver_sigma(mq_plan *p, const double *L, const double *R, const double *x, double *y, int prec) {
+    if (p->ctx->graph) sigma_graph_launch(p, L, R, x, y, prec);
+    else sigma_launch(p, L, R, x, y, prec);
+}
+
 static const char *prec_error(int prec) {
     if (prec == MQ_PREC_FP64 || prec == MQ_PREC_FP32) return nullptr;
     if (prec == MQ_PREC_BF16X3) return "MQ_PREC_BF16X3 is not implemented";
@@ -336,6 +396,24 @@ int mq_sigma(mq_plan *plan, const double *L, const double *R, const double *x, d
     }
     try {
         sigma_launch(plan, L, R, x, y, prec);
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+int mq_sigma_graph(mq_plan *plan, const double *L, const double *R, const double *x, double *y, int prec) {
+    if (!plan || !x || !y) return MQ_ERR_ARG;
+    if (plan->hp.kind != PLAN_SIGMA) {
+        plan->ctx->err = "mq_sigma_graph: plan is not a sigma plan";
+        return MQ_ERR_ARG;
+    }
+    if (const char *e = prec_error(prec)) {
+        plan->ctx->err = std::string("mq_sigma_graph: ") + e;
+        return MQ_ERR_ARG;
+    }
+    try {
+        sigma_graph_launch(plan, L, R, x, y, prec);
         return MQ_OK;
     } catch (std::exception &e) {
         return fail(plan->ctx, e, MQ_ERR_HIP);
@@ -425,6 +503,7 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const 
 int mq_plan_rebind(mq_plan *plan) {
     if (!plan) return MQ_ERR_ARG;
     plan->bound_L = plan->bound_R = nullptr;
+    plan->gL = plan->gR = nullptr;
     return MQ_OK;
 }
 
@@ -701,7 +780,7 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             }
             const double nt = std::sqrt(nt2);
             jd.lincomb({t}, {1.0 / nt - 1.0}, 1.0, t);
-            sigma_launch(plan, L, R, t, VA(it), prec);
+            solver_sigma(plan, L, R, t, VA(it), prec);
             // Rayleigh column (jacobi.h:401-402)
             xs.clear();
             ys.clear();
@@ -880,7 +959,7 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
                 nt2 = jd.dot(t, t);
             }
             jd.lincomb({t}, {1.0 / std::sqrt(nt2) - 1.0}, 1.0, t);
-            sigma_launch(plan, L, R, t, VA(dim), prec);
+            solver_sigma(plan, L, R, t, VA(dim), prec);
             ++dim;
             // new Rayleigh column (davidson.h:104-106)
             xs.clear();

@@ -51,6 +51,7 @@ def lib():
         L.mq_plan_exec_flops.restype = C.c_double
         L.mq_sigma.argtypes = [vp, vp, vp, vp, vp, C.c_int]
         L.mq_sigma_timed.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_float), f64p]
+        L.mq_sigma_graph.argtypes = [vp, vp, vp, vp, vp, C.c_int]
         L.mq_plan_rebind.argtypes = [vp]
         L.mq_gemm_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
                                   C.c_int, C.c_int]
@@ -156,7 +157,12 @@ class GpuEngine:
         self.ctx = C.c_void_p()
         if lib().mq_ctx_create(device, C.byref(self.ctx)) != 0:
             raise MQError("mq_ctx_create failed")
-        self.set_fuse(fuse)
+        # the library's stream as a torch stream: work on it is ordered against
+        # torch's current stream with events (enter: the library waits for the
+        # inputs torch produced; leave: torch waits for the library's outputs)
+        self.stream = torch.cuda.ExternalStream(lib().mq_ctx_stream(self.ctx), device=self.device)
+        from .autotune import Schedule
+        self.set_schedule(Schedule(fuse=fuse))
         self.prec = prec
         self.flops = 0.0
         self.stats = []
@@ -165,6 +171,23 @@ class GpuEngine:
         """Fused-column rule of the plans built afterwards (MQ_OPT_FUSE*)."""
         for opt, v in ((0, mode), (1, ratio), (2, min_entries)):
             self.check(lib().mq_ctx_set_option(self.ctx, opt, v), "mq_ctx_set_option")
+
+    def enter(self):
+        self.stream.wait_stream(self.torch.cuda.current_stream(self.device))
+
+    def leave(self):
+        self.torch.cuda.current_stream(self.device).wait_stream(self.stream)
+
+    def set_graph(self, on):
+        """Davidson drivers run their sigma vectors through captured HIP graphs (MQ_OPT_GRAPH)."""
+        self.check(lib().mq_ctx_set_option(self.ctx, 4, 1 if on else 0), "mq_ctx_set_option")
+
+    def set_schedule(self, rule):
+        """Plan-construction schedule of the plans built afterwards: a
+        Schedule (fused-column rule and split-K chunk)."""
+        self.set_fuse(rule.fuse, rule.ratio, rule.min_entries)
+        self.check(lib().mq_ctx_set_option(self.ctx, 3, rule.kchunk), "mq_ctx_set_option")
+        self.schedule = rule
 
     def close(self):
         if self.ctx:
@@ -324,8 +347,22 @@ class SigmaPlan:
         if y is None:
             y = t.empty(self.n, dtype=t.float64, device=self.eng.device)
         prec = self.eng.prec if prec is None else prec
+        self.eng.enter()
         self.eng.check(lib().mq_sigma(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y), prec),
                        "mq_sigma")
+        self.eng.leave()
+        return y
+
+    def apply_graph(self, x, y=None, prec=None):
+        """y = H x through the plan's captured HIP graph (mq_sigma_graph)."""
+        t = self.eng.torch
+        if y is None:
+            y = t.empty(self.n, dtype=t.float64, device=self.eng.device)
+        prec = self.eng.prec if prec is None else prec
+        self.eng.enter()
+        self.eng.check(lib().mq_sigma_graph(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y), prec),
+                       "mq_sigma_graph")
+        self.eng.leave()
         return y
 
     def timed(self, x, y, prec=None):
@@ -338,6 +375,7 @@ class SigmaPlan:
         ms = (C.c_float * 5)()
         work = (C.c_double * 6)()
         prec = self.eng.prec if prec is None else prec
+        self.eng.enter()
         self.eng.check(lib().mq_sigma_timed(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y), prec,
                                             ms, work), "mq_sigma_timed")
         return list(ms), list(work)
@@ -348,8 +386,9 @@ class SigmaPlan:
         if getattr(self, "_dp", None) is None:
             self._dp = self.eng._plan("diag", self.site)
         d = t.empty(self.n, dtype=t.float64, device=self.eng.device)
-        t.cuda.synchronize()
+        self.eng.enter()
         self.eng.check(lib().mq_diagonal(self._dp, _ptr(self.L.data), _ptr(self.R.data), _ptr(d)), "mq_diagonal")
+        self.eng.leave()
         return d
 
     def davidson(self, x0, max_iter=10, tol=1e-8, ortho=(), prec=None, diag=None):

@@ -112,3 +112,13 @@ def test_davidson_is_su2_only(fixtures, oracle_engine):
         oracle_engine.diagonal(ham.symm, psi, L, R, W)
     with pytest.raises(MQError, match="spin-adapted"):
         HostPlan(SiteDesc(ham.symm, psi.phys, psi.left, psi.right, W, L.layout, R.layout), "diag")
+
+
+@pytest.mark.parametrize("name", ["h2", "lih"])
+def test_sweeps_with_davidson_reach_golden_energies(fixtures, oracle_engine, name):
+    """Two-site sweeps driven by the Davidson eigensolver (IETL_DAVIDSON) reach
+    the reference's golden energies (tests/test1.cpp:93, tests/test2.cpp:57)."""
+    from qcmaquis_amd.sweep import TwoSiteDMRG
+    fx = fixtures[name]
+    E = TwoSiteDMRG(ham_from_fixture(fx), oracle_engine, 100, eigensolver="davidson").run(2)
+    assert abs(E - fx["energy"]) <= fx["rtol"] * abs(fx["energy"]), (E, fx["energy"])

@@ -122,3 +122,12 @@ def test_davidson_gpu_rejects_twou1(fixtures, oracle_engine, gpu_engine):
             sp.davidson(gpu_engine.to_dev(psi.to_flat()), 10, 1e-8, diag=gpu_engine.to_dev(np.ones(psi.to_flat().size)))
     finally:
         sp.close()
+
+
+@pytest.mark.parametrize("name", ["h2", "lih", "benzene"])
+def test_golden_energies_gpu_davidson(fixtures, gpu_engine, name):
+    from conftest import ham_from_fixture
+    from qcmaquis_amd.sweep import TwoSiteDMRG
+    fx = fixtures[name]
+    E = TwoSiteDMRG(ham_from_fixture(fx), gpu_engine, 100, eigensolver="davidson").run(2)
+    assert abs(E - fx["energy"]) <= fx["rtol"] * abs(fx["energy"]), (E, fx["energy"])

@@ -1,0 +1,82 @@
+"""Captured HIP graphs of the sigma stages and the schedule autotuner, on the
+GPU: both change how the work is issued, never the result."""
+import numpy as np
+import pytest
+
+from conftest import ham_from_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _benzene(oracle_engine, gpu_engine, fixtures):
+    from qcmaquis_amd.sweep import TwoSiteDMRG
+    ham = ham_from_fixture(fixtures["benzene"])
+    d = TwoSiteDMRG(ham, oracle_engine, 60)
+    for i in range(3):
+        d.microiteration(0, i)
+    psi, L, R, W = d.site_problem(3)
+    return ham, psi, gpu_engine.upload_boundary(L.layout, L.data), gpu_engine.upload_boundary(R.layout, R.data), W
+
+
+def test_sigma_graph_replays_match_eager(fixtures, oracle_engine, gpu_engine):
+    """Several replays with new inputs, eager sigma vectors and host syncs in
+    between, then a precision change (new capture): every replay equals the
+    eager result bit for bit."""
+    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
+    sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+    try:
+        rng = np.random.default_rng(11)
+        for prec in (MQ_PREC_FP64, MQ_PREC_FP32, MQ_PREC_FP64):
+            for rep in range(4):
+                x = gpu_engine.to_dev(rng.standard_normal(sp.n))
+                print(f"graph replay prec {prec} rep {rep}", flush=True)
+                yg = sp.apply_graph(x, prec=prec).cpu().numpy()
+                other = gpu_engine.to_dev(rng.standard_normal(sp.n))
+                sp.apply(other, prec=prec)
+                gpu_engine.sync()
+                ye = sp.apply(x, prec=prec).cpu().numpy()
+                assert np.array_equal(yg, ye), (prec, rep, np.abs(yg - ye).max())
+    finally:
+        sp.close()
+
+
+def test_davidson_drivers_with_graphs(fixtures, oracle_engine, gpu_engine):
+    ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
+    x = psi.to_flat()
+    res = {}
+    try:
+        for on in (False, True):
+            gpu_engine.set_graph(on)
+            sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+            try:
+                xd = gpu_engine.to_dev(x)
+                res[on] = (sp.jd(xd, 12, 1e-12)[::2], sp.davidson(xd, 40, 1e-12)[::2])
+            finally:
+                sp.close()
+    finally:
+        gpu_engine.set_graph(False)
+    (j0, d0), (j1, d1) = res[False], res[True]
+    assert j0[1] == j1[1] and d0[1] == d1[1]
+    assert abs(j0[0] - j1[0]) <= 1e-13 * abs(j0[0])
+    assert abs(d0[0] - d1[0]) <= 1e-13 * abs(d0[0])
+
+
+def test_autotune_picks_a_schedule_with_the_same_sigma(fixtures, oracle_engine, gpu_engine):
+    from qcmaquis_amd.autotune import DEFAULT, candidates, tune
+    from qcmaquis_amd.desc import MQ_PREC_FP64
+    ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
+    best, table = tune(gpu_engine, ham.symm, psi, Lg, Rg, W, prec=MQ_PREC_FP64, reps=2)
+    assert len(table) == len(candidates()) and table[0][0] == best
+    assert all(table[i][1] <= table[i + 1][1] for i in range(len(table) - 1))
+    x = gpu_engine.to_dev(np.random.default_rng(3).standard_normal(psi.to_flat().size))
+    ys = []
+    for sched in (DEFAULT, best):
+        gpu_engine.set_schedule(sched)
+        sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+        try:
+            ys.append(sp.apply(x).cpu().numpy())
+        finally:
+            sp.close()
+    gpu_engine.set_schedule(DEFAULT)
+    assert np.abs(ys[0] - ys[1]).max() <= 1e-13 * np.abs(ys[0]).max()
