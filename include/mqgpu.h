@@ -181,6 +181,14 @@ double mq_plan_exec_flops(const mq_plan *plan);
  * given (L, R) pointer pair and reused while the pair stays the same. */
 int mq_sigma(mq_plan *plan, const double *L, const double *R,
              const double *x, double *y, int prec);
+/* Y(:, j) = H_eff X(:, j), j < nvec (1..64): nvec device vectors stored one
+ * after another (vector j at X + j*vec_len), with one launch per stage for all
+ * of them; the arranged boundary panels are read once per tile for all
+ * vectors.  Each Y(:, j) equals mq_sigma of X(:, j) (no reference
+ * counterpart: the several Davidson roots of a block solver in one pass).
+ * The plan keeps per-vector work arenas for the last nvec used. */
+int mq_sigma_batch(mq_plan *plan, const double *L, const double *R,
+                   const double *X, double *Y, int nvec, int prec);
 /* y = H_eff x like mq_sigma, through a HIP graph of the sigma stages captured
  * on the plan's own staging vectors (x is copied in, y copied out); the graph
  * is captured on the first call and again when L, R or prec change, and

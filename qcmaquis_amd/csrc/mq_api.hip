@@ -95,6 +95,19 @@ struct mq_plan {
     // pinned host staging of the Davidson coefficients / pointers / partial sums
     double *h_coef = nullptr, *h_partial = nullptr;
     const double **h_ptrs = nullptr;
+    // several vectors per launch (mq_sigma_batch): replicated task lists and
+    // per-vector copies of the psi, T and C arenas, built for the last batch size
+    struct Batch {
+        int nvec = 0;
+        int64_t nstrips = 0, ncopies = 0;
+        DevArray<Copy2D> reshape;
+        DevArray<Strip> strips;
+        DevArray<Src> srcs;
+        DevGemm g1, g2;
+        DevArray<double> prp, t, c, ws;
+        DevArray<float> prp32, t32, c32, ws32;
+        bool have32 = false;
+    } batch;
     // captured sigma stages on fixed staging vectors (mq_sigma_graph)
     hipGraphExec_t gexec = nullptr;
     int gprec = -1;
@@ -334,6 +347,70 @@ static void sigma_launch(mq_plan *p, const double *L, const double *R, const dou
     sigma_stages(p, x, y, prec, nullptr);
 }
 
+// Task lists and arenas of a batch of nvec vectors (rebuilt when nvec changes).
+static void batch_setup(mq_plan *p, int nvec, bool f32) {
+    HostPlan &h = p->hp;
+    auto &B = p->batch;
+    hipStream_t s = p->ctx->stream;
+    if (B.nvec != nvec) {
+        B.nvec = 0;
+        B.have32 = false;  // every array below is reallocated (alloc frees the old one)
+        B.reshape.upload(batch_copies(h.reshape, nvec, h.vec_len, h.n_prp), s);
+        B.ncopies = (int64_t)h.reshape.size() * nvec;
+        StripList sl = batch_strips(h.wapply, nvec, h.n_t, h.n_c);
+        B.strips.upload(sl.strips, s);
+        B.srcs.upload(sl.srcs, s);
+        B.nstrips = (int64_t)sl.strips.size();
+        GemmList g1 = batch_gemm(h.g1, nvec, 0, h.n_prp, h.n_t);
+        GemmList g2 = batch_gemm(h.g2, nvec, h.n_c, 0, h.vec_len);
+        B.g1.upload(g1, s);
+        B.g2.upload(g2, s);
+        B.prp.alloc((size_t)h.n_prp * nvec);
+        B.t.alloc((size_t)h.n_t * nvec);
+        B.c.alloc((size_t)h.n_c * nvec);
+        B.ws.alloc(std::max(g1.ws, g2.ws));
+        check(hipStreamSynchronize(s), "batch upload");
+        B.nvec = nvec;
+    }
+    if (f32 && !B.have32) {
+        B.prp32.alloc((size_t)h.n_prp * nvec);
+        B.t32.alloc((size_t)h.n_t * nvec);
+        B.c32.alloc((size_t)h.n_c * nvec);
+        B.ws32.alloc(B.ws.n);
+        B.have32 = true;
+    }
+}
+
+// Y(:, j) = H X(:, j) for nvec vectors stored one after another: one launch per
+// stage for all of them; the LA and RB panels are shared.
+static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, const double *X, double *Y, int nvec,
+                               int prec) {
+    hipStream_t s = p->ctx->stream;
+    bind(p, L, R, s);
+    if (prec != MQ_PREC_FP64) bind32(p, s);
+    batch_setup(p, nvec, prec != MQ_PREC_FP64);
+    auto &B = p->batch;
+    if (prec == MQ_PREC_FP64) {
+        check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s), "reshape");
+        check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la.p, B.prp.p,
+                          B.t.p, B.ws.p, ST_SIGMA, s),
+              "gemm T");
+        check(launch_wapply(B.strips.p, (int)B.nstrips, B.srcs.p, B.t.p, B.c.p, ST_SIGMA, s), "W apply");
+        check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c.p, p->rb.p, Y,
+                          B.ws.p, ST_SIGMA_OUT, s),
+              "gemm R");
+    } else {
+        check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp32.p, s), "reshape");
+        check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la32.p,
+                          B.prp32.p, B.t32.p, B.ws32.p, ST_SIGMA, s),
+              "gemm T");
+        check(launch_wapply(B.strips.p, (int)B.nstrips, B.srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s), "W apply");
+        check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c32.p,
+                          p->rb32.p, Y, B.ws32.p, ST_SIGMA_OUT, s),
+              "gemm R");
+    }
+}
+
 // The sigma stages captured once into a HIP graph on the plan's staging vectors
 // (gx -> gy) and replayed: the graph holds kernel launches only (no memset or
 // host work), and is captured again when the boundaries or the precision change.
@@ -396,6 +473,25 @@ int mq_sigma(mq_plan *plan, const double *L, const double *R, const double *x, d
     }
     try {
         sigma_launch(plan, L, R, x, y, prec);
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+int mq_sigma_batch(mq_plan *plan, const double *L, const double *R, const double *X, double *Y, int nvec,
+                   int prec) {
+    if (!plan || !X || !Y || nvec < 1 || nvec > 64) return MQ_ERR_ARG;
+    if (plan->hp.kind != PLAN_SIGMA) {
+        plan->ctx->err = "mq_sigma_batch: plan is not a sigma plan";
+        return MQ_ERR_ARG;
+    }
+    if (const char *e = prec_error(prec)) {
+        plan->ctx->err = std::string("mq_sigma_batch: ") + e;
+        return MQ_ERR_ARG;
+    }
+    try {
+        sigma_batch_launch(plan, L, R, X, Y, nvec, prec);
         return MQ_OK;
     } catch (std::exception &e) {
         return fail(plan->ctx, e, MQ_ERR_HIP);

@@ -198,15 +198,28 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
 constexpr int kXcds = 8;
 
 std::vector<GemmTile> xcd_order(const std::vector<GemmTile> &tiles, const std::vector<GemmGroup> &groups) {
-    // chains: consecutive tiles of one (group, K chunk) sharing an operand panel --
-    // the same row block, or, in groups with a single column tile, the same column
-    // block (cut into chains of at most 16 tiles so the chains still balance)
+    // chains: consecutive tiles that read one operand block -- the same A row block
+    // (A offset, row tile, K range), or, in groups with a single column tile, the
+    // same B column block (B offset, K range; cut into chains of at most 16 tiles so
+    // the chains still balance).  Tiles of different groups that read the same
+    // block (the per-vector copies of a batched plan) join one chain.
+    auto kbeg = [&](const GemmTile &t) { return (int64_t)t.ks * groups[t.g].kchunk; };
+    auto same_a = [&](const GemmTile &x, const GemmTile &y) {
+        const GemmGroup &gx = groups[x.g], &gy = groups[y.g];
+        return gx.a == gy.a && gx.lda == gy.lda && gx.transA == gy.transA && gx.tm == gy.tm && x.mt == y.mt &&
+               kbeg(x) == kbeg(y);
+    };
+    auto same_b = [&](const GemmTile &x, const GemmTile &y) {
+        const GemmGroup &gx = groups[x.g], &gy = groups[y.g];
+        return gx.b == gy.b && gx.ldb == gy.ldb && gx.transB == gy.transB && gx.tn == gy.tn && x.nt == y.nt &&
+               kbeg(x) == kbeg(y);
+    };
     std::vector<std::pair<size_t, size_t>> chains;
     for (size_t i = 0; i < tiles.size();) {
         size_t j = i + 1;
         const bool along_m = groups[tiles[i].g].N <= groups[tiles[i].g].tn;
-        while (j < tiles.size() && tiles[j].g == tiles[i].g && tiles[j].ks == tiles[i].ks &&
-               (along_m ? (j - i < 16) : tiles[j].mt == tiles[i].mt))
+        while (j < tiles.size() &&
+               (along_m ? (same_b(tiles[j], tiles[i]) && j - i < 16) : same_a(tiles[j], tiles[i])))
             ++j;
         chains.push_back({i, j});
         i = j;
@@ -824,6 +837,66 @@ void order_tiles(GemmList &gl) {
     for (auto &x : gl.tiles)
         if (x.g >= 0) t.push_back(x);
     gl.tiles = xcd_order(t, gl.groups);
+}
+
+// ---------------- several vectors per launch ------------------------------
+std::vector<Copy2D> batch_copies(const std::vector<Copy2D> &v, int nvec, int64_t src_step, int64_t dst_step) {
+    std::vector<Copy2D> out;
+    out.reserve(v.size() * nvec);
+    for (int j = 0; j < nvec; ++j)
+        for (Copy2D c : v) {
+            c.src += j * src_step;
+            c.dst += j * dst_step;
+            out.push_back(c);
+        }
+    return out;
+}
+
+StripList batch_strips(const StripList &sl, int nvec, int64_t src_step, int64_t dst_step) {
+    StripList out;
+    out.strips.reserve(sl.strips.size() * nvec);
+    out.srcs.reserve(sl.srcs.size() * nvec);
+    for (int j = 0; j < nvec; ++j) {
+        const int base = (int)out.srcs.size();
+        for (Src x : sl.srcs) {
+            x.off += j * src_step;
+            out.srcs.push_back(x);
+        }
+        for (Strip st : sl.strips) {
+            st.dst += j * dst_step;
+            st.s0 += base;
+            st.s1 += base;
+            out.strips.push_back(st);
+        }
+    }
+    return out;
+}
+
+GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step, int64_t c_step) {
+    GemmList out;
+    const int G = (int)gl.groups.size();
+    for (int j = 0; j < nvec; ++j)
+        for (GemmGroup g : gl.groups) {
+            g.a += j * a_step;
+            g.b += j * b_step;
+            g.c += j * c_step;
+            g.part += j * gl.ws;
+            out.groups.push_back(g);
+        }
+    out.ws = gl.ws * nvec;
+    // the single-vector list is XCD-interleaved (tile i on XCD i % 8): recover each
+    // XCD's sequence, put the per-vector copies of a tile next to each other, order again
+    std::vector<GemmTile> seq;
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t i = x; i < gl.tiles.size(); i += kXcds)
+            if (gl.tiles[i].g >= 0)
+                for (int j = 0; j < nvec; ++j) {
+                    GemmTile t = gl.tiles[i];
+                    t.g += j * G;
+                    seq.push_back(t);
+                }
+    out.tiles = seq.empty() ? seq : xcd_order(seq, out.groups);
+    return out;
 }
 
 // ============================================================ sigma

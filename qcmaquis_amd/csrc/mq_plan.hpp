@@ -119,6 +119,13 @@ void tile_gemm(GemmList &gl, GemmGroup g, int tile_m, int tile_n, int kchunk_tar
 // XCD-aware tile order (call once every group has been added).
 void order_tiles(GemmList &gl);
 
+// Task lists for nvec vectors in one launch per stage: vector j's copies are the
+// single-vector tasks with their per-vector arena offsets advanced by j steps
+// (a step of 0 marks an operand all vectors share).
+std::vector<Copy2D> batch_copies(const std::vector<Copy2D> &v, int nvec, int64_t src_step, int64_t dst_step);
+StripList batch_strips(const StripList &sl, int nvec, int64_t src_step, int64_t dst_step);
+GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step, int64_t c_step);
+
 // FLOPs of the reference schedule (site_hamil.hpp:28-204 for sigma, move_boundary.hpp for steps)
 double reference_sigma_flops(const mq_site_desc &d);
 

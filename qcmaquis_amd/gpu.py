@@ -52,6 +52,7 @@ def lib():
         L.mq_sigma.argtypes = [vp, vp, vp, vp, vp, C.c_int]
         L.mq_sigma_timed.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_float), f64p]
         L.mq_sigma_graph.argtypes = [vp, vp, vp, vp, vp, C.c_int]
+        L.mq_sigma_batch.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int]
         L.mq_plan_rebind.argtypes = [vp]
         L.mq_gemm_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
                                   C.c_int, C.c_int]
@@ -352,6 +353,20 @@ class SigmaPlan:
                        "mq_sigma")
         self.eng.leave()
         return y
+
+    def apply_batch(self, X, Y=None, prec=None):
+        """Y[j] = H X[j] for the rows of a (nvec, n) device tensor, one launch per stage."""
+        t = self.eng.torch
+        X = X.contiguous()
+        nvec = X.shape[0]
+        if Y is None:
+            Y = t.empty((nvec, self.n), dtype=t.float64, device=self.eng.device)
+        prec = self.eng.prec if prec is None else prec
+        self.eng.enter()
+        self.eng.check(lib().mq_sigma_batch(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(X), _ptr(Y), nvec,
+                                            prec), "mq_sigma_batch")
+        self.eng.leave()
+        return Y
 
     def apply_graph(self, x, y=None, prec=None):
         """y = H x through the plan's captured HIP graph (mq_sigma_graph)."""

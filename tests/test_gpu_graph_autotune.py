@@ -80,3 +80,22 @@ def test_autotune_picks_a_schedule_with_the_same_sigma(fixtures, oracle_engine, 
             sp.close()
     gpu_engine.set_schedule(DEFAULT)
     assert np.abs(ys[0] - ys[1]).max() <= 1e-13 * np.abs(ys[0]).max()
+
+
+@pytest.mark.parametrize("nvec", [1, 3, 4])
+def test_sigma_batch_equals_single_vectors(fixtures, oracle_engine, gpu_engine, nvec):
+    """mq_sigma_batch: each output row equals mq_sigma of its input row (the
+    same tiles and K chunks per vector), in FP64 and in the mixed form."""
+    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
+    sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+    try:
+        X = gpu_engine.torch.as_tensor(np.random.default_rng(nvec).standard_normal((nvec, sp.n))).to(gpu_engine.device)
+        for prec in (MQ_PREC_FP64, MQ_PREC_FP32):
+            print(f"batch nvec {nvec} prec {prec}", flush=True)
+            Y = sp.apply_batch(X, prec=prec).cpu().numpy()
+            for j in range(nvec):
+                y = sp.apply(X[j].contiguous(), prec=prec).cpu().numpy()
+                assert np.array_equal(Y[j], y), (prec, j, np.abs(Y[j] - y).max())
+    finally:
+        sp.close()
