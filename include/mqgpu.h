@@ -256,6 +256,25 @@ int mq_davidson(mq_plan *plan, const double *L, const double *R,
                 int max_iter, double tol, int prec,
                 const double *const *ortho, int n_ortho, int *iterations);
 
+/* The nroots (1..16) lowest eigenpairs of H_eff together (no counterpart in
+ * the reference, which finds excited states one at a time under
+ * orthogonality constraints): a block Davidson whose sigma vectors for a
+ * whole block go through one batched launch (mq_sigma_batch).  Corrections
+ * are t = -r + (r.u)/(u.u) u (as mq_jacobi_davidson) or, when diag (from
+ * mq_diagonal) is given, the diagonal preconditioner of mq_davidson; the
+ * basis holds up to 60 vectors and restarts from the lowest 2*nroots Ritz
+ * vectors.  X0: nroots device start vectors one after another (linearly
+ * independent); U: the nroots Ritz vectors, same layout; theta: nroots
+ * eigenvalues, ascending.  Convergence per root as mq_jacobi_davidson;
+ * max_iter counts block iterations.  With prec != MQ_PREC_FP64 every
+ * eigenvalue is the FP64 Rayleigh quotient of its Ritz vector.  ortho as for
+ * mq_jacobi_davidson. */
+int mq_block_davidson(mq_plan *plan, const double *L, const double *R,
+                      const double *diag, const double *X0, double *U,
+                      double *theta, int nroots, int max_iter, double tol,
+                      int prec, const double *const *ortho, int n_ortho,
+                      int *iterations);
+
 /* -- diagnostics (no reference counterpart) ------------------------------ */
 /* C(M x N, ldc) = op(A) op(B) with the library's grouped FP64 GEMM kernel on
  * device arrays (column-major; transA: A stored K x M, transB: B stored

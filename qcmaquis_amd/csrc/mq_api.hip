@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 
@@ -95,19 +96,23 @@ struct mq_plan {
     // pinned host staging of the Davidson coefficients / pointers / partial sums
     double *h_coef = nullptr, *h_partial = nullptr;
     const double **h_ptrs = nullptr;
-    // several vectors per launch (mq_sigma_batch): replicated task lists and
-    // per-vector copies of the psi, T and C arenas, built for the last batch size
-    struct Batch {
-        int nvec = 0;
-        int64_t nstrips = 0, ncopies = 0;
+    // several vectors per launch (mq_sigma_batch): replicated task lists per batch
+    // size, and per-vector copies of the psi, T and C arenas for the largest size used
+    struct BatchLists {
+        int64_t nstrips = 0, ncopies = 0, ws = 0;
         DevArray<Copy2D> reshape;
         DevArray<Strip> strips;
         DevArray<Src> srcs;
         DevGemm g1, g2;
-        DevArray<double> prp, t, c, ws;
+    };
+    std::map<int, std::unique_ptr<BatchLists>> blists;
+    struct BatchArenas {
+        int cap = 0;
+        int64_t ws = 0;
+        DevArray<double> prp, t, c, ws64;
         DevArray<float> prp32, t32, c32, ws32;
-        bool have32 = false;
-    } batch;
+        int cap32 = 0;
+    } barena;
     // captured sigma stages on fixed staging vectors (mq_sigma_graph)
     hipGraphExec_t gexec = nullptr;
     int gprec = -1;
@@ -347,38 +352,48 @@ static void sigma_launch(mq_plan *p, const double *L, const double *R, const dou
     sigma_stages(p, x, y, prec, nullptr);
 }
 
-// Task lists and arenas of a batch of nvec vectors (rebuilt when nvec changes).
-static void batch_setup(mq_plan *p, int nvec, bool f32) {
+// Task lists of a batch of nvec vectors (built once per size) and arenas for at
+// least nvec vectors.
+static mq_plan::BatchLists &batch_setup(mq_plan *p, int nvec, bool f32) {
     HostPlan &h = p->hp;
-    auto &B = p->batch;
     hipStream_t s = p->ctx->stream;
-    if (B.nvec != nvec) {
-        B.nvec = 0;
-        B.have32 = false;  // every array below is reallocated (alloc frees the old one)
-        B.reshape.upload(batch_copies(h.reshape, nvec, h.vec_len, h.n_prp), s);
-        B.ncopies = (int64_t)h.reshape.size() * nvec;
+    auto &slot = p->blists[nvec];
+    if (!slot) {
+        std::unique_ptr<mq_plan::BatchLists> B(new mq_plan::BatchLists());
+        B->reshape.upload(batch_copies(h.reshape, nvec, h.vec_len, h.n_prp), s);
+        B->ncopies = (int64_t)h.reshape.size() * nvec;
         StripList sl = batch_strips(h.wapply, nvec, h.n_t, h.n_c);
-        B.strips.upload(sl.strips, s);
-        B.srcs.upload(sl.srcs, s);
-        B.nstrips = (int64_t)sl.strips.size();
+        B->strips.upload(sl.strips, s);
+        B->srcs.upload(sl.srcs, s);
+        B->nstrips = (int64_t)sl.strips.size();
         GemmList g1 = batch_gemm(h.g1, nvec, 0, h.n_prp, h.n_t);
         GemmList g2 = batch_gemm(h.g2, nvec, h.n_c, 0, h.vec_len);
-        B.g1.upload(g1, s);
-        B.g2.upload(g2, s);
-        B.prp.alloc((size_t)h.n_prp * nvec);
-        B.t.alloc((size_t)h.n_t * nvec);
-        B.c.alloc((size_t)h.n_c * nvec);
-        B.ws.alloc(std::max(g1.ws, g2.ws));
+        B->g1.upload(g1, s);
+        B->g2.upload(g2, s);
+        B->ws = std::max(g1.ws, g2.ws);
         check(hipStreamSynchronize(s), "batch upload");
-        B.nvec = nvec;
+        slot = std::move(B);
     }
-    if (f32 && !B.have32) {
-        B.prp32.alloc((size_t)h.n_prp * nvec);
-        B.t32.alloc((size_t)h.n_t * nvec);
-        B.c32.alloc((size_t)h.n_c * nvec);
-        B.ws32.alloc(B.ws.n);
-        B.have32 = true;
+    auto &A = p->barena;
+    if (A.cap < nvec || A.ws < slot->ws) {
+        check(hipStreamSynchronize(s), "batch arenas");
+        const int cap = std::max(A.cap, nvec);
+        A.prp.alloc((size_t)h.n_prp * cap);
+        A.t.alloc((size_t)h.n_t * cap);
+        A.c.alloc((size_t)h.n_c * cap);
+        A.ws = std::max(A.ws, slot->ws);
+        A.ws64.alloc((size_t)A.ws);
+        A.cap = cap;
+        A.cap32 = 0;
     }
+    if (f32 && A.cap32 < A.cap) {
+        A.prp32.alloc((size_t)h.n_prp * A.cap);
+        A.t32.alloc((size_t)h.n_t * A.cap);
+        A.c32.alloc((size_t)h.n_c * A.cap);
+        A.ws32.alloc((size_t)A.ws);
+        A.cap32 = A.cap;
+    }
+    return *slot;
 }
 
 // Y(:, j) = H X(:, j) for nvec vectors stored one after another: one launch per
@@ -388,8 +403,18 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
     hipStream_t s = p->ctx->stream;
     bind(p, L, R, s);
     if (prec != MQ_PREC_FP64) bind32(p, s);
-    batch_setup(p, nvec, prec != MQ_PREC_FP64);
-    auto &B = p->batch;
+    mq_plan::BatchLists &Bl = batch_setup(p, nvec, prec != MQ_PREC_FP64);
+    auto &A = p->barena;
+    struct {
+        DevArray<Copy2D> &reshape;
+        DevArray<Strip> &strips;
+        DevArray<Src> &srcs;
+        DevGemm &g1, &g2;
+        int64_t ncopies, nstrips;
+        DevArray<double> &prp, &t, &c, &ws;
+        DevArray<float> &prp32, &t32, &c32, &ws32;
+    } B{Bl.reshape, Bl.strips, Bl.srcs, Bl.g1, Bl.g2, Bl.ncopies, Bl.nstrips,
+        A.prp, A.t, A.c, A.ws64, A.prp32, A.t32, A.c32, A.ws32};
     if (prec == MQ_PREC_FP64) {
         check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la.p, B.prp.p,
@@ -1112,6 +1137,231 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
         check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
         jd.sync();
         *theta_out = theta;
+        if (iterations) *iterations = it;
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+// ------------------------------------------------------------------ block Davidson
+// The nroots lowest eigenpairs together: a Jacobi-Davidson iteration on a block
+// (no counterpart in the reference, which solves excited states one at a time
+// with orthogonality constraints).  Each iteration orthonormalises the new
+// block against the basis (Gram-Schmidt with one refinement, as jacobi.h:384-393;
+// vectors that vanish are dropped), applies H to the whole block with one
+// batched sigma, extends the Rayleigh matrix, takes the lowest nroots Ritz pairs,
+// and adds the correction t = -r + (r.u)/(u.u) u of every root that has not
+// converged (|r| <= tol*|theta| or |r| < tol).  When the basis would exceed its
+// capacity it restarts from the current Ritz vectors.
+extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R, const double *diag,
+                                 const double *X0, double *U, double *theta_out, int nroots, int max_iter, double tol,
+                                 int prec, const double *const *ortho, int n_ortho, int *iterations) {
+    if (!plan || !X0 || !U || !theta_out || nroots < 1 || nroots > 16 || max_iter < 1 || max_iter > 100000 ||
+        n_ortho < 0 || n_ortho > 60)
+        return MQ_ERR_ARG;
+    if (plan->hp.kind != PLAN_SIGMA) {
+        plan->ctx->err = "mq_block_davidson: needs a sigma plan";
+        return MQ_ERR_ARG;
+    }
+    if (const char *e = prec_error(prec)) {
+        plan->ctx->err = std::string("mq_block_davidson: ") + e;
+        return MQ_ERR_ARG;
+    }
+    const int k = nroots;
+    const int cap = 60;  // basis capacity
+    try {
+        JD jd;
+        jd.p = plan;
+        jd.s = plan->ctx->stream;
+        jd.n = plan->hp.vec_len;
+        jd.nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kJdBlocks, (jd.n + 255) / 256));
+        jd.init();
+        const int64_t n = jd.n;
+        const int need = 2 * cap + 7 * k;
+        if (plan->jd_cap < need) {
+            plan->jd_vecs.alloc((size_t)need * n);
+            plan->jd_cap = need;
+        }
+        auto V = [&](int i) { return jd.vec(i); };
+        auto VA = [&](int i) { return jd.vec(cap + i); };
+        auto Ur = [&](int i) { return jd.vec(2 * cap + i); };          // Ritz vectors
+        auto UA = [&](int i) { return jd.vec(2 * cap + k + i); };      // their images
+        auto Rr = [&](int i) { return jd.vec(2 * cap + 2 * k + i); };  // residuals
+        auto SV = [&](int i) { return jd.vec(2 * cap + 3 * k + i); };  // restart vectors / preconditioner scratch
+        auto SA = [&](int i) { return jd.vec(2 * cap + 5 * k + i); };  // restart images
+        std::vector<double> oo(n_ortho);
+        for (int q = 0; q < n_ortho; ++q) oo[q] = jd.dot(ortho[q], ortho[q]);
+        auto project = [&](double *t) {
+            if (n_ortho == 0) return;
+            std::vector<const double *> xs, ys;
+            for (int q = 0; q < n_ortho; ++q) {
+                xs.push_back(ortho[q]);
+                ys.push_back(t);
+            }
+            std::vector<double> d = jd.dots(xs, ys), c(n_ortho);
+            for (int q = 0; q < n_ortho; ++q) c[q] = -d[q] / oo[q];
+            jd.lincomb(std::vector<const double *>(ortho, ortho + n_ortho), c, 1.0, t);
+        };
+        std::vector<double> M((size_t)cap * cap, 0.0);  // M[j * cap + i] = V_i . VA_j, i <= j
+        // orthonormalise the block V(dim .. dim+nb) in place against V(0 .. dim) and
+        // itself, compacting the kept vectors; returns how many were kept
+        auto orthonormalise = [&](int dim, int nb) {
+            int kept = 0;
+            for (int b = 0; b < nb; ++b) {
+                double *t = V(dim + b);
+                const int m = dim + kept;
+                std::vector<const double *> xs{t}, ys{t};
+                for (int i = 0; i < m; ++i) {
+                    xs.push_back(V(i));
+                    ys.push_back(t);
+                }
+                std::vector<double> d = jd.dots(xs, ys);
+                const double tau = std::sqrt(d[0]);
+                double nt2 = d[0];
+                for (int pass = 0; pass < 2 && m > 0; ++pass) {
+                    std::vector<const double *> vv;
+                    std::vector<double> c;
+                    for (int i = 0; i < m; ++i) {
+                        vv.push_back(V(i));
+                        c.push_back(-d[1 + i]);
+                    }
+                    jd.lincomb(vv, c, 1.0, t);
+                    d = jd.dots(xs, ys);
+                    nt2 = d[0];
+                    if (std::sqrt(nt2) >= 0.25 * tau) break;
+                }
+                if (n_ortho > 0) {
+                    project(t);
+                    nt2 = jd.dot(t, t);
+                }
+                if (!(std::sqrt(nt2) > 1e-10 * std::max(tau, 1e-300))) continue;  // vanished: dropped
+                double *dst = V(dim + kept);
+                if (dst != t) jd.lincomb({t}, {1.0 / std::sqrt(nt2)}, 0.0, dst);
+                else jd.lincomb({t}, {1.0 / std::sqrt(nt2) - 1.0}, 1.0, t);
+                ++kept;
+            }
+            return kept;
+        };
+        for (int j = 0; j < k; ++j)
+            check(hipMemcpyAsync(V(j), X0 + (int64_t)j * n, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "x0");
+        int dim = 0, nb = k, it = 0;
+        std::vector<double> theta(k, 0.0);
+        while (true) {
+            const int kept = orthonormalise(dim, nb);
+            if (dim + kept < k) throw std::runtime_error("block Davidson: the start vectors are linearly dependent");
+            if (kept > 0) sigma_batch_launch(plan, L, R, V(dim), VA(dim), kept, prec);
+            for (int j = dim; j < dim + kept; ++j) {
+                std::vector<const double *> xs, ys;
+                for (int i = 0; i <= j; ++i) {
+                    xs.push_back(V(i));
+                    ys.push_back(VA(j));
+                }
+                std::vector<double> col = jd.dots(xs, ys);
+                for (int i = 0; i <= j; ++i) M[(size_t)j * cap + i] = col[i];
+            }
+            dim += kept;
+            // Ritz pairs of the basis
+            std::vector<double> A((size_t)dim * dim), w, Z;
+            for (int j = 0; j < dim; ++j)
+                for (int i = 0; i <= j; ++i) {
+                    A[(size_t)j * dim + i] = M[(size_t)j * cap + i];
+                    A[(size_t)i * dim + j] = M[(size_t)j * cap + i];
+                }
+            sym_eig(dim, A, w, Z);
+            std::vector<int> order(dim);
+            for (int i = 0; i < dim; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return w[x] < w[y]; });
+            std::vector<const double *> vv, va;
+            for (int j = 0; j < dim; ++j) {
+                vv.push_back(V(j));
+                va.push_back(VA(j));
+            }
+            std::vector<const double *> xs, ys;
+            for (int r = 0; r < k; ++r) {
+                std::vector<double> y(Z.begin() + (size_t)order[r] * dim, Z.begin() + (size_t)(order[r] + 1) * dim);
+                int big = 0;
+                for (int i = 1; i < dim; ++i)
+                    if (std::fabs(y[i]) > std::fabs(y[big])) big = i;
+                if (y[big] < 0)
+                    for (auto &v : y) v = -v;
+                theta[r] = w[order[r]];
+                jd.lincomb(vv, y, 0.0, Ur(r));
+                jd.lincomb(va, y, 0.0, UA(r));
+                jd.lincomb({UA(r), Ur(r)}, {1.0, -theta[r]}, 0.0, Rr(r));
+                xs.insert(xs.end(), {Rr(r), Rr(r), Ur(r)});
+                ys.insert(ys.end(), {Rr(r), Ur(r), Ur(r)});
+            }
+            ++it;
+            std::vector<double> rr = jd.dots(xs, ys);
+            std::vector<int> open;
+            for (int r = 0; r < k; ++r) {
+                const double rn = std::sqrt(rr[3 * r]);
+                if (!(rn <= tol * std::fabs(theta[r]) || rn < tol)) open.push_back(r);
+            }
+            if (open.empty() || it >= max_iter) break;
+            if (dim + (int)open.size() > cap) {
+                // thick restart: the lowest min(dim, 2 nroots) Ritz vectors and their images
+                const int keep = std::min(dim, 2 * k);
+                for (int r = 0; r < keep; ++r) {
+                    std::vector<double> y(Z.begin() + (size_t)order[r] * dim,
+                                          Z.begin() + (size_t)(order[r] + 1) * dim);
+                    jd.lincomb(vv, y, 0.0, SV(r));
+                    jd.lincomb(va, y, 0.0, SA(r));
+                }
+                for (int r = 0; r < keep; ++r) {
+                    check(hipMemcpyAsync(V(r), SV(r), sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "restart");
+                    check(hipMemcpyAsync(VA(r), SA(r), sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "restart");
+                }
+                dim = keep;
+                for (int j = 0; j < keep; ++j) {
+                    std::vector<const double *> a, b;
+                    for (int i = 0; i <= j; ++i) {
+                        a.push_back(V(i));
+                        b.push_back(VA(j));
+                    }
+                    std::vector<double> col = jd.dots(a, b);
+                    for (int i = 0; i <= j; ++i) M[(size_t)j * cap + i] = col[i];
+                }
+            }
+            nb = (int)open.size();
+            if (diag) {
+                // diagonal preconditioner (MultDiagonal::precondition, ietl_davidson.h:96-104)
+                std::vector<const double *> px, py;
+                for (int c = 0; c < nb; ++c) {
+                    const int r = open[c];
+                    check(launch_diag_scale(Ur(r), Ur(r), 0.0, diag, theta[r], SV(c), n, jd.s), "precondition");
+                    px.insert(px.end(), {SV(c), Ur(r)});
+                    py.insert(py.end(), {Rr(r), SV(c)});
+                }
+                std::vector<double> ab = jd.dots(px, py);
+                for (int c = 0; c < nb; ++c) {
+                    const int r = open[c];
+                    check(launch_diag_scale(Rr(r), Ur(r), ab[2 * c] / ab[2 * c + 1], diag, theta[r], V(dim + c), n,
+                                            jd.s),
+                          "precondition");
+                }
+            } else {
+                for (int c = 0; c < nb; ++c) {
+                    const int r = open[c];
+                    jd.lincomb({Rr(r), Ur(r)}, {-1.0, rr[3 * r + 1] / rr[3 * r + 2]}, 0.0, V(dim + c));
+                }
+            }
+        }
+        if (prec != MQ_PREC_FP64) {
+            // FP64 defect correction per root: Rayleigh quotients from one batched FP64 sigma
+            sigma_batch_launch(plan, L, R, Ur(0), UA(0), k, MQ_PREC_FP64);
+            std::vector<const double *> xs, ys;
+            for (int r = 0; r < k; ++r) {
+                xs.insert(xs.end(), {Ur(r), Ur(r)});
+                ys.insert(ys.end(), {UA(r), Ur(r)});
+            }
+            std::vector<double> d = jd.dots(xs, ys);
+            for (int r = 0; r < k; ++r) theta[r] = d[2 * r] / d[2 * r + 1];
+        }
+        check(hipMemcpyAsync(U, Ur(0), sizeof(double) * n * k, hipMemcpyDeviceToDevice, jd.s), "U out");
+        jd.sync();
+        for (int r = 0; r < k; ++r) theta_out[r] = theta[r];
         if (iterations) *iterations = it;
         return MQ_OK;
     } catch (std::exception &e) {

@@ -53,6 +53,8 @@ def lib():
         L.mq_sigma_timed.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_float), f64p]
         L.mq_sigma_graph.argtypes = [vp, vp, vp, vp, vp, C.c_int]
         L.mq_sigma_batch.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int]
+        L.mq_block_davidson.argtypes = [vp, vp, vp, vp, vp, vp, f64p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                        C.POINTER(vp), C.c_int, C.POINTER(C.c_int)]
         L.mq_plan_rebind.argtypes = [vp]
         L.mq_gemm_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
                                   C.c_int, C.c_int]
@@ -421,6 +423,25 @@ class SigmaPlan:
                                          _ptr(u), C.byref(th), max_iter, tol, prec, arr, len(od), C.byref(its)),
                        "mq_davidson")
         return th.value, u, its.value
+
+    def block_davidson(self, X0, max_iter=50, tol=1e-8, ortho=(), prec=None, diag=None):
+        """Lowest nroots eigenpairs from the rows of X0 (nroots, n): (thetas, U (nroots, n), iterations).
+        diag: a device diagonal (SigmaPlan.diagonal()) to precondition the corrections."""
+        t = self.eng.torch
+        prec = self.eng.prec if prec is None else prec
+        X0 = X0.contiguous()
+        k = X0.shape[0]
+        U = t.empty((k, self.n), dtype=t.float64, device=self.eng.device)
+        th = (C.c_double * k)()
+        its = C.c_int()
+        od = [self.eng.to_dev(o) for o in ortho]
+        arr = (C.c_void_p * max(1, len(od)))(*[o.data_ptr() for o in od])
+        t.cuda.synchronize()
+        self.eng.check(lib().mq_block_davidson(self.p, _ptr(self.L.data), _ptr(self.R.data),
+                                               None if diag is None else _ptr(diag), _ptr(X0), _ptr(U), th,
+                                               k, max_iter, tol, prec, arr, len(od), C.byref(its)),
+                       "mq_block_davidson")
+        return list(th), U, its.value
 
     def jd(self, x0, max_iter=10, tol=1e-8, ortho=(), prec=None):
         t = self.eng.torch

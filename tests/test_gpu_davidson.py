@@ -131,3 +131,36 @@ def test_golden_energies_gpu_davidson(fixtures, gpu_engine, name):
     fx = fixtures[name]
     E = TwoSiteDMRG(ham_from_fixture(fx), gpu_engine, 100, eigensolver="davidson").run(2)
     assert abs(E - fx["energy"]) <= fx["rtol"] * abs(fx["energy"]), (E, fx["energy"])
+
+
+@pytest.mark.parametrize("nroots", [1, 3, 4])
+def test_block_davidson_lowest_roots(fixtures, oracle_engine, gpu_engine, nroots):
+    """The nroots lowest eigenvalues of the dense H_eff from random start vectors
+    (through restarts), in FP64 and in the mixed form with the FP64 Rayleigh
+    quotient per root; Ritz vectors orthonormal with small residuals."""
+    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    ham, d = _problem(fixtures, oracle_engine, "benzene")
+    psi, L, R, W = d.site_problem(2)
+    H = _dense(ham, oracle_engine, psi, L, R, W)
+    w = np.linalg.eigvalsh(H)
+    Lg = gpu_engine.upload_boundary(L.layout, L.data)
+    Rg = gpu_engine.upload_boundary(R.layout, R.data)
+    sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+    try:
+        X0 = gpu_engine.torch.as_tensor(np.random.default_rng(nroots).standard_normal((nroots, w.size)))
+        X0 = X0.to(gpu_engine.device)
+        print(f"block davidson nroots {nroots} n={w.size}", flush=True)
+        th, U, its = sp.block_davidson(X0, 200, 1e-10, prec=MQ_PREC_FP64)
+        th32, _, _ = sp.block_davidson(X0, 200, 1e-10, prec=MQ_PREC_FP32)
+        thd, _, itsd = sp.block_davidson(X0, 200, 1e-10, prec=MQ_PREC_FP64, diag=sp.diagonal())
+    finally:
+        sp.close()
+    U = U.cpu().numpy()
+    print(f"block iterations: {its} (JD corrections), {itsd} (diagonal preconditioner)", flush=True)
+    assert its < 200 and itsd < 200
+    assert np.abs(np.array(thd) - w[:nroots]).max() <= 1e-9, (thd, w[:nroots])
+    assert np.abs(np.array(th) - w[:nroots]).max() <= 1e-9, (th, w[:nroots])
+    assert np.abs(np.array(th32) - w[:nroots]).max() <= 1e-8, (th32, w[:nroots])
+    assert np.abs(U @ U.T - np.eye(nroots)).max() <= 1e-10
+    for r in range(nroots):
+        assert np.linalg.norm(H @ U[r] - th[r] * U[r]) <= 1.01e-10 * abs(th[r])
