@@ -298,7 +298,8 @@ typedef struct mq_host_plan mq_host_plan;
 enum mq_plan_part {
     MQ_PART_ARR_L = 0, MQ_PART_ARR_R = 1, MQ_PART_RESHAPE = 2, MQ_PART_G1_GROUPS = 3,
     MQ_PART_G1_TILES = 4, MQ_PART_STRIPS = 5, MQ_PART_SRCS = 6, MQ_PART_G2_GROUPS = 7,
-    MQ_PART_G2_TILES = 8, MQ_PART_SCATTER = 9, MQ_PART_COMB_STRIPS = 10, MQ_PART_COMB_SRCS = 11
+    MQ_PART_G2_TILES = 8, MQ_PART_SCATTER = 9, MQ_PART_COMB_STRIPS = 10, MQ_PART_COMB_SRCS = 11,
+    MQ_PART_G1_ROWMAP = 12, MQ_PART_G2_ROWMAP = 13
 };
 int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out);
 int mq_host_plan_part(const mq_host_plan *hp, int part, const void **ptr, int64_t *count, int64_t *elem_bytes);

@@ -53,11 +53,16 @@ struct DevGemm {
     DevArray<GemmGroup> groups;
     DevArray<GemmTile> tiles;
     DevArray<int> split;
+    DevArray<RowMap> rowmap;
     int ntiles = 0, nsplit = 0;
     void upload(const GemmList &g, hipStream_t s) {
-        for (auto &x : g.groups)
+        for (auto &x : g.groups) {
             if ((x.tm != 64 && x.tm != 128) || x.tn < 16 || x.tn > 128 || x.tn % 16 != 0)
                 throw std::runtime_error("GEMM group with an unsupported tile shape");
+            if (x.rowmap < 0 || (x.rowmap > 0 && (x.split != 1 || x.rowmap - 1 + x.M > (int64_t)g.rowmap.size())))
+                throw std::runtime_error("GEMM group with an invalid row map");
+        }
+        rowmap.upload(g.rowmap, s);
         for (auto &t : g.tiles)
             if (t.g >= (int)g.groups.size()) throw std::runtime_error("GEMM tile outside the group list");
         groups.upload(g.groups, s);
@@ -319,27 +324,27 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
         check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s), "reshape");
         mark(2);
         check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la.p,
-                          p->prp.p, p->t.p, p->ws.p, ST_SIGMA, s),
+                          p->prp.p, p->t.p, p->ws.p, p->g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
         mark(3);
         check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t.p, p->c.p, ST_SIGMA, s),
               "W apply");
         mark(4);
         check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c.p,
-                          p->rb.p, y, p->ws.p, ST_SIGMA_OUT, s),
+                          p->rb.p, y, p->ws.p, p->g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
     } else {
         check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp32.p, s), "reshape");
         mark(2);
         check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la32.p,
-                          p->prp32.p, p->t32.p, p->ws32.p, ST_SIGMA, s),
+                          p->prp32.p, p->t32.p, p->ws32.p, p->g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
         mark(3);
         check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t32.p, p->c32.p, ST_SIGMA, s),
               "W apply");
         mark(4);
         check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c32.p,
-                          p->rb32.p, y, p->ws32.p, ST_SIGMA_OUT, s),
+                          p->rb32.p, y, p->ws32.p, p->g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
     }
     mark(5);
@@ -418,20 +423,20 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
     if (prec == MQ_PREC_FP64) {
         check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la.p, B.prp.p,
-                          B.t.p, B.ws.p, ST_SIGMA, s),
+                          B.t.p, B.ws.p, B.g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
         check(launch_wapply(B.strips.p, (int)B.nstrips, B.srcs.p, B.t.p, B.c.p, ST_SIGMA, s), "W apply");
         check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c.p, p->rb.p, Y,
-                          B.ws.p, ST_SIGMA_OUT, s),
+                          B.ws.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
     } else {
         check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp32.p, s), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la32.p,
-                          B.prp32.p, B.t32.p, B.ws32.p, ST_SIGMA, s),
+                          B.prp32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
         check(launch_wapply(B.strips.p, (int)B.nstrips, B.srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s), "W apply");
         check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c32.p,
-                          p->rb32.p, Y, B.ws32.p, ST_SIGMA_OUT, s),
+                          p->rb32.p, Y, B.ws32.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
     }
 }
@@ -610,7 +615,7 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const 
         dg.upload(gl, ctx->stream);
         DevArray<double> ws;
         ws.alloc((size_t)gl.ws);
-        check(launch_gemm(dg.groups.p, dg.tiles.p, dg.ntiles, dg.split.p, dg.nsplit, A, B, C, ws.p, ST_MISC,
+        check(launch_gemm(dg.groups.p, dg.tiles.p, dg.ntiles, dg.split.p, dg.nsplit, A, B, C, ws.p, dg.rowmap.p, ST_MISC,
                           ctx->stream),
               "gemm");
         check(hipStreamSynchronize(ctx->stream), "gemm sync");
@@ -653,24 +658,24 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
         if (h.kind == PLAN_LSTEP) {
             arrange_left(plan, in, s);
             check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
-                              plan->g1.nsplit, plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, ST_STEP, s),
+                              plan->g1.nsplit, plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, plan->g1.rowmap.p, ST_STEP, s),
                   "gemm T");
             check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p,
                                 ST_STEP, s),
                   "W apply");
             check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
-                              plan->g2.nsplit, plan->c.p, A, plan->tmp.p, plan->ws.p, ST_STEP_OUT, s),
+                              plan->g2.nsplit, plan->c.p, A, plan->tmp.p, plan->ws.p, plan->g2.rowmap.p, ST_STEP_OUT, s),
                   "gemm out");
         } else {
             check(launch_copy2d(plan->arrR.p, (int)h.arrR.size(), in, plan->rb.p, s), "arrange R");
             check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
-                              plan->g1.nsplit, A, plan->rb.p, plan->t.p, plan->ws.p, ST_STEP, s),
+                              plan->g1.nsplit, A, plan->rb.p, plan->t.p, plan->ws.p, plan->g1.rowmap.p, ST_STEP, s),
                   "gemm t");
             check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p,
                                 ST_STEP, s),
                   "W apply");
             check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
-                              plan->g2.nsplit, plan->c.p, plan->prp.p, plan->tmp.p, plan->ws.p, ST_STEP_OUT, s),
+                              plan->g2.nsplit, plan->c.p, plan->prp.p, plan->tmp.p, plan->ws.p, plan->g2.rowmap.p, ST_STEP_OUT, s),
                   "gemm out");
         }
         check(launch_copy2d(plan->scatter.p, (int)h.scatter.size(), plan->tmp.p, out, s), "scatter");
@@ -695,7 +700,7 @@ int mq_diagonal(mq_plan *plan, const double *L, const double *R, double *diag) {
                             ST_MISC, s),
               "diagonal W apply");
         check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p, plan->g2.nsplit,
-                          plan->t.p, plan->rb.p, diag, plan->ws.p, ST_MISC, s),
+                          plan->t.p, plan->rb.p, diag, plan->ws.p, plan->g2.rowmap.p, ST_MISC, s),
               "diagonal gemm");
         return MQ_OK;
     } catch (std::exception &e) {
@@ -1419,6 +1424,8 @@ int mq_host_plan_part(const mq_host_plan *h, int part, const void **ptr, int64_t
         case MQ_PART_SCATTER: *ptr = p.scatter.data(); *count = p.scatter.size(); *elem = sizeof(Copy2D); break;
         case MQ_PART_COMB_STRIPS: *ptr = p.combL.strips.data(); *count = p.combL.strips.size(); *elem = sizeof(Strip); break;
         case MQ_PART_COMB_SRCS: *ptr = p.combL.srcs.data(); *count = p.combL.srcs.size(); *elem = sizeof(Src); break;
+        case MQ_PART_G1_ROWMAP: *ptr = p.g1.rowmap.data(); *count = p.g1.rowmap.size(); *elem = sizeof(RowMap); break;
+        case MQ_PART_G2_ROWMAP: *ptr = p.g2.rowmap.data(); *count = p.g2.rowmap.size(); *elem = sizeof(RowMap); break;
         default: return MQ_ERR_ARG;
     }
     return MQ_OK;

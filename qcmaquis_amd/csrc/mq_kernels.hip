@@ -193,7 +193,7 @@ __device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<T
 template <int TM, int TN, class T, class TO>
 __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t, const T *__restrict__ A,
                                           const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
-                                          double *lds) {
+                                          const RowMap *__restrict__ rowmap, double *lds) {
     typedef Mfma<T> MF;
     constexpr int TK = kstep<T>();
     // the four waves split the tile along M; each holds TM/4 rows x TN columns
@@ -238,18 +238,24 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
         buf ^= 1;
     }
     if (g.split == 1) {
-        TO *out = C + g.c;
-        const int64_t ld = g.ldc;
 #pragma unroll
         for (int i = 0; i < MI; ++i) {
             const int gm = m0 + wm + 16 * i + lr;
             if (gm >= g.M) continue;
+            // row gm of C: column-major with ldc, or placed by the group's row map
+            TO *out = C + g.c + gm;
+            int64_t ld = g.ldc;
+            if (g.rowmap > 0) {
+                const RowMap rm = rowmap[g.rowmap - 1 + gm];
+                out = C + g.c + rm.off;
+                ld = rm.ld;
+            }
 #pragma unroll
             for (int j = 0; j < NJ; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int gn = n0 + 16 * j + MF::dcol(r, lq);
-                    if (gn < g.N) out[gm + gn * ld] = (TO)acc[i][j][r];
+                    if (gn < g.N) out[gn * ld] = (TO)acc[i][j][r];
                 }
         }
     } else {
@@ -274,16 +280,16 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
 template <int TM, class T, class TO>
 __device__ __forceinline__ void gemm_tile_n(const GemmGroup &g, const GemmTile &t, const T *__restrict__ A,
                                             const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
-                                            double *lds) {
+                                            const RowMap *__restrict__ rowmap, double *lds) {
     switch (g.tn) {
-        case 16: gemm_tile<TM, 16, T, TO>(g, t, A, B, C, W, lds); break;
-        case 32: gemm_tile<TM, 32, T, TO>(g, t, A, B, C, W, lds); break;
-        case 48: gemm_tile<TM, 48, T, TO>(g, t, A, B, C, W, lds); break;
-        case 64: gemm_tile<TM, 64, T, TO>(g, t, A, B, C, W, lds); break;
-        case 80: gemm_tile<TM, 80, T, TO>(g, t, A, B, C, W, lds); break;
-        case 96: gemm_tile<TM, 96, T, TO>(g, t, A, B, C, W, lds); break;
-        case 112: gemm_tile<TM, 112, T, TO>(g, t, A, B, C, W, lds); break;
-        default: gemm_tile<TM, 128, T, TO>(g, t, A, B, C, W, lds); break;
+        case 16: gemm_tile<TM, 16, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        case 32: gemm_tile<TM, 32, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        case 48: gemm_tile<TM, 48, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        case 64: gemm_tile<TM, 64, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        case 80: gemm_tile<TM, 80, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        case 96: gemm_tile<TM, 96, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        case 112: gemm_tile<TM, 112, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        default: gemm_tile<TM, 128, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
     }
 }
 
@@ -291,15 +297,15 @@ __device__ __forceinline__ void gemm_tile_n(const GemmGroup &g, const GemmTile &
 template <int STAGE, class T, class TO>
 __global__ void __launch_bounds__(256) k_gemm(const GemmGroup *__restrict__ groups, const GemmTile *__restrict__ tiles,
                                               const T *__restrict__ A, const T *__restrict__ B, TO *__restrict__ C,
-                                              T *__restrict__ W) {
+                                              T *__restrict__ W, const RowMap *__restrict__ rowmap) {
     __shared__ double lds[(2 * kstep<T>() * (TMAX + 2) * 2 * sizeof(T) + sizeof(double) - 1) / sizeof(double)];
     const GemmTile t = tiles[blockIdx.x];
     if (t.g < 0) return;  // padding tile of the XCD-ordered list
     const GemmGroup g = groups[t.g];
     if (g.tm == 128)
-        gemm_tile_n<128, T, TO>(g, t, A, B, C, W, lds);
+        gemm_tile_n<128, T, TO>(g, t, A, B, C, W, rowmap, lds);
     else
-        gemm_tile_n<64, T, TO>(g, t, A, B, C, W, lds);
+        gemm_tile_n<64, T, TO>(g, t, A, B, C, W, rowmap, lds);
 }
 
 // C = sum over splits of the partial tiles, in split order (deterministic),
@@ -390,19 +396,21 @@ hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const floa
 
 template <int STAGE, class T, class TO>
 static void gemm_stage(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const T *A, const T *B, TO *C,
-                       T *W, hipStream_t s) {
-    if (ntiles > 0) hipLaunchKernelGGL((k_gemm<STAGE, T, TO>), dim3(ntiles), dim3(256), 0, s, groups, tiles, A, B, C, W);
+                       T *W, const RowMap *rowmap, hipStream_t s) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL((k_gemm<STAGE, T, TO>), dim3(ntiles), dim3(256), 0, s, groups, tiles, A, B, C, W, rowmap);
 }
 
 template <class T, class TO>
 static hipError_t gemm_launch(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
-                              int nsplit, const T *A, const T *B, TO *C, T *W, int stage, hipStream_t s) {
+                              int nsplit, const T *A, const T *B, TO *C, T *W, const RowMap *rowmap, int stage,
+                              hipStream_t s) {
     switch (stage) {
-        case ST_SIGMA: gemm_stage<ST_SIGMA, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
-        case ST_SIGMA_OUT: gemm_stage<ST_SIGMA_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
-        case ST_STEP: gemm_stage<ST_STEP, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
-        case ST_STEP_OUT: gemm_stage<ST_STEP_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
-        default: gemm_stage<ST_MISC, T, TO>(groups, tiles, ntiles, A, B, C, W, s); break;
+        case ST_SIGMA: gemm_stage<ST_SIGMA, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        case ST_SIGMA_OUT: gemm_stage<ST_SIGMA_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        case ST_STEP: gemm_stage<ST_STEP, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        case ST_STEP_OUT: gemm_stage<ST_STEP_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        default: gemm_stage<ST_MISC, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
     }
     if (nsplit > 0)
         hipLaunchKernelGGL((k_split_reduce<T, TO>), dim3(64, nsplit), dim3(256), 0, s, groups, split_gids, W, C);
@@ -410,16 +418,19 @@ static hipError_t gemm_launch(const GemmGroup *groups, const GemmTile *tiles, in
 }
 
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
-                       const double *A, const double *B, double *C, double *W, int stage, hipStream_t s) {
-    return gemm_launch<double, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, stage, s);
+                       const double *A, const double *B, double *C, double *W, const RowMap *rowmap, int stage,
+                       hipStream_t s) {
+    return gemm_launch<double, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage, s);
 }
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
-                       const float *A, const float *B, float *C, float *W, int stage, hipStream_t s) {
-    return gemm_launch<float, float>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, stage, s);
+                       const float *A, const float *B, float *C, float *W, const RowMap *rowmap, int stage,
+                       hipStream_t s) {
+    return gemm_launch<float, float>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage, s);
 }
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
-                       const float *A, const float *B, double *C, float *W, int stage, hipStream_t s) {
-    return gemm_launch<float, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, stage, s);
+                       const float *A, const float *B, double *C, float *W, const RowMap *rowmap, int stage,
+                       hipStream_t s) {
+    return gemm_launch<float, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage, s);
 }
 
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,

@@ -162,6 +162,8 @@ int column_tile(int N) {
 
 void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     (void)tn;
+    if (g.rowmap < 0 || (g.rowmap > 0 && g.rowmap - 1 + g.M > (int64_t)gl.rowmap.size()))
+        throw std::runtime_error("GEMM row map out of range");
     g.split = 1;
     g.kchunk = std::max(g.K, 1);
     g.tm = g.M > 64 ? 128 : 64;
@@ -170,7 +172,7 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     if (g.M == 0 || g.N == 0) return;
     int mt = ceil_div(g.M, g.tm), nt = ceil_div(g.N, g.tn);
     // split K when the output is small and K long: chunks of about kchunk_target
-    if (g.K > 2 * kchunk_target && mt * nt < 64) {
+    if (g.K > 2 * kchunk_target && mt * nt < 64 && g.rowmap == 0) {
         int s = std::min(ceil_div(g.K, kchunk_target), 64);
         int kc = ceil_div(ceil_div(g.K, s), 16) * 16;
         s = ceil_div(g.K, kc);
@@ -306,8 +308,8 @@ struct CTerm {
 struct FBlk {  // one (lb, lk) block of a fused group
     int lb, lk, ar, al;
     int64_t lf;  // LA-arena offset of its first combined row (column 0)
-    int64_t tf;  // T-arena offset of its first T row (column 0)
-    int ld;      // rows of the fused panel (leading dimension in both arenas)
+    int64_t tf;  // T-arena offset of its T block (stored contiguously, leading dimension |lb|)
+    int ld;      // rows of the fused LA panel (its leading dimension)
 };
 struct FGroup {
     int b2, pin, pout, row, col, rspin, cspin, k, a;
@@ -471,11 +473,15 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
         t.t_total += (int64_t)t.M[lk] * s.pbr_sz_l[lk];
         t.gemm_flops += 2.0 * t.M[lk] * s.pbr_sz_l[lk] * s.lsize(lk);
     }
+    // T_lk stores each (b1, lb) row block contiguously (|lb| x N, leading dimension
+    // |lb|), so a W-application source slice is one contiguous range
     for (int b1 = 0; b1 < nb1; ++b1) {
         if (!real[b1]) continue;
         bool in_t = !cp || cp->direct[b1];
         for (auto &a : blk[b1]) {
-            if (in_t) t.tb[b1].push_back({a.lb, a.lk, t.t_off[a.lk] + a.row, t.M[a.lk]});
+            if (in_t)
+                t.tb[b1].push_back(
+                    {a.lb, a.lk, t.t_off[a.lk] + (int64_t)a.row * s.pbr_sz_l[a.lk], s.lsize(a.lb)});
             if (arr) {
                 int lbs = s.lsize(a.lb), lks = s.lsize(a.lk);
                 arr->push_back(mk_copy(t.la_off[a.lk] + a.row, a.blk->off, lbs, lks, t.Mtot[a.lk],
@@ -483,7 +489,12 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
             }
         }
     }
-    if (g1)
+    if (g1) {
+        // row placements of the T_lk panels: rows [row, row + |lb|) of a direct block
+        std::vector<std::vector<std::pair<int, int>>> rows_of(s.nl);  // [lk] (row, |lb|)
+        for (int b1 = 0; b1 < nb1; ++b1)
+            if (real[b1] && (!cp || cp->direct[b1]))
+                for (auto &a : blk[b1]) rows_of[a.lk].push_back({a.row, s.lsize(a.lb)});
         for (int lk = 0; lk < s.nl; ++lk) {
             if (t.M[lk] == 0) continue;
             GemmGroup g{};
@@ -496,8 +507,17 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
             g.lda = t.Mtot[lk];
             g.ldb = g.K;
             g.ldc = g.M;
+            g.rowmap = (int64_t)g1->rowmap.size() + 1;
+            g1->rowmap.resize(g1->rowmap.size() + g.M, RowMap{-1, 0, 0});
+            RowMap *rm = g1->rowmap.data() + (g.rowmap - 1);
+            for (auto &rb : rows_of[lk])
+                for (int r = 0; r < rb.second; ++r)
+                    rm[rb.first + r] = {(int64_t)rb.first * g.N + r, rb.second, 0};
+            for (int r = 0; r < g.M; ++r)
+                if (rm[r].off < 0) throw std::runtime_error("T panel row without a block");
             add_gemm(*g1, g, tm, tn, kc);
         }
+    }
     if (!cp || cp->fg.empty()) return t;
 
     // ---- fused groups: blocks, panels, combination strips, T GEMMs
@@ -507,6 +527,7 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     struct Panel {
         int lk, pin, row, ar, rows = 0;
         int64_t lf = 0, tf = 0;
+        std::vector<std::pair<int, int>> blocks;  // (first row, rows) in row order
     };
     struct Cand {
         int lk, lb, ar, al;
@@ -554,6 +575,7 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
             }
             g.blk.push_back({c.lb, c.lk, c.ar, c.al, 0, 0, 0});
             fb_panel[gi].push_back({pid, panels[pid].rows});
+            panels[pid].blocks.push_back({panels[pid].rows, s.lsize(c.lb)});
             panels[pid].rows += s.lsize(c.lb);
         }
     }
@@ -573,7 +595,7 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
             const Panel &pn = panels[fb_panel[gi][k].first];
             int row = fb_panel[gi][k].second;
             fb.lf = pn.lf + row;
-            fb.tf = pn.tf + row;
+            fb.tf = pn.tf + (int64_t)row * s.rsize(pn.ar);
             fb.ld = pn.rows;
         }
         if (!comb) return;
@@ -656,6 +678,9 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
         g.lda = pn.rows;
         g.ldb = ksz;
         g.ldc = pn.rows;
+        g.rowmap = (int64_t)g1->rowmap.size() + 1;
+        for (auto &rb : pn.blocks)
+            for (int r = 0; r < rb.second; ++r) g1->rowmap.push_back({(int64_t)rb.first * rsz + r, rb.second, 0});
         add_gemm(*g1, g, tm, tn, kc);
     }
     return t;
@@ -721,7 +746,7 @@ void lbtm_column(const Site &s, const TLay &t, const ColProg *cp, const std::vec
                     s.su2 ? coupling(j, std::abs(j - jp), jp, g.a, g.k, ap, i, std::abs(i - ip), ip) : one;
                 double coef = s.su2 ? cpl[case_of(g.rspin, g.cspin)] : 1.0;
                 int row0 = s.pbl[(size_t)g.pout * s.nl + fb.lb] + g.col * s.lsize(fb.lb);
-                raw.push_back({fb.al, fb.ar, row0, fb.tf, fb.ld, coef});
+                raw.push_back({fb.al, fb.ar, row0, fb.tf, s.lsize(fb.lb), coef});
             }
         }
     std::sort(raw.begin(), raw.end(), [](const Raw &x, const Raw &y) {
@@ -884,6 +909,7 @@ GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step
             out.groups.push_back(g);
         }
     out.ws = gl.ws * nvec;
+    out.rowmap = gl.rowmap;  // placements are relative to each group's c
     // the single-vector list is XCD-interleaved (tile i on XCD i % 8): recover each
     // XCD's sequence, put the per-vector copies of a tile next to each other, order again
     std::vector<GemmTile> seq;

@@ -19,7 +19,10 @@ struct Copy2D {
 // C(M x N) = op(A)(M x K) * op(B)(K x N), column-major with leading dims.
 // transA: A stored K x M; transB: B stored N x K.  split > 1: K is cut into
 // `split` chunks whose partial tiles go to the workspace at `part`.  tm, tn:
-// tile height and width (64 or 128) of the group's tiles.
+// tile height (64 or 128) and width (a multiple of 16 up to 128).  rowmap > 0:
+// row r of C is stored at c + map[rowmap - 1 + r].off + col * map[rowmap - 1 + r].ld
+// (row blocks of different heights each stored contiguously; such groups are
+// never split); rowmap == 0: C is column-major with ldc.
 struct GemmGroup {
     int64_t a, b, c, part;
     int M, N, K;
@@ -28,6 +31,12 @@ struct GemmGroup {
     int split;
     int kchunk;
     int tm, tn;
+    int64_t rowmap;
+};
+struct RowMap {
+    int64_t off;  // relative to the group's c
+    int ld;
+    int pad;
 };
 // one output tile; g < 0 marks a padding tile that does nothing
 struct GemmTile {
@@ -58,6 +67,7 @@ struct Src {
 struct GemmList {
     std::vector<GemmGroup> groups;
     std::vector<GemmTile> tiles;
+    std::vector<RowMap> rowmap;  // row placements of the groups with rowmap >= 0
     int64_t ws = 0;  // workspace elements for split partials
     double flops() const {
         double f = 0;

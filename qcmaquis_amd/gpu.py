@@ -81,15 +81,17 @@ COPY2D = np.dtype([("dst", "<i8"), ("src", "<i8"), ("rows", "<i4"), ("cols", "<i
                    ("lds", "<i4"), ("trans", "<i4"), ("scale", "<f8")], align=True)
 GEMMGROUP = np.dtype([("a", "<i8"), ("b", "<i8"), ("c", "<i8"), ("part", "<i8"), ("M", "<i4"), ("N", "<i4"),
                       ("K", "<i4"), ("lda", "<i4"), ("ldb", "<i4"), ("ldc", "<i4"), ("transA", "<i4"),
-                      ("transB", "<i4"), ("split", "<i4"), ("kchunk", "<i4"), ("tm", "<i4"), ("tn", "<i4")],
-                     align=True)
+                      ("transB", "<i4"), ("split", "<i4"), ("kchunk", "<i4"), ("tm", "<i4"), ("tn", "<i4"),
+                      ("rowmap", "<i8")], align=True)
+ROWMAP = np.dtype([("off", "<i8"), ("ld", "<i4"), ("pad", "<i4")], align=True)
 GEMMTILE = np.dtype([("g", "<i4"), ("mt", "<i4"), ("nt", "<i4"), ("ks", "<i4")], align=True)
 STRIP = np.dtype([("dst", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("ldd", "<i4"), ("s0", "<i4"),
                   ("s1", "<i4")], align=True)
 SRC = np.dtype([("off", "<i8"), ("ld", "<i4"), ("coef", "<f8")], align=True)
-PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D, STRIP, SRC]
+PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D, STRIP, SRC,
+               ROWMAP, ROWMAP]
 PART_NAMES = ["arrL", "arrR", "reshape", "g1_groups", "g1_tiles", "strips", "srcs", "g2_groups", "g2_tiles",
-              "scatter", "comb_strips", "comb_srcs"]
+              "scatter", "comb_strips", "comb_srcs", "g1_rowmap", "g2_rowmap"]
 KIND = {"sigma": 0, "lstep": 1, "rstep": 2, "diag": 3}
 
 

@@ -38,7 +38,7 @@ def wapply(strips, srcs, T, Cbuf):
         _mat(Cbuf, int(st["dst"]), r, c, int(st["ldd"]))[...] = acc
 
 
-def gemm(groups, tiles, A, B, C):
+def gemm(groups, tiles, A, B, C, rowmap=None):
     # tile coverage: every (m, n, k-split) tile exactly once (g < 0: padding tiles)
     seen = {}
     for t in tiles:
@@ -56,7 +56,7 @@ def gemm(groups, tiles, A, B, C):
         for ks in range(int(g["split"])):
             for n in range(nt):
                 for m in range(mt):
-                    assert (gi, m, n, ks) in seen
+                    assert (gi, m, n, ks) in seen, ("tile not in the list", gi, m, n, ks, M, N, K, tm, tn)
         assert int(g["split"]) * int(g["kchunk"]) >= K
         if g["transA"]:
             a = _mat(A, int(g["a"]), K, M, int(g["lda"])).T
@@ -66,7 +66,21 @@ def gemm(groups, tiles, A, B, C):
             b = _mat(B, int(g["b"]), N, K, int(g["ldb"])).T
         else:
             b = _mat(B, int(g["b"]), K, N, int(g["ldb"]))
-        _mat(C, int(g["c"]), M, N, int(g["ldc"]))[...] = a @ b if K else 0.0
+        prod = a @ b if K else np.zeros((M, N))
+        if int(g["rowmap"]) > 0:
+            # row r at c + map[rowmap - 1 + r].off + col * map[...].ld; the rows must not overlap
+            assert int(g["split"]) == 1
+            rm = rowmap[int(g["rowmap"]) - 1:int(g["rowmap"]) - 1 + M]
+            assert len(rm) == M
+            used = set()
+            for r in range(M):
+                addr = int(g["c"]) + int(rm["off"][r]) + np.arange(N) * int(rm["ld"][r])
+                assert addr.min() >= 0 and addr.max() < C.size
+                assert not used.intersection(addr.tolist())
+                used.update(addr.tolist())
+                C[addr] = prod[r]
+        else:
+            _mat(C, int(g["c"]), M, N, int(g["ldc"]))[...] = prod
 
 
 def run_sigma(hp, Ldata, Rdata, x):
@@ -79,7 +93,7 @@ def run_sigma(hp, Ldata, Rdata, x):
     wapply(P["comb_strips"], P["comb_srcs"], la, la)
     copy2d(P["arrR"], Rdata, rb)
     copy2d(P["reshape"], x, prp)
-    gemm(P["g1_groups"], P["g1_tiles"], la, prp, T)
+    gemm(P["g1_groups"], P["g1_tiles"], la, prp, T, P["g1_rowmap"])
     wapply(P["strips"], P["srcs"], T, Cb)
     gemm(P["g2_groups"], P["g2_tiles"], Cb, rb, y)
     return y
@@ -95,12 +109,12 @@ def run_step(hp, kind, Bdata, A):
     if kind == "lstep":
         copy2d(P["arrL"], Bdata, la)
         wapply(P["comb_strips"], P["comb_srcs"], la, la)
-        gemm(P["g1_groups"], P["g1_tiles"], la, prp, T)
+        gemm(P["g1_groups"], P["g1_tiles"], la, prp, T, P["g1_rowmap"])
         wapply(P["strips"], P["srcs"], T, Cb)
         gemm(P["g2_groups"], P["g2_tiles"], Cb, A, tmp)
     else:
         copy2d(P["arrR"], Bdata, rb)
-        gemm(P["g1_groups"], P["g1_tiles"], A, rb, T)
+        gemm(P["g1_groups"], P["g1_tiles"], A, rb, T, P["g1_rowmap"])
         wapply(P["strips"], P["srcs"], T, Cb)
         gemm(P["g2_groups"], P["g2_tiles"], Cb, prp, tmp)
     copy2d(P["scatter"], tmp, out)
