@@ -121,7 +121,8 @@ def main():
         if rec["L"] != a.L or rec.get("symm", "su2u1") != a.symm or rec.get("decay") != a.decay:
             raise SystemExit(f"{a.sectors} does not describe L={a.L} {a.symm} decay={a.decay}")
         sec_src = os.path.relpath(a.sectors, ROOT)
-    sp_ = centre_problem(eng, a.L, m=a.m, sectors=sectors, symm=a.symm, decay=a.decay)
+    sp_ = centre_problem(eng, a.L, m=a.m, sectors=sectors, symm=a.symm, decay=a.decay,
+                         log=(lambda msg: log(f"  {msg} ({time.perf_counter() - t0:.0f}s)")))
     ham, psi, Lb, Rb, W, site = sp_.ham, sp_.psi, sp_.L, sp_.R, sp_.W, sp_.site
     log(f"setup {time.perf_counter() - t0:.1f}s: site {site}, psi {psi.to_flat().size}, W {W.row_dim}x{W.col_dim}")
     # schedule per form (fused-column rule, split-K chunk): tuned on this problem

@@ -420,18 +420,26 @@ def op_kron_abelian(phys_a, phys_b, A: SiteOp, B: SiteOp):
     return SiteOp(blocks)
 
 
-def make_twosite_mpo(m1: MPOTensor, m2: MPOTensor, phys1, phys2, su2=True):
-    """Two-site MPO tensor for sites (p, p+1) (ts_ops.h:77-126)."""
+def make_twosite_mpo(m1: MPOTensor, m2: MPOTensor, phys1, phys2, su2=True, progress=None):
+    """Two-site MPO tensor for sites (p, p+1) (ts_ops.h:77-126).  progress(msg),
+    when given, is called after every tenth of the rows."""
     assert m1.col_dim == m2.row_dim
     table = OpTable()
     prempo = []
     cache = {}
+    step = max(1, m1.row_dim // 10)
     for b1 in range(m1.row_dim):
+        if progress and b1 and b1 % step == 0:
+            progress(f"two-site MPO: row {b1}/{m1.row_dim}")
         row1 = m1.rows[b1]
-        for b3 in range(m2.col_dim):
-            summands = [b2 for b2 in row1 if m2.has(b2, b3)]
-            if not summands:
-                continue
+        # the b3 reached from b1 through some b2, in ascending order; for each, the
+        # b2 in the order of row b1
+        reach = {}
+        for b2 in row1:
+            for b3 in m2.rows[b2]:
+                reach.setdefault(b3, []).append(b2)
+        for b3 in sorted(reach):
+            summands = reach[b3]
             ret = {}
             for b2 in summands:
                 p1 = m1.at(b1, b2)

@@ -27,10 +27,12 @@ from .twou1 import twou1_terms
 class Hamiltonian:
     """Quantum-chemistry Hamiltonian as an MPO (harness input)."""
 
-    def __init__(self, L, nelec, spin, integrals, symm="su2u1", herm=True):
+    def __init__(self, L, nelec, spin, integrals, symm="su2u1", herm=True, progress=None):
         """spin is 2S.  symm "su2u1" (models/chem/su2u1) or "2u1" (models/chem/2u1,
         spin-orbital basis with N_alpha - N_beta = 2S).  herm=False builds the SU2
-        MPO without hermitian-partner bonds (every boundary slice is stored)."""
+        MPO without hermitian-partner bonds (every boundary slice is stored).
+        progress(msg), when given, reports the stages of a long construction."""
+        self._progress = progress
         self.L, self.nelec, self.spin = L, nelec, spin
         table = OpTable()
         if symm == "su2u1":
@@ -47,21 +49,28 @@ class Hamiltonian:
             self.symm = MQ_SYMM_TWOU1
             self.su2 = False
             ops = TwoU1Ops(table)
-            self.terms = twou1_terms(integrals, L, ops)
+            self.terms = twou1_terms(integrals, L, ops, progress=progress)
             mk = TaggedMPOMaker(L, table, ops.ident, ops.fill, su2=False, herm=False)
             self.phys = twou1_phys_index()
             self.right_end = ((nelec + spin) // 2, (nelec - spin) // 2)
         else:
             raise ValueError(f"unknown symmetry {symm!r}")
-        for t in self.terms:
+        step = max(1, len(self.terms) // 8)
+        for i, t in enumerate(self.terms):
+            if progress and i and i % step == 0:
+                progress(f"MPO: {i}/{len(self.terms)} terms placed")
             mk.add_term(t)
         self.mpo, self.ecore = mk.create_mpo()
+        if progress:
+            progress(f"MPO built: {len(self.terms)} terms, bond dimensions up to "
+                     f"{max(w.col_dim for w in self.mpo)}")
         self._ts = {}
 
     def ts_mpo(self, p):
         w = self._ts.get(p)
         if w is None:
-            w = make_twosite_mpo(self.mpo[p], self.mpo[p + 1], self.phys, self.phys, su2=self.su2)
+            w = make_twosite_mpo(self.mpo[p], self.mpo[p + 1], self.phys, self.phys, su2=self.su2,
+                                 progress=getattr(self, "_progress", None))
             self._ts[p] = w
         return w
 

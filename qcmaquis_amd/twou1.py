@@ -105,8 +105,10 @@ class TwoU1Terms:
         return Term(c, True, ops)
 
 
-def twou1_terms(integrals, L, ops: TwoU1Ops, cutoff=1e-14):
-    """Terms of the spin-orbital Hamiltonian from 1-based FCIDUMP tuples."""
+def twou1_terms(integrals, L, ops: TwoU1Ops, cutoff=1e-14, progress=None):
+    """Terms of the spin-orbital Hamiltonian from 1-based FCIDUMP tuples.
+    progress(msg), when given, is called after each first orbital index of the
+    two-body terms."""
     h, g, ecore = dense_tensors(L, integrals)
     tb = TwoU1Terms(ops)
     acc = {}
@@ -127,6 +129,8 @@ def twou1_terms(integrals, L, ops: TwoU1Ops, cutoff=1e-14):
             add(v, [(p, CD), (q, AD)])
     spins = ((CU, AU), (CD, AD))
     for p in range(L):
+        if progress and p:
+            progress(f"two-body terms: orbital {p}/{L}, {len(acc)} strings")
         for q in range(L):
             for r in range(L):
                 for s in range(L):

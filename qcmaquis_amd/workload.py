@@ -32,12 +32,14 @@ class SiteProblem:
         self.ham, self.psi, self.L, self.R, self.W, self.site, self.mps = ham, psi, L, R, W, site, mps
 
 
-def centre_problem(engine, L, m=None, sectors=None, site=None, seed=42, ham=None, symm="su2u1", decay=None):
+def centre_problem(engine, L, m=None, sectors=None, site=None, seed=42, ham=None, symm="su2u1", decay=None,
+                   log=None):
     """Build the SiteProblem of the two-site problem at `site` (default
     L/2 - 1).  `sectors` is a list of Index (one per bond, L + 1 entries);
-    when None the degeneracy-proportional structure capped at m is used."""
+    when None the degeneracy-proportional structure capped at m is used.
+    log(msg), when given, reports progress."""
     if ham is None:
-        ham = Hamiltonian(L, L, 0, synthetic(L, seed=1000 + L, decay=decay), symm=symm)
+        ham = Hamiltonian(L, L, 0, synthetic(L, seed=1000 + L, decay=decay), symm=symm, progress=log)
     if sectors is None:
         sectors = bounded_sectors(L, ham.phys, ham.right_end, m, su2=ham.su2)
     if len(sectors) != L + 1:
@@ -48,8 +50,12 @@ def centre_problem(engine, L, m=None, sectors=None, site=None, seed=42, ham=None
     R = engine.seed_boundary(ham.right_end)
     for i in range(L - 1, c + 1, -1):
         R = engine.right_step(ham.symm, mps[i], R, ham.mpo[i])
+        if log and i % 4 == 0:
+            log(f"right boundary at site {i}")
     Lb = engine.seed_boundary((0, 0))
     for i in range(0, c):
+        if log and i % 4 == 0:
+            log(f"left boundary at site {i}")
         Rf = left_normalize_and_return(mps[i])
         multiply_from_left(mps[i + 1], Rf)
         Lb = engine.left_step(ham.symm, mps[i], Lb, ham.mpo[i])
