@@ -186,9 +186,12 @@ def main():
     # solves of 20 iterations), then both forms run one more solve from it
     conv, dE = None, None
     if not a.no_delta_e:
+        # restarted FP64 solves of 40 iterations until one converges (at most 12)
         u = x
-        for _ in range(4):
-            _, u, _ = sp.jd(u, 20, 1e-12, prec=MQ_PREC_FP64)
+        for _ in range(12):
+            _, u, its = sp.jd(u, 40, 1e-11, prec=MQ_PREC_FP64)
+            if its < 40:
+                break
         conv = {}
         for name, prec in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)):
             th, _, its = plans[name].jd(u, 10, 1e-10, prec=prec)

@@ -224,6 +224,11 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
  * initial guess; u (device, written) the Ritz vector; *theta the eigenvalue.
  * n_ortho vectors (device, each vec_len) are projected out as
  * SingleSiteVS::project does (ietl_lanczos_solver.h:90-95). */
+/* With prec != MQ_PREC_FP64 the iterations run in that form and the returned
+ * eigenvalue is the FP64 Rayleigh quotient of the Ritz vector; when the FP64
+ * residual of that pair misses tol (tol > 0), the solve continues in FP64
+ * from the Ritz vector (up to max_iter more iterations, counted in
+ * *iterations).  The same holds for mq_davidson and mq_block_davidson. */
 int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R,
                        const double *x0, double *u, double *theta,
                        int max_iter, double tol, int prec,

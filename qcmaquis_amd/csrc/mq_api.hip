@@ -90,6 +90,7 @@ struct mq_plan {
     DevArray<Src> srcs, csrcs;
     DevGemm g1, g2;
     DevArray<double> la, rb, prp, t, c, tmp, ws;
+    double w_src_el = 0, w_dst_el = 0;  // W-application source / destination elements
     const double *bound_L = nullptr, *bound_R = nullptr;
     // mixed form: FP32 copies of the panels and FP32 intermediates (allocated on first use)
     DevArray<float> la32, rb32, prp32, t32, c32, ws32;
@@ -230,6 +231,10 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         p->c.alloc(h.n_c);
         p->tmp.alloc(h.n_tmp);
         p->ws.alloc(std::max(h.g1.ws, h.g2.ws));
+        for (auto &st : h.wapply.strips) {
+            p->w_src_el += (double)st.rows * st.cols * (st.s1 - st.s0);
+            p->w_dst_el += (double)st.rows * st.cols;
+        }
         // diagonal plans: T columns that no strip writes stay zero for the plan's life
         if (kind == PLAN_DIAG && h.n_t) check(hipMemsetAsync(p->t.p, 0, sizeof(double) * h.n_t, s), "clear T");
         check(hipStreamSynchronize(s), "plan upload");
@@ -572,11 +577,7 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
         for (auto &e : ev) (void)hipEventDestroy(e);
         if (work) {
             const double es = prec == MQ_PREC_FP64 ? 8.0 : 4.0;  // bytes per stored element
-            double srcel = 0, dstel = 0;
-            for (auto &st : h.wapply.strips) {
-                srcel += (double)st.rows * st.cols * (st.s1 - st.s0);
-                dstel += (double)st.rows * st.cols;
-            }
+            const double srcel = plan->w_src_el, dstel = plan->w_dst_el;
             work[0] = h.g1.flops();
             work[1] = 2.0 * srcel;
             work[2] = h.g2.flops();
@@ -952,15 +953,31 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             if (rn <= tol * std::fabs(theta) || rn < tol || it >= max_iter) break;
             jd.lincomb({uA, u}, {-1.0, rr[1] / rr[2]}, 0.0, V(it));
         }
+        bool refine = false;
         if (prec != MQ_PREC_FP64) {
             // FP64 defect correction: the eigenvalue is the FP64 Rayleigh quotient of the
-            // final Ritz vector (its error is second order in the vector's error)
+            // final Ritz vector (its error is second order in the vector's error) ...
             sigma_launch(plan, L, R, u, uA, MQ_PREC_FP64);
             std::vector<double> d = jd.dots({u, u}, {uA, u});
             theta = d[0] / d[1];
+            // ... and when its FP64 residual misses the tolerance (the mixed iterations
+            // stop at their rounding floor, e.g. next to a nearly degenerate state), the
+            // solve continues in FP64 from that vector
+            if (tol > 0) {
+                jd.lincomb({u}, {-theta}, 1.0, uA);
+                const double rn = std::sqrt(jd.dot(uA, uA));
+                refine = !(rn <= tol * std::fabs(theta) || rn < tol);
+            }
         }
         check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
         jd.sync();
+        if (refine) {
+            int its2 = 0;
+            const int rc = mq_jacobi_davidson(plan, L, R, u_out, u_out, &theta, max_iter, tol, MQ_PREC_FP64, ortho,
+                                              n_ortho, &its2);
+            if (rc != MQ_OK) return rc;
+            it += its2;
+        }
         *theta_out = theta;
         if (iterations) *iterations = it;
         return MQ_OK;
@@ -1133,14 +1150,28 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
             // t = (r - (Vcpy.r)/(u.Vcpy) u) / (theta - D), normalised by the next pass
             check(launch_diag_scale(r, u, rr[1] / rr[2], diag, theta, V(dim), n, jd.s), "precondition");
         }
+        bool refine = false;
         if (prec != MQ_PREC_FP64) {
-            // FP64 defect correction, as in the Jacobi-Davidson driver
+            // FP64 defect correction and, when the FP64 residual misses the tolerance,
+            // FP64 iterations from the Ritz vector, as in the Jacobi-Davidson driver
             sigma_launch(plan, L, R, u, r, MQ_PREC_FP64);
             std::vector<double> d = jd.dots({u, u}, {r, u});
             theta = d[0] / d[1];
+            if (tol > 0) {
+                jd.lincomb({u}, {-theta}, 1.0, r);
+                const double rn = std::sqrt(jd.dot(r, r));
+                refine = !(rn <= tol * std::fabs(theta) || rn < tol);
+            }
         }
         check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
         jd.sync();
+        if (refine) {
+            int its2 = 0;
+            const int rc = mq_davidson(plan, L, R, diag, u_out, u_out, &theta, max_iter, tol, MQ_PREC_FP64, ortho,
+                                       n_ortho, &its2);
+            if (rc != MQ_OK) return rc;
+            it += its2;
+        }
         *theta_out = theta;
         if (iterations) *iterations = it;
         return MQ_OK;
@@ -1353,8 +1384,11 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
                 }
             }
         }
+        bool refine = false;
         if (prec != MQ_PREC_FP64) {
-            // FP64 defect correction per root: Rayleigh quotients from one batched FP64 sigma
+            // FP64 defect correction per root: Rayleigh quotients from one batched FP64
+            // sigma; when a root's FP64 residual misses the tolerance, the block continues
+            // in FP64 from the Ritz vectors
             sigma_batch_launch(plan, L, R, Ur(0), UA(0), k, MQ_PREC_FP64);
             std::vector<const double *> xs, ys;
             for (int r = 0; r < k; ++r) {
@@ -1363,9 +1397,28 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
             }
             std::vector<double> d = jd.dots(xs, ys);
             for (int r = 0; r < k; ++r) theta[r] = d[2 * r] / d[2 * r + 1];
+            if (tol > 0) {
+                xs.clear();
+                for (int r = 0; r < k; ++r) {
+                    jd.lincomb({UA(r), Ur(r)}, {1.0, -theta[r]}, 0.0, Rr(r));
+                    xs.push_back(Rr(r));
+                }
+                std::vector<double> rr = jd.dots(xs, xs);
+                for (int r = 0; r < k; ++r) {
+                    const double rn = std::sqrt(rr[r]);
+                    if (!(rn <= tol * std::fabs(theta[r]) || rn < tol)) refine = true;
+                }
+            }
         }
         check(hipMemcpyAsync(U, Ur(0), sizeof(double) * n * k, hipMemcpyDeviceToDevice, jd.s), "U out");
         jd.sync();
+        if (refine) {
+            int its2 = 0;
+            const int rc = mq_block_davidson(plan, L, R, diag, U, U, theta.data(), k, max_iter, tol, MQ_PREC_FP64,
+                                             ortho, n_ortho, &its2);
+            if (rc != MQ_OK) return rc;
+            it += its2;
+        }
         for (int r = 0; r < k; ++r) theta_out[r] = theta[r];
         if (iterations) *iterations = it;
         return MQ_OK;

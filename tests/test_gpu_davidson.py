@@ -164,3 +164,30 @@ def test_block_davidson_lowest_roots(fixtures, oracle_engine, gpu_engine, nroots
     assert np.abs(U @ U.T - np.eye(nroots)).max() <= 1e-10
     for r in range(nroots):
         assert np.linalg.norm(H @ U[r] - th[r] * U[r]) <= 1.01e-10 * abs(th[r])
+
+
+def test_mixed_solvers_refine_in_fp64(fixtures, oracle_engine, gpu_engine):
+    """A tolerance below the mixed form's rounding floor: after the mixed
+    iterations the FP64 residual misses it, the solve continues in FP64, and
+    the eigenvalue equals the FP64 solve's (JD, Davidson, block Davidson)."""
+    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    ham, d = _problem(fixtures, oracle_engine, "benzene")
+    psi, L, R, W = d.site_problem(2)
+    w = np.linalg.eigvalsh(_dense(ham, oracle_engine, psi, L, R, W))
+    Lg = gpu_engine.upload_boundary(L.layout, L.data)
+    Rg = gpu_engine.upload_boundary(R.layout, R.data)
+    sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+    try:
+        x = gpu_engine.to_dev(psi.to_flat())
+        t64, _, i64 = sp.jd(x, 60, 1e-13, prec=MQ_PREC_FP64)
+        t32, _, i32 = sp.jd(x, 60, 1e-13, prec=MQ_PREC_FP32)
+        d64, _, _ = sp.davidson(x, 200, 1e-13, prec=MQ_PREC_FP64)
+        d32, _, _ = sp.davidson(x, 200, 1e-13, prec=MQ_PREC_FP32)
+        X0 = gpu_engine.torch.as_tensor(np.random.default_rng(2).standard_normal((2, w.size))).to(gpu_engine.device)
+        b32, _, _ = sp.block_davidson(X0, 200, 1e-13, prec=MQ_PREC_FP32)
+    finally:
+        sp.close()
+    print(f"JD iterations fp64 {i64}, mixed + fp64 refinement {i32}", flush=True)
+    assert abs(t32 - w[0]) <= 1e-11 and abs(t64 - w[0]) <= 1e-11
+    assert abs(d32 - w[0]) <= 1e-11 and abs(d64 - w[0]) <= 1e-11
+    assert np.abs(np.array(b32) - w[:2]).max() <= 1e-11
