@@ -183,7 +183,7 @@ def main():
     sig_err = float(np.abs(r32["y"] - r64["y"]).max() / np.abs(r64["y"]).max())
     # energy agreement at convergence, the regime of the solves that fix a sweep's
     # final energy: the lowest eigenvector is first converged in FP64 (restarted
-    # solves of 20 iterations), then both forms run one more solve from it
+    # solves of 40 iterations), then both forms run one more solve from it
     conv, dE = None, None
     if not a.no_delta_e:
         # restarted FP64 solves of 40 iterations until one converges (at most 12)

@@ -80,6 +80,17 @@ void check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// packs must cover the strips in order, each with 1..kPackStrips strips
+void check_packs(const StripList &sl) {
+    const auto &p = sl.packs;
+    if (sl.strips.empty() && p.size() <= 1) return;
+    if (p.size() < 2 || p.front() != 0 || p.back() != (int)sl.strips.size())
+        throw std::runtime_error("strip packs do not cover the strip list");
+    for (size_t i = 0; i + 1 < p.size(); ++i)
+        if (p[i + 1] - p[i] < 1 || p[i + 1] - p[i] > kPackStrips) throw std::runtime_error("strip pack out of range");
+}
+int npacks(const StripList &sl) { return sl.packs.empty() ? 0 : (int)sl.packs.size() - 1; }
+
 }  // namespace
 
 struct mq_plan {
@@ -88,6 +99,7 @@ struct mq_plan {
     DevArray<Copy2D> arrL, arrR, reshape, scatter;
     DevArray<Strip> strips, cstrips;
     DevArray<Src> srcs, csrcs;
+    DevArray<int> packs, cpacks;  // strip packs of wapply / combL
     DevGemm g1, g2;
     DevArray<double> la, rb, prp, t, c, tmp, ws;
     double w_src_el = 0, w_dst_el = 0;  // W-application source / destination elements
@@ -105,9 +117,10 @@ struct mq_plan {
     // several vectors per launch (mq_sigma_batch): replicated task lists per batch
     // size, and per-vector copies of the psi, T and C arenas for the largest size used
     struct BatchLists {
-        int64_t nstrips = 0, ncopies = 0, ws = 0;
+        int64_t nstrips = 0, npacks = 0, ncopies = 0, ws = 0;
         DevArray<Copy2D> reshape;
         DevArray<Strip> strips;
+        DevArray<int> packs;
         DevArray<Src> srcs;
         DevGemm g1, g2;
     };
@@ -222,6 +235,10 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         p->srcs.upload(h.wapply.srcs, s);
         p->cstrips.upload(h.combL.strips, s);
         p->csrcs.upload(h.combL.srcs, s);
+        check_packs(h.wapply);
+        check_packs(h.combL);
+        p->packs.upload(h.wapply.packs, s);
+        p->cpacks.upload(h.combL.packs, s);
         p->g1.upload(h.g1, s);
         p->g2.upload(h.g2, s);
         p->la.alloc(h.n_la);
@@ -287,7 +304,7 @@ double mq_plan_exec_flops(const mq_plan *plan) { return plan ? plan->hp.exec_flo
 static void arrange_left(mq_plan *p, const double *L, hipStream_t s) {
     HostPlan &h = p->hp;
     check(launch_copy2d(p->arrL.p, (int)h.arrL.size(), L, p->la.p, s), "arrange L");
-    check(launch_wapply(p->cstrips.p, (int)h.combL.strips.size(), p->csrcs.p, p->la.p, p->la.p, ST_FUSE, s), "fuse L rows");
+    check(launch_wapply(p->cstrips.p, p->cpacks.p, npacks(h.combL), p->csrcs.p, p->la.p, p->la.p, ST_FUSE, s), "fuse L rows");
 }
 
 // (Re)bind the boundaries: arrange them into the FP64 panels when the pointer
@@ -332,7 +349,7 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
                           p->prp.p, p->t.p, p->ws.p, p->g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
         mark(3);
-        check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t.p, p->c.p, ST_SIGMA, s),
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, p->t.p, p->c.p, ST_SIGMA, s),
               "W apply");
         mark(4);
         check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c.p,
@@ -345,7 +362,7 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
                           p->prp32.p, p->t32.p, p->ws32.p, p->g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
         mark(3);
-        check(launch_wapply(p->strips.p, (int)h.wapply.strips.size(), p->srcs.p, p->t32.p, p->c32.p, ST_SIGMA, s),
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, p->t32.p, p->c32.p, ST_SIGMA, s),
               "W apply");
         mark(4);
         check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c32.p,
@@ -373,9 +390,12 @@ static mq_plan::BatchLists &batch_setup(mq_plan *p, int nvec, bool f32) {
         B->reshape.upload(batch_copies(h.reshape, nvec, h.vec_len, h.n_prp), s);
         B->ncopies = (int64_t)h.reshape.size() * nvec;
         StripList sl = batch_strips(h.wapply, nvec, h.n_t, h.n_c);
+        check_packs(sl);
         B->strips.upload(sl.strips, s);
         B->srcs.upload(sl.srcs, s);
+        B->packs.upload(sl.packs, s);
         B->nstrips = (int64_t)sl.strips.size();
+        B->npacks = npacks(sl);
         GemmList g1 = batch_gemm(h.g1, nvec, 0, h.n_prp, h.n_t);
         GemmList g2 = batch_gemm(h.g2, nvec, h.n_c, 0, h.vec_len);
         B->g1.upload(g1, s);
@@ -419,18 +439,19 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
         DevArray<Copy2D> &reshape;
         DevArray<Strip> &strips;
         DevArray<Src> &srcs;
+        DevArray<int> &packs;
         DevGemm &g1, &g2;
-        int64_t ncopies, nstrips;
+        int64_t ncopies, nstrips, npacks;
         DevArray<double> &prp, &t, &c, &ws;
         DevArray<float> &prp32, &t32, &c32, &ws32;
-    } B{Bl.reshape, Bl.strips, Bl.srcs, Bl.g1, Bl.g2, Bl.ncopies, Bl.nstrips,
+    } B{Bl.reshape, Bl.strips, Bl.srcs, Bl.packs, Bl.g1, Bl.g2, Bl.ncopies, Bl.nstrips, Bl.npacks,
         A.prp, A.t, A.c, A.ws64, A.prp32, A.t32, A.c32, A.ws32};
     if (prec == MQ_PREC_FP64) {
         check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la.p, B.prp.p,
                           B.t.p, B.ws.p, B.g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
-        check(launch_wapply(B.strips.p, (int)B.nstrips, B.srcs.p, B.t.p, B.c.p, ST_SIGMA, s), "W apply");
+        check(launch_wapply(B.strips.p, B.packs.p, (int)B.npacks, B.srcs.p, B.t.p, B.c.p, ST_SIGMA, s), "W apply");
         check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c.p, p->rb.p, Y,
                           B.ws.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
@@ -439,7 +460,7 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la32.p,
                           B.prp32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
-        check(launch_wapply(B.strips.p, (int)B.nstrips, B.srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s), "W apply");
+        check(launch_wapply(B.strips.p, B.packs.p, (int)B.npacks, B.srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s), "W apply");
         check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c32.p,
                           p->rb32.p, Y, B.ws32.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
@@ -661,7 +682,7 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
             check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
                               plan->g1.nsplit, plan->la.p, plan->prp.p, plan->t.p, plan->ws.p, plan->g1.rowmap.p, ST_STEP, s),
                   "gemm T");
-            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p,
+            check(launch_wapply(plan->strips.p, plan->packs.p, npacks(h.wapply), plan->srcs.p, plan->t.p, plan->c.p,
                                 ST_STEP, s),
                   "W apply");
             check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
@@ -672,7 +693,7 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
             check(launch_gemm(plan->g1.groups.p, plan->g1.tiles.p, plan->g1.ntiles, plan->g1.split.p,
                               plan->g1.nsplit, A, plan->rb.p, plan->t.p, plan->ws.p, plan->g1.rowmap.p, ST_STEP, s),
                   "gemm t");
-            check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->t.p, plan->c.p,
+            check(launch_wapply(plan->strips.p, plan->packs.p, npacks(h.wapply), plan->srcs.p, plan->t.p, plan->c.p,
                                 ST_STEP, s),
                   "W apply");
             check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p,
@@ -697,7 +718,7 @@ int mq_diagonal(mq_plan *plan, const double *L, const double *R, double *diag) {
     try {
         check(launch_copy2d(plan->arrL.p, (int)h.arrL.size(), L, plan->la.p, s), "L diagonals");
         check(launch_copy2d(plan->arrR.p, (int)h.arrR.size(), R, plan->rb.p, s), "R diagonals");
-        check(launch_wapply(plan->strips.p, (int)h.wapply.strips.size(), plan->srcs.p, plan->la.p, plan->t.p,
+        check(launch_wapply(plan->strips.p, plan->packs.p, npacks(h.wapply), plan->srcs.p, plan->la.p, plan->t.p,
                             ST_MISC, s),
               "diagonal W apply");
         check(launch_gemm(plan->g2.groups.p, plan->g2.tiles.p, plan->g2.ntiles, plan->g2.split.p, plan->g2.nsplit,

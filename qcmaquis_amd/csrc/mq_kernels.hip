@@ -45,10 +45,13 @@ __global__ void __launch_bounds__(256) k_to_f32(const double *__restrict__ src, 
 // reciprocal and corrected (exact for strips below 2^24 elements).
 constexpr int WU = 4;
 
-template <int STAGE, class T>
-__global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips, const Src *__restrict__ srcs,
-                                                const T *__restrict__ src, T *__restrict__ C) {
-    const Strip st = strips[blockIdx.x];
+// One workgroup per pack of consecutive strips (see StripList): a pack of one
+// large strip runs the strip with all threads; a pack of several small strips
+// runs their elements as one index range, each element finding its strip with a
+// binary search over the pack's element offsets in LDS.
+template <class T>
+__device__ __forceinline__ void wapply_strip(const Strip &st, const Src *__restrict__ srcs, const T *__restrict__ src,
+                                             T *__restrict__ C) {
     const int rows = st.rows, n = st.rows * st.cols;
     const float inv = 1.0f / (float)max(rows, 1);
     for (int base = 0; base < n; base += 256 * WU) {
@@ -77,6 +80,81 @@ __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips
         for (int u = 0; u < WU; ++u)
             if (jj[u] >= 0) C[st.dst + ii[u] + (int64_t)jj[u] * st.ldd] = acc[u];
     }
+}
+
+static_assert(kPackElems <= 256 * WU, "a pack of small strips is run in one pass");
+
+template <int STAGE, class T>
+__global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips, const int *__restrict__ packs,
+                                                const Src *__restrict__ srcs, const T *__restrict__ src,
+                                                T *__restrict__ C) {
+    const int p0 = packs[blockIdx.x], ns = packs[blockIdx.x + 1] - p0;
+    if (ns == 1) {
+        wapply_strip<T>(strips[p0], srcs, src, C);
+        return;
+    }
+    __shared__ Strip sst[kPackStrips];
+    __shared__ int pre[kPackStrips + 1];
+    if ((int)threadIdx.x < ns) sst[threadIdx.x] = strips[p0 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < ns; ++k) {
+            pre[k] = acc;
+            acc += sst[k].rows * sst[k].cols;
+        }
+        pre[ns] = acc;
+    }
+    __syncthreads();
+    const int total = pre[ns];  // at most kPackElems = 256 * WU: one pass
+    int s0[WU], s1[WU], ii[WU], jj[WU];
+    int64_t dst[WU];
+    int nsrc = 0;
+#pragma unroll
+    for (int u = 0; u < WU; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        s0[u] = s1[u] = 0;
+        ii[u] = jj[u] = 0;
+        dst[u] = -1;
+        if (e < total) {
+            int lo = 0, hi = ns - 1;  // last k with pre[k] <= e
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pre[mid] <= e) lo = mid;
+                else hi = mid - 1;
+            }
+            const Strip &st = sst[lo];
+            const int el = e - pre[lo];
+            jj[u] = el / st.rows;
+            ii[u] = el - jj[u] * st.rows;
+            s0[u] = st.s0;
+            s1[u] = st.s1;
+            dst[u] = st.dst + ii[u] + (int64_t)jj[u] * st.ldd;
+            nsrc = max(nsrc, st.s1 - st.s0);
+        }
+    }
+    // the elements' sources in lock step, so their loads are issued together
+    T acc[WU];
+#pragma unroll
+    for (int u = 0; u < WU; ++u) acc[u] = 0;
+    for (int q = 0; q < nsrc; ++q) {
+        T v[WU], c[WU];
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+            v[u] = 0;
+            c[u] = 0;
+            if (s0[u] + q < s1[u]) {
+                const Src sr = srcs[s0[u] + q];
+                c[u] = (T)sr.coef;
+                v[u] = src[sr.off + ii[u] + (int64_t)jj[u] * sr.ld];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < WU; ++u) acc[u] += c[u] * v[u];
+    }
+#pragma unroll
+    for (int u = 0; u < WU; ++u)
+        if (dst[u] >= 0) C[dst[u]] = acc[u];
 }
 
 // ------------------------------------------------------------------ grouped GEMM (MFMA)
@@ -374,24 +452,25 @@ hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s
 }
 
 template <class T>
-static hipError_t wapply_launch(const Strip *strips, int n, const Src *srcs, const T *src, T *C, int stage,
-                                hipStream_t s) {
-    if (n == 0) return hipSuccess;
+static hipError_t wapply_launch(const Strip *strips, const int *packs, int npacks, const Src *srcs, const T *src,
+                                T *C, int stage, hipStream_t s) {
+    if (npacks == 0) return hipSuccess;
+    const dim3 g(npacks), b(256);
     switch (stage) {
-        case ST_SIGMA: hipLaunchKernelGGL((k_wapply<ST_SIGMA, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
-        case ST_FUSE: hipLaunchKernelGGL((k_wapply<ST_FUSE, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
-        case ST_MISC: hipLaunchKernelGGL((k_wapply<ST_MISC, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
-        default: hipLaunchKernelGGL((k_wapply<ST_STEP, T>), dim3(n), dim3(256), 0, s, strips, srcs, src, C); break;
+        case ST_SIGMA: hipLaunchKernelGGL((k_wapply<ST_SIGMA, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
+        case ST_FUSE: hipLaunchKernelGGL((k_wapply<ST_FUSE, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
+        case ST_MISC: hipLaunchKernelGGL((k_wapply<ST_MISC, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
+        default: hipLaunchKernelGGL((k_wapply<ST_STEP, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
     }
     return hipGetLastError();
 }
-hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const double *T, double *C, int stage,
-                         hipStream_t s) {
-    return wapply_launch<double>(strips, n, srcs, T, C, stage, s);
+hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const double *T,
+                         double *C, int stage, hipStream_t s) {
+    return wapply_launch<double>(strips, packs, npacks, srcs, T, C, stage, s);
 }
-hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const float *T, float *C, int stage,
-                         hipStream_t s) {
-    return wapply_launch<float>(strips, n, srcs, T, C, stage, s);
+hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const float *T, float *C,
+                         int stage, hipStream_t s) {
+    return wapply_launch<float>(strips, packs, npacks, srcs, T, C, stage, s);
 }
 
 template <int STAGE, class T, class TO>

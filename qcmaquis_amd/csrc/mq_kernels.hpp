@@ -16,10 +16,11 @@ enum Stage { ST_SIGMA = 0, ST_SIGMA_OUT = 1, ST_STEP = 2, ST_STEP_OUT = 3, ST_FU
 hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *dst, hipStream_t s);
 hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s);
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s);
-hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const double *T, double *C, int stage,
-                         hipStream_t s);
-hipError_t launch_wapply(const Strip *strips, int n, const Src *srcs, const float *T, float *C, int stage,
-                         hipStream_t s);
+// one workgroup per pack of strips (StripList::packs, npacks = packs.size() - 1)
+hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const double *T,
+                         double *C, int stage, hipStream_t s);
+hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const float *T, float *C,
+                         int stage, hipStream_t s);
 // grouped GEMM over a plan's tile list (FP64; FP32 operands with FP32 or FP64 output);
 // rowmap: the row placements of groups with rowmap > 0 (may be null when none has)
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,

@@ -864,6 +864,26 @@ void order_tiles(GemmList &gl) {
     gl.tiles = xcd_order(t, gl.groups);
 }
 
+void make_packs(StripList &sl) {
+    sl.packs.clear();
+    const int n = (int)sl.strips.size();
+    int i = 0;
+    while (i < n) {
+        sl.packs.push_back(i);
+        int64_t el = (int64_t)sl.strips[i].rows * sl.strips[i].cols;
+        int j = i + 1;
+        if (el <= kPackElems)
+            while (j < n && j - i < kPackStrips) {
+                const int64_t e = (int64_t)sl.strips[j].rows * sl.strips[j].cols;
+                if (el + e > kPackElems) break;
+                el += e;
+                ++j;
+            }
+        i = j;
+    }
+    sl.packs.push_back(n);
+}
+
 // ---------------- several vectors per launch ------------------------------
 std::vector<Copy2D> batch_copies(const std::vector<Copy2D> &v, int nvec, int64_t src_step, int64_t dst_step) {
     std::vector<Copy2D> out;
@@ -894,6 +914,7 @@ StripList batch_strips(const StripList &sl, int nvec, int64_t src_step, int64_t 
             out.strips.push_back(st);
         }
     }
+    make_packs(out);
     return out;
 }
 
@@ -1009,6 +1030,8 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     P.min_bytes = 8.0 * (L.total + R.total + 2.0 * s.X.total);
     order_tiles(P.g1);
     order_tiles(P.g2);
+    make_packs(P.wapply);
+    make_packs(P.combL);
     return P;
 }
 
@@ -1118,6 +1141,8 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     P.min_bytes = 8.0 * (L.total + off + s.X.total);
     order_tiles(P.g1);
     order_tiles(P.g2);
+    make_packs(P.wapply);
+    make_packs(P.combL);
     return P;
 }
 
@@ -1388,6 +1413,8 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     P.min_bytes = 8.0 * (R.total + off + s.X.total);
     order_tiles(P.g1);
     order_tiles(P.g2);
+    make_packs(P.wapply);
+    make_packs(P.combL);
     return P;
 }
 
@@ -1638,6 +1665,7 @@ HostPlan build_diag_plan(const mq_site_desc &d, int tm, int tn, int kc) {
         }
     }
     order_tiles(P.g2);
+    make_packs(P.wapply);
     P.exec_flops = P.g2.flops() + 2.0 * src_el;
     return P;
 }

@@ -75,10 +75,17 @@ struct GemmList {
         return f;
     }
 };
+// Consecutive strips are run in packs of one workgroup each: a pack is either
+// one strip of more than kPackElems elements, or up to kPackStrips strips of
+// together at most kPackElems elements (the fragmented blocks of small sectors).
+constexpr int kPackStrips = 64, kPackElems = 1024;
 struct StripList {
     std::vector<Strip> strips;
     std::vector<Src> srcs;
+    std::vector<int> packs;  // pack p runs strips [packs[p], packs[p + 1])
 };
+// (re)compute the packs of a strip list, keeping the strip order
+void make_packs(StripList &sl);
 
 enum PlanKind { PLAN_SIGMA = 0, PLAN_LSTEP = 1, PLAN_RSTEP = 2, PLAN_DIAG = 3 };
 
