@@ -156,6 +156,18 @@ class OracleEngine:
         """Davidson with the diagonal preconditioner (SU2 only), optimize/davidson.h:64-161."""
         return self._solve(lib().orc_davidson, symm, psi, L, R, W, max_iter, tol, ortho, x0)
 
+    def block_solve(self, symm, psis, L, R, W, max_iter=10, tol=1e-8):
+        """The len(psis) lowest eigenpairs as the reference finds excited states: one
+        Jacobi-Davidson solve per root, each constrained orthogonal to the roots
+        before it (SingleSiteVS ortho vectors)."""
+        thetas, us, its = [], [], 0
+        for psi in psis:
+            th, u, it = self.jd(symm, psi, L, R, W, max_iter, tol, ortho=us)
+            thetas.append(th)
+            us.append(u)
+            its += it
+        return thetas, us, its
+
     def diagonal(self, symm, psi, L, R, W):
         """Diagonal of H_eff, non-abelian/h_diag.hpp:120-152, flat left-paired layout."""
         sd = SiteDesc(symm, psi.phys, psi.left, psi.right, W, L.layout, R.layout)

@@ -308,6 +308,21 @@ class GpuEngine:
         finally:
             sp.close()
 
+    def block_solve(self, symm, psis, L, R, W, max_iter=10, tol=1e-8):
+        """The len(psis) lowest eigenpairs together (mq_block_davidson, one batched
+        sigma per block iteration): (thetas, [u_r], block iterations).  SU2U1
+        problems precondition the corrections with the diagonal of H_eff."""
+        sp = self.sigma_plan(symm, psis[0], L, R, W)
+        try:
+            X0 = self.torch.as_tensor(np.stack([p.to_flat() for p in psis])).to(self.device)
+            diag = sp.diagonal() if symm == "su2u1" else None
+            thetas, U, its = sp.block_davidson(X0, max_iter, tol, diag=diag)
+            self.flops += its * len(psis) * sp.ref_flops
+            Uh = U.cpu().numpy()
+            return thetas, [Uh[r] for r in range(len(psis))], its
+        finally:
+            sp.close()
+
     def diagonal(self, symm, psi, L, R, W):
         """Diagonal of H_eff (SU2U1), contraction::SU2::diagonal_hamiltonian."""
         sp = self.sigma_plan(symm, psi, L, R, W)

@@ -579,3 +579,38 @@ def split_twosite(u: MPSTensor, p1, p2, left, right, Mmax, cutoff, l2r=True, su2
         US = BlockMatrix({(lc, rc): u_ * Sb[rc][None, :] for (lc, rc), u_ in Ub.blocks.items()})
         A = MPSTensor(p1, left, mid, US, "L")
     return A, B, info
+
+
+def split_twosite_sa(us, p1, p2, left, right, Mmax, cutoff, l2r=True, su2=True):
+    """State-averaged split of several two-site tensors sharing one basis.
+
+    The two-site tensors of the roots are laid side by side (l2r: columns, r2l:
+    rows) with equal weights 1/sqrt(k); the block SVD of that matrix with the
+    global truncation of svd_truncate gives the shared tensor (the leading
+    eigenvectors of the averaged density matrix), and each root keeps its own
+    projection onto it: l2r returns (A, [B_r]), r2l returns ([A_r], B)."""
+    boths = []
+    for u in us:
+        u.make_left_paired()
+        lp = u.data
+        if su2:
+            lp = unreduce_left(p1, p2, left, right, lp)
+        boths.append(ts_left_to_both(p1, p2, left, right, lp))
+    keys = list(boths[0].blocks.keys())
+    for b in boths[1:]:
+        if set(b.blocks.keys()) != set(keys):
+            raise ValueError("state-averaged split: the roots have different block structures")
+    w = 1.0 / np.sqrt(len(us))
+    join = np.hstack if l2r else np.vstack
+    stacked = BlockMatrix({k: w * join([b.blocks[k] for b in boths]) for k in keys})
+    Ub, Sb, Vb, info = svd_truncate(stacked, Mmax, cutoff)
+    mid = Index.from_dict({rc: len(s_) for rc, s_ in Sb.items()})
+    if l2r:
+        A = MPSTensor(p1, left, mid, Ub, "L")
+        Bs = [MPSTensor(p2, mid, right, BlockMatrix({k: Ub.blocks[k].T @ b.blocks[k] for k in Ub.blocks}), "R")
+              for b in boths]
+        return A, Bs, info
+    B = MPSTensor(p2, mid, right, Vb, "R")
+    As = [MPSTensor(p1, left, mid, BlockMatrix({k: b.blocks[k] @ Vb.blocks[k].T for k in Vb.blocks}), "L")
+          for b in boths]
+    return As, B, info

@@ -169,6 +169,83 @@ class TwoSiteDMRG:
         return self.energies[-1]
 
 
+class StateAveragedDMRG(TwoSiteDMRG):
+    """The nroots lowest states in one shared basis (state-averaged two-site DMRG,
+    equal weights; no counterpart in the reference, which optimises excited
+    states one at a time under orthogonality constraints).
+
+    Every local problem is solved for its nroots lowest eigenpairs together
+    (engine.block_solve: the batched-sigma block Davidson on the GPU, sequential
+    constrained Jacobi-Davidson on the oracle), and the truncation keeps the
+    leading singular vectors of the roots' two-site tensors side by side
+    (split_twosite_sa).  The roots' own parts travel with the sweep as
+    `centers` at `center_site`, one of the two sites of the next problem (the
+    backward pass ends at site 1 and the next forward pass starts at site 0)."""
+
+    def __init__(self, ham, engine, max_bond_dimension, nroots, seed=42, **kw):
+        super().__init__(ham, engine, max_bond_dimension, seed=seed, **kw)
+        import numpy as np
+        rng = np.random.default_rng(seed + 1)
+        c0 = self.mps[0]
+        self.nroots = nroots
+        self.centers = [c0.copy()]
+        for _ in range(1, nroots):
+            c = c0.copy()
+            c.make_left_paired()
+            for k in c.data.blocks:
+                c.data.blocks[k] = rng.standard_normal(c.data.blocks[k].shape)
+            self.centers.append(c)
+        self.center_site = 0
+        self.root_energies = []
+
+    def microiteration(self, sweep, i):
+        from .mps import split_twosite_sa
+        ham, L = self.ham, self.ham.L
+        site = micro_site(L, i)
+        forward = i < L - 1
+        change = (i == L - 2) or (i == n_microiterations(L) - 1)
+        t0 = time.perf_counter()
+        if self.center_site == site:
+            psis = [make_twosite_mps(c, self.mps[site + 1], su2=ham.su2) for c in self.centers]
+        elif self.center_site == site + 1:
+            psis = [make_twosite_mps(self.mps[site], c, su2=ham.su2) for c in self.centers]
+        else:
+            raise RuntimeError(f"state-averaged sweep: roots at site {self.center_site}, problem at {site}")
+        Lb, Rb, W = self.left[site], self.right[site + 2], ham.ts_mpo(site)
+        thetas, us, its = self.engine.block_solve(ham.symm, psis, Lb, Rb, W, self.maxiter, self.tol)
+        t1 = time.perf_counter()
+        psi = psis[0]
+        uts = [MPSTensor.from_flat(psi.phys, psi.left, psi.right, u) for u in us]
+        l2r = forward and not change
+        left_i, right_i = self.mps[site].left, self.mps[site + 1].right
+        if l2r:
+            A, Bs, info = split_twosite_sa(uts, ham.phys, ham.phys, left_i, right_i, self.m, self.cutoff, l2r=True,
+                                           su2=ham.su2)
+            self.mps[site], self.mps[site + 1] = A, Bs[0]
+            self.centers, self.center_site = Bs, site + 1
+            self.left[site + 1] = self.engine.left_step(ham.symm, A, self.left[site], ham.mpo[site])
+        else:
+            As, B, info = split_twosite_sa(uts, ham.phys, ham.phys, left_i, right_i, self.m, self.cutoff, l2r=False,
+                                           su2=ham.su2)
+            self.mps[site], self.mps[site + 1] = As[0], B
+            self.centers, self.center_site = As, site
+            self.right[site + 1] = self.engine.right_step(ham.symm, B, self.right[site + 2], ham.mpo[site + 1])
+        energies = [t + ham.ecore for t in thetas]
+        rec = dict(sweep=sweep, micro=i, site=site, energies=energies, iters=its, bond=info["bond_dimension"],
+                   truncated_weight=info["truncated_weight"], t_solve=t1 - t0, t_rest=time.perf_counter() - t1)
+        self.records.append(rec)
+        self.root_energies.append(energies)
+        self.energies.append(energies[0])
+        if self.log:
+            self.log(rec)
+        return energies
+
+    def run(self, nsweeps):
+        for s in range(nsweeps):
+            self.sweep(s)
+        return self.root_energies[-1]
+
+
 def expval(engine, ham: Hamiltonian, mps, from_right=False):
     """<mps|H|mps> / <mps|mps> by propagating a boundary through the whole MPS
     with the engine's boundary steps (mp_tensors/mps_mpo_ops.h:42-103: the trace
