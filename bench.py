@@ -9,12 +9,15 @@ boundary steps.
 
 A step is one sigma-vector H.psi with the boundaries and psi resident in HBM.
 value = algorithmic FLOPs of the reference schedule / wall time per step.
-Both forms are measured: the plain FP64 sigma and the mixed form (FP32
-panels, intermediates and MFMA, FP64 output; Davidson in FP64 with one FP64
-defect-correction sigma per solve).  The mixed form is reported as the value
-when its Davidson eigenvalue agrees with the FP64 one to 1e-8 Eh at
-convergence (both solves started from the FP64-converged eigenvector);
-otherwise the FP64 form is reported.
+Three forms are measured: the plain FP64 sigma, the mixed form (FP32 panels,
+intermediates and MFMA, FP64 output; Davidson in FP64 with an FP64 defect
+correction per solve) and the BF16x3 form (the mixed form with every GEMM
+product as three bf16 MFMA products accumulated in FP32).  The value is the
+fastest form whose Davidson eigenvalue agrees with the FP64 one to 1e-8 Eh at
+convergence (all solves started from the FP64-converged eigenvector).
+The 4-root block Davidson (one batched sigma of the block per iteration) is
+timed in each form on the same problem, with its converged roots compared
+with the FP64 ones.
 The JSON line also carries ms per Jacobi-Davidson iteration, per-stage HIP
 event timings on the library stream, the roofline of the dominant kernel and
 the CPU oracle timed on a bounded sample of the same sigma.
@@ -33,6 +36,8 @@ sys.path.insert(0, ROOT)
 METRIC = "H·ψ TFLOP/s + ms/Davidson iter, L=24 m=512; mixed vs FP64 at ΔE≤1e-8 Eh"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+# BF16x3: three bf16 MFMA products (dense bf16 peak 2.5 PFLOP/s) per FP32-equivalent product
+PEAK_BF16X3_TFLOPS = 2500.0 / 3
 PEAK_HBM_GBS = 8000.0
 STAGES = ["arrange", "reshape", "gemm_T", "w_apply", "gemm_out"]
 
@@ -77,6 +82,7 @@ def main():
                     help="recorded bond sectors (default: tests/golden/sectors_L24_m512.json for SU2U1 L=24)")
     ap.add_argument("--symm", default="su2u1", choices=["su2u1", "2u1"])
     ap.add_argument("--decay", type=float, default=None, help="integral decay length (exp(-|p-q|/decay))")
+    ap.add_argument("--roots", type=int, default=4, help="block Davidson roots to time (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-delta-e", action="store_true", help="skip the converged-solve energy comparison")
@@ -92,7 +98,7 @@ def main():
 
     import numpy as np
     import torch
-    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    from qcmaquis_amd.desc import MQ_PREC_BF16X3, MQ_PREC_FP32, MQ_PREC_FP64
     from dataclasses import asdict
 
     from qcmaquis_amd.autotune import DEFAULT as DEFAULT_SCHEDULE, table_json, tune
@@ -176,11 +182,13 @@ def main():
         ms_jd = (time.perf_counter() - t0) / max(its, 1) * 1e3
         return dict(dt=dt, avg=avg, work=work, theta=theta, its=its, ms_jd=ms_jd, y=ycopy)
 
-    r64 = measure(MQ_PREC_FP64)
-    log("fp64 stages ms: " + ", ".join(f"{n} {v:.3f}" for n, v in zip(STAGES, r64["avg"])))
-    r32 = measure(MQ_PREC_FP32)
-    log("mixed stages ms: " + ", ".join(f"{n} {v:.3f}" for n, v in zip(STAGES, r32["avg"])))
-    sig_err = float(np.abs(r32["y"] - r64["y"]).max() / np.abs(r64["y"]).max())
+    FORMS = (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32), ("bf16x3", MQ_PREC_BF16X3))
+    res = {}
+    for name, prec in FORMS:
+        res[name] = measure(prec)
+        log(f"{name} stages ms: " + ", ".join(f"{n} {v:.3f}" for n, v in zip(STAGES, res[name]["avg"])))
+    r64, r32 = res["fp64"], res["mixed"]
+    sig_err = {k: float(np.abs(res[k]["y"] - r64["y"]).max() / np.abs(r64["y"]).max()) for k in ("mixed", "bf16x3")}
     # energy agreement at convergence, the regime of the solves that fix a sweep's
     # final energy: the lowest eigenvector is first converged in FP64 (restarted
     # solves of 40 iterations), then both forms run one more solve from it
@@ -193,26 +201,62 @@ def main():
             if its < 40:
                 break
         conv = {}
-        for name, prec in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)):
-            th, _, its = plans[name].jd(u, 10, 1e-10, prec=prec)
+        for name, prec in FORMS:
+            th, _, its = plans["fp64" if name == "fp64" else "mixed"].jd(u, 10, 1e-10, prec=prec)
             conv[name] = dict(theta=th + ham.ecore, iterations=its)
-        dE = abs(conv["mixed"]["theta"] - conv["fp64"]["theta"])
-        log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it) mixed "
-            f"{conv['mixed']['theta']:.12f} ({conv['mixed']['iterations']} it) dE {dE:.2e}; "
-            f"mixed sigma rel err {sig_err:.2e}")
-    mixed_ok = dE is not None and dE <= 1e-8
-    ch, prec_name = (r32, "mixed") if mixed_ok else (r64, "fp64")
+        dE = {k: abs(conv[k]["theta"] - conv["fp64"]["theta"]) for k in ("mixed", "bf16x3")}
+        log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it); "
+            + "; ".join(f"{k} {conv[k]['theta']:.12f} ({conv[k]['iterations']} it) dE {dE[k]:.2e}, sigma rel err "
+                        f"{sig_err[k]:.2e}" for k in dE))
+    ok = ["fp64"] + [k for k in ("mixed", "bf16x3") if dE is not None and dE[k] <= 1e-8]
+    prec_name = min(ok, key=lambda k: res[k]["dt"])
+    ch = res[prec_name]
     dt, avg, work = ch["dt"], ch["avg"], ch["work"]
+
+    # block Davidson over `roots` roots: ms per block iteration (one batched sigma of
+    # the block) in each form, and the converged roots against the FP64 ones
+    block = None
+    if a.roots > 0:
+        k = a.roots
+        rng = np.random.default_rng(11)
+        X0 = torch.as_tensor(np.stack([psi.to_flat()] + [rng.standard_normal(sp.n) for _ in range(k - 1)]))
+        X0 = X0.to(eng.device)
+        block = {"roots": k}
+        ref = None
+        for name, prec in FORMS:
+            bp = plans["fp64" if name == "fp64" else "mixed"]
+            diag = bp.diagonal() if ham.symm == "su2u1" else None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, its = bp.block_davidson(X0, a.jd_iters, 0.0, prec=prec, diag=diag)
+            eng.sync()
+            torch.cuda.synchronize()
+            ms_it = (time.perf_counter() - t0) / max(its, 1) * 1e3
+            t0 = time.perf_counter()
+            th, _, itc = bp.block_davidson(X0, 400, 1e-9, prec=prec, diag=diag)
+            eng.sync()
+            t_conv = time.perf_counter() - t0
+            th = [t + ham.ecore for t in th]
+            if ref is None:
+                ref = th
+            block[name] = dict(ms_per_block_iter=ms_it, converged_iterations=itc, converge_s=t_conv, energies=th,
+                               max_delta_E_vs_fp64=max(abs(x - y) for x, y in zip(th, ref)))
+            log(f"block Davidson {k} roots {name}: {ms_it:.2f} ms/iteration; converged in {itc} iterations "
+                f"({t_conv:.2f} s), max dE vs fp64 {block[name]['max_delta_E_vs_fp64']:.2e}")
+        for name in ("mixed", "bf16x3"):
+            block[name]["speedup_vs_fp64"] = block["fp64"]["ms_per_block_iter"] / block[name]["ms_per_block_iter"]
 
     # roofline of the dominant kernel of the reported form: the binding roof is the
     # one with the larger lower bound on the time (algorithmic flops / MFMA peak vs
     # algorithmic bytes / HBM peak)
     dom = max(range(2, 5), key=lambda i: avg[i])
-    tname = "float" if prec_name == "mixed" else "double"
-    kname = {2: f"k_gemm<0, {tname}, {tname}>", 3: f"k_wapply<0, {tname}>", 4: f"k_gemm<1, {tname}, double>"}[dom]
+    tname = "double" if prec_name == "fp64" else "float"
+    x3 = ", true" if prec_name == "bf16x3" else ", false"
+    kname = {2: f"k_gemm<0, {tname}, {tname}{x3}>", 3: f"k_wapply<0, {tname}>",
+             4: f"k_gemm<1, {tname}, double{x3}>"}[dom]
     flops = {2: work[0], 3: work[1], 4: work[2]}[dom]
     nbytes = {2: work[3], 3: work[4], 4: work[5]}[dom]
-    peak_f = PEAK_FP64_TFLOPS if prec_name == "fp64" else PEAK_FP32_TFLOPS
+    peak_f = {"fp64": PEAK_FP64_TFLOPS, "mixed": PEAK_FP32_TFLOPS, "bf16x3": PEAK_BF16X3_TFLOPS}[prec_name]
     t_s = avg[dom] * 1e-3
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "sigma_L24_m512_traffic.json")
@@ -253,7 +297,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if prec_name == "mixed" else "f64",
+        "dtype": {"fp64": "f64", "mixed": "f32", "bf16x3": "bf16x3 (f32 accumulation)"}[prec_name],
         "data": "synthetic",
         "config": {
             "workload": f"{'SU2U1' if a.symm == 'su2u1' else 'TwoU1'} two-site sigma, L={a.L} N={a.L} S=0 "
@@ -268,16 +312,21 @@ def main():
         "ms_per_davidson_iter": ch["ms_jd"],
         "davidson_iters": ch["its"],
         "flops_per_sigma": sp.ref_flops,
-        "executed_flops_per_sigma": plans[prec_name].exec_flops,
+        "executed_flops_per_sigma": plans["fp64" if prec_name == "fp64" else "mixed"].exec_flops,
         "plan_build_s": t_plan,
         "schedule": {k: asdict(v[0]) for k, v in tuned.items()},
         "autotune": None if a.no_autotune else {k: table_json(v[1][:5]) for k, v in tuned.items()},
         "fp64": summary(r64),
         "mixed": summary(r32),
+        "bf16x3": summary(res["bf16x3"]),
         "mixed_speedup_vs_fp64": r64["dt"] / r32["dt"],
-        "delta_E_mixed_vs_fp64_Eh": dE,
+        "bf16x3_speedup_vs_fp64": r64["dt"] / res["bf16x3"]["dt"],
+        "delta_E_mixed_vs_fp64_Eh": None if dE is None else dE["mixed"],
+        "delta_E_bf16x3_vs_fp64_Eh": None if dE is None else dE["bf16x3"],
         "converged_solves": conv,
-        "mixed_sigma_max_rel_err": sig_err,
+        "mixed_sigma_max_rel_err": sig_err["mixed"],
+        "bf16x3_sigma_max_rel_err": sig_err["bf16x3"],
+        "block_davidson": block,
         "parity_max_rel_err_vs_oracle": err,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -225,10 +225,13 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
  * n_ortho vectors (device, each vec_len) are projected out as
  * SingleSiteVS::project does (ietl_lanczos_solver.h:90-95). */
 /* With prec != MQ_PREC_FP64 the iterations run in that form and the returned
- * eigenvalue is the FP64 Rayleigh quotient of the Ritz vector; when the FP64
- * residual of that pair misses tol (tol > 0), the solve continues in FP64
- * from the Ritz vector (up to max_iter more iterations, counted in
- * *iterations).  The same holds for mq_davidson and mq_block_davidson. */
+ * eigenvalue is the FP64 Rayleigh quotient of the Ritz vector.  With tol > 0
+ * those iterations stop at tol or at the form's rounding floor, whichever is
+ * larger (|r| <= 1e-6 |theta| for MQ_PREC_FP32, 2e-5 |theta| for
+ * MQ_PREC_BF16X3), and when the FP64 residual of the pair misses tol, the
+ * solve continues in FP64 from the Ritz vector (up to max_iter more
+ * iterations, counted in *iterations).  The same holds for mq_davidson and
+ * mq_block_davidson. */
 int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R,
                        const double *x0, double *u, double *theta,
                        int max_iter, double tol, int prec,
@@ -288,6 +291,12 @@ int mq_block_davidson(mq_plan *plan, const double *L, const double *R,
 int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
                 const double *A, int lda, const double *B, int ldb,
                 double *C, int ldc, int kchunk);
+/* The same for the FP32-operand kernel of the mixed forms, with the tile shape
+ * tm x tn (tm 64 or 128, tn a multiple of 16 up to 128): FP32 products, or the
+ * BF16x3 products of MQ_PREC_BF16X3 when bf16x3 != 0. */
+int mq_gemm_f32(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
+                const float *A, int lda, const float *B, int ldb,
+                float *C, int ldc, int kchunk, int tm, int tn, int bf16x3);
 
 /* -- host-side plan introspection (no GPU needed) ------------------------ */
 /* Builds the same task lists the device plans execute, on the host only, so

@@ -339,6 +339,7 @@ static void bind32(mq_plan *p, hipStream_t s) {
 static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEvent_t *ev) {
     hipStream_t s = p->ctx->stream;
     HostPlan &h = p->hp;
+    const bool x3 = prec == MQ_PREC_BF16X3;
     auto mark = [&](int i) {
         if (ev) check(hipEventRecord(ev[i], s), "record");
     };
@@ -359,14 +360,14 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
         check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp32.p, s), "reshape");
         mark(2);
         check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la32.p,
-                          p->prp32.p, p->t32.p, p->ws32.p, p->g1.rowmap.p, ST_SIGMA, s),
+                          p->prp32.p, p->t32.p, p->ws32.p, p->g1.rowmap.p, ST_SIGMA, s, x3),
               "gemm T");
         mark(3);
         check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, p->t32.p, p->c32.p, ST_SIGMA, s),
               "W apply");
         mark(4);
         check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c32.p,
-                          p->rb32.p, y, p->ws32.p, p->g2.rowmap.p, ST_SIGMA_OUT, s),
+                          p->rb32.p, y, p->ws32.p, p->g2.rowmap.p, ST_SIGMA_OUT, s, x3),
               "gemm R");
     }
     mark(5);
@@ -458,11 +459,11 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
     } else {
         check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp32.p, s), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la32.p,
-                          B.prp32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s),
+                          B.prp32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s, prec == MQ_PREC_BF16X3),
               "gemm T");
         check(launch_wapply(B.strips.p, B.packs.p, (int)B.npacks, B.srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s), "W apply");
         check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c32.p,
-                          p->rb32.p, Y, B.ws32.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
+                          p->rb32.p, Y, B.ws32.p, B.g2.rowmap.p, ST_SIGMA_OUT, s, prec == MQ_PREC_BF16X3),
               "gemm R");
     }
 }
@@ -512,8 +513,7 @@ static void solver_sigma(mq_plan *p, const double *L, const double *R, const dou
 }
 
 static const char *prec_error(int prec) {
-    if (prec == MQ_PREC_FP64 || prec == MQ_PREC_FP32) return nullptr;
-    if (prec == MQ_PREC_BF16X3) return "MQ_PREC_BF16X3 is not implemented";
+    if (prec == MQ_PREC_FP64 || prec == MQ_PREC_FP32 || prec == MQ_PREC_BF16X3) return nullptr;
     return "unknown precision";
 }
 
@@ -639,6 +639,41 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const 
         ws.alloc((size_t)gl.ws);
         check(launch_gemm(dg.groups.p, dg.tiles.p, dg.ntiles, dg.split.p, dg.nsplit, A, B, C, ws.p, dg.rowmap.p, ST_MISC,
                           ctx->stream),
+              "gemm");
+        check(hipStreamSynchronize(ctx->stream), "gemm sync");
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(ctx, e, MQ_ERR_HIP);
+    }
+}
+
+// The FP32-operand GEMM of the mixed forms on one group with a given tile
+// shape: FP32 products, or BF16x3 products (bf16x3 != 0).
+int mq_gemm_f32(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const float *A, int lda, const float *B,
+                int ldb, float *C, int ldc, int kchunk, int tm, int tn, int bf16x3) {
+    if (!ctx || M < 0 || N < 0 || K < 0 || kchunk < 16 || (tm != 64 && tm != 128) || tn < 16 || tn > 128 ||
+        tn % 16)
+        return MQ_ERR_ARG;
+    try {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        GemmGroup g{};
+        g.M = M;
+        g.N = N;
+        g.K = K;
+        g.lda = lda;
+        g.ldb = ldb;
+        g.ldc = ldc;
+        g.transA = transA ? 1 : 0;
+        g.transB = transB ? 1 : 0;
+        GemmList gl;
+        tile_gemm(gl, g, tm, tn, kchunk);
+        order_tiles(gl);
+        DevGemm dg;
+        dg.upload(gl, ctx->stream);
+        DevArray<float> ws;
+        ws.alloc((size_t)gl.ws);
+        check(launch_gemm(dg.groups.p, dg.tiles.p, dg.ntiles, dg.split.p, dg.nsplit, A, B, C, ws.p, dg.rowmap.p, ST_MISC,
+                          ctx->stream, bf16x3 != 0),
               "gemm");
         check(hipStreamSynchronize(ctx->stream), "gemm sync");
         return MQ_OK;
@@ -851,6 +886,16 @@ void sym_eig(int n, std::vector<double> A, std::vector<double> &w, std::vector<d
 
 }  // namespace
 
+// Convergence threshold of the iterations in form prec: with tol > 0 the mixed
+// forms stop at their rounding floor (the residual relative to |theta| that
+// their sigma vectors cannot resolve: FP32 about 1e-6, BF16x3 about 2e-5) when
+// that is above tol, and the FP64 continuation finishes the solve; tol <= 0
+// (run max_iter iterations) is kept as given.
+static double phase_tol(double tol, int prec) {
+    if (tol <= 0 || prec == MQ_PREC_FP64) return tol;
+    return std::max(tol, prec == MQ_PREC_BF16X3 ? 2e-5 : 1e-6);
+}
+
 extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R, const double *x0, double *u_out,
                                   double *theta_out, int max_iter, double tol, int prec, const double *const *ortho,
                                   int n_ortho, int *iterations) {
@@ -865,6 +910,7 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
         return MQ_ERR_ARG;
     }
     try {
+        const double ptol = phase_tol(tol, prec);
         JD jd;
         jd.p = plan;
         jd.s = plan->ctx->stream;
@@ -971,7 +1017,7 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             // |r|, r.u and u.u in one batch: convergence test and correction t = -r + (r.u)/(u.u) u
             std::vector<double> rr = jd.dots({uA, uA, u}, {uA, u, u});
             double rn = std::sqrt(rr[0]);
-            if (rn <= tol * std::fabs(theta) || rn < tol || it >= max_iter) break;
+            if (rn <= ptol * std::fabs(theta) || rn < ptol || it >= max_iter) break;
             jd.lincomb({uA, u}, {-1.0, rr[1] / rr[2]}, 0.0, V(it));
         }
         bool refine = false;
@@ -1034,6 +1080,7 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
         return MQ_ERR_ARG;
     }
     try {
+        const double ptol = phase_tol(tol, prec);
         JD jd;
         jd.p = plan;
         jd.s = plan->ctx->stream;
@@ -1151,7 +1198,7 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
             ++it;
             std::vector<double> rr = jd.dots({r, vc, u}, {r, r, vc});
             const double rn = std::sqrt(rr[0]);
-            if (rn <= tol * std::fabs(theta) || rn < tol || it >= max_iter) break;
+            if (rn <= ptol * std::fabs(theta) || rn < ptol || it >= max_iter) break;
             if (dim >= kMaxDim) {
                 // restart with the two lowest Ritz vectors and their images
                 std::vector<double> y1(E.begin() + dim, E.begin() + 2 * dim);
@@ -1228,6 +1275,7 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
     const int k = nroots;
     const int cap = 60;  // basis capacity
     try {
+        const double ptol = phase_tol(tol, prec);
         JD jd;
         jd.p = plan;
         jd.s = plan->ctx->stream;
@@ -1354,7 +1402,7 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
             std::vector<int> open;
             for (int r = 0; r < k; ++r) {
                 const double rn = std::sqrt(rr[3 * r]);
-                if (!(rn <= tol * std::fabs(theta[r]) || rn < tol)) open.push_back(r);
+                if (!(rn <= ptol * std::fabs(theta[r]) || rn < ptol)) open.push_back(r);
             }
             if (open.empty() || it >= max_iter) break;
             if (dim + (int)open.size() > cap) {

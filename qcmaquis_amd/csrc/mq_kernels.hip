@@ -201,6 +201,30 @@ struct Mfma<float> {
     static __device__ __forceinline__ int dcol(int r, int lq) { return 4 * lq + r; }
 };
 
+// BF16x3 form of an FP32 product: each operand element x is split into bf16
+// parts hi = bf16(x) and lo = bf16(x - hi) (round to nearest even) when it is
+// staged into LDS, and C += A_lo B_hi + A_hi B_lo + A_hi B_hi is accumulated in
+// FP32 with v_mfma_f32_16x16x16_bf16 (operand lanes: X[lane % 16][4 * (lane / 16) + q]
+// and Y[4 * (lane / 16) + q][lane % 16] for q = 0..3; accumulators as the FP32
+// instruction).  The dropped A_lo B_lo term and the split leave a relative error
+// of about 2^-16 per product.  LDS holds, per row of A (or column of B) and
+// group of four k, the four hi parts followed by the four lo parts (16 bytes),
+// rows padded to five such groups.
+typedef short sh4 __attribute__((ext_vector_type(4)));
+struct Bf16Quad {
+    sh4 hi, lo;
+};
+constexpr int X3_QUADS = 5;  // 4 groups of k per K step of 16, plus one of padding
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ void bf16_split(float x, short &hi, short &lo) {
+    const unsigned short h = bf16_rne(x);
+    hi = (short)h;
+    lo = (short)bf16_rne(x - __uint_as_float((unsigned)h << 16));
+}
+
 template <int TM, int TN, class T>
 struct GemmStage {
     T a[TM * kstep<T>() / 256], b[TN * kstep<T>() / 256];
@@ -266,18 +290,47 @@ __device__ __forceinline__ void gemm_stash(const GemmGroup &g, const GemmStage<T
     }
 }
 
+// Split the staged FP32 tile into the bf16 hi/lo groups of the BF16x3 form.
+template <int TM, int TN>
+__device__ __forceinline__ void gemm_stash_x3(const GemmGroup &g, const GemmStage<TM, TN, float> &st,
+                                              Bf16Quad (*Ax)[X3_QUADS], Bf16Quad (*Bx)[X3_QUADS]) {
+    constexpr int TK = kstep<float>();
+#pragma unroll
+    for (int r = 0; r < TM * TK / 256; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        const int i = !g.transA ? e % TM : e / TK, k = !g.transA ? e / TM : e % TK;
+        short h, l;
+        bf16_split(st.a[r], h, l);
+        Ax[i][k >> 2].hi[k & 3] = h;
+        Ax[i][k >> 2].lo[k & 3] = l;
+    }
+#pragma unroll
+    for (int r = 0; r < TN * TK / 256; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        const int j = !g.transB ? e / TK : e % TN, k = !g.transB ? e % TK : e / TN;
+        short h, l;
+        bf16_split(st.b[r], h, l);
+        Bx[j][k >> 2].hi[k & 3] = h;
+        Bx[j][k >> 2].lo[k & 3] = l;
+    }
+}
+
 // T: operand type; TO: output type; partial tiles of split-K groups are
-// stored in T and summed by k_split_reduce.
-template <int TM, int TN, class T, class TO>
+// stored in T and summed by k_split_reduce.  X3 (T = float only): BF16x3 products.
+template <int TM, int TN, class T, class TO, bool X3>
 __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t, const T *__restrict__ A,
                                           const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
                                           const RowMap *__restrict__ rowmap, double *lds) {
+    static_assert(!X3 || sizeof(T) == 4, "BF16x3 products take FP32 operands");
     typedef Mfma<T> MF;
     constexpr int TK = kstep<T>();
     // the four waves split the tile along M; each holds TM/4 rows x TN columns
     constexpr int MI = TM / 64, NJ = TN / 16;  // 16x16 MFMA blocks per wave
     T(*As)[TK][TM + 2] = reinterpret_cast<T(*)[TK][TM + 2]>(lds);
     T(*Bs)[TK][TN + 2] = reinterpret_cast<T(*)[TK][TN + 2]>(reinterpret_cast<T *>(lds) + 2 * TK * (TM + 2));
+    Bf16Quad(*Ax)[TM][X3_QUADS] = reinterpret_cast<Bf16Quad(*)[TM][X3_QUADS]>(lds);
+    Bf16Quad(*Bx)[TN][X3_QUADS] = reinterpret_cast<Bf16Quad(*)[TN][X3_QUADS]>(
+        reinterpret_cast<Bf16Quad *>(lds) + 2 * TM * X3_QUADS);
     const int m0 = t.mt * TM, n0 = t.nt * TN;
     const int kbeg = t.ks * g.kchunk;
     const int kend = min(g.K, kbeg + g.kchunk);
@@ -293,25 +346,50 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
     int buf = 0;
     if (kbeg < kend) {
         gemm_fetch<TM, TN, T>(g, A, B, m0, n0, kbeg, kend, st);
-        gemm_stash<TM, TN, T>(g, st, As[0], Bs[0]);
+        if constexpr (X3)
+            gemm_stash_x3<TM, TN>(g, st, Ax[0], Bx[0]);
+        else
+            gemm_stash<TM, TN, T>(g, st, As[0], Bs[0]);
     }
     __syncthreads();
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
         const bool more = k0 + TK < kend;
         if (more) gemm_fetch<TM, TN, T>(g, A, B, m0, n0, k0 + TK, kend, st);
+        if constexpr (X3) {
+            // one 16x16x16 step covers the K tile: lane group lq holds k = 4 lq .. 4 lq + 3
+            Bf16Quad av[MI];
 #pragma unroll
-        for (int kk = 0; kk < TK; kk += 4) {
-            T av[MI];
-#pragma unroll
-            for (int i = 0; i < MI; ++i) av[i] = As[buf][kk + lq][wm + 16 * i + lr];
+            for (int i = 0; i < MI; ++i) av[i] = Ax[buf][wm + 16 * i + lr][lq];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
-                T bj = Bs[buf][kk + lq][16 * j + lr];
+                const Bf16Quad bj = Bx[buf][16 * j + lr][lq];
 #pragma unroll
-                for (int i = 0; i < MI; ++i) acc[i][j] = MF::run(bj, av[i], acc[i][j]);
+                for (int i = 0; i < MI; ++i) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bj.hi, av[i].lo, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bj.lo, av[i].hi, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bj.hi, av[i].hi, acc[i][j], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < TK; kk += 4) {
+                T av[MI];
+#pragma unroll
+                for (int i = 0; i < MI; ++i) av[i] = As[buf][kk + lq][wm + 16 * i + lr];
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    T bj = Bs[buf][kk + lq][16 * j + lr];
+#pragma unroll
+                    for (int i = 0; i < MI; ++i) acc[i][j] = MF::run(bj, av[i], acc[i][j]);
+                }
             }
         }
-        if (more) gemm_stash<TM, TN, T>(g, st, As[buf ^ 1], Bs[buf ^ 1]);
+        if (more) {
+            if constexpr (X3)
+                gemm_stash_x3<TM, TN>(g, st, Ax[buf ^ 1], Bx[buf ^ 1]);
+            else
+                gemm_stash<TM, TN, T>(g, st, As[buf ^ 1], Bs[buf ^ 1]);
+        }
         __syncthreads();
         buf ^= 1;
     }
@@ -355,35 +433,42 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
 }
 
 // Column tile widths are multiples of 16 up to 128.
-template <int TM, class T, class TO>
+template <int TM, class T, class TO, bool X3>
 __device__ __forceinline__ void gemm_tile_n(const GemmGroup &g, const GemmTile &t, const T *__restrict__ A,
                                             const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W,
                                             const RowMap *__restrict__ rowmap, double *lds) {
     switch (g.tn) {
-        case 16: gemm_tile<TM, 16, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
-        case 32: gemm_tile<TM, 32, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
-        case 48: gemm_tile<TM, 48, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
-        case 64: gemm_tile<TM, 64, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
-        case 80: gemm_tile<TM, 80, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
-        case 96: gemm_tile<TM, 96, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
-        case 112: gemm_tile<TM, 112, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
-        default: gemm_tile<TM, 128, T, TO>(g, t, A, B, C, W, rowmap, lds); break;
+        case 16: gemm_tile<TM, 16, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
+        case 32: gemm_tile<TM, 32, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
+        case 48: gemm_tile<TM, 48, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
+        case 64: gemm_tile<TM, 64, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
+        case 80: gemm_tile<TM, 80, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
+        case 96: gemm_tile<TM, 96, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
+        case 112: gemm_tile<TM, 112, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
+        default: gemm_tile<TM, 128, T, TO, X3>(g, t, A, B, C, W, rowmap, lds); break;
     }
 }
 
+// LDS bytes of a tile pair (two buffers): FP64/FP32 K-major tiles, or the
+// bf16 hi/lo groups of the BF16x3 form.
+template <class T, bool X3>
+constexpr int gemm_lds_bytes() {
+    return X3 ? 2 * 2 * TMAX * X3_QUADS * (int)sizeof(Bf16Quad) : 2 * kstep<T>() * (TMAX + 2) * 2 * (int)sizeof(T);
+}
+
 // One launch per stage: each tile runs the body of its group's tile shape.
-template <int STAGE, class T, class TO>
+template <int STAGE, class T, class TO, bool X3>
 __global__ void __launch_bounds__(256) k_gemm(const GemmGroup *__restrict__ groups, const GemmTile *__restrict__ tiles,
                                               const T *__restrict__ A, const T *__restrict__ B, TO *__restrict__ C,
                                               T *__restrict__ W, const RowMap *__restrict__ rowmap) {
-    __shared__ double lds[(2 * kstep<T>() * (TMAX + 2) * 2 * sizeof(T) + sizeof(double) - 1) / sizeof(double)];
+    __shared__ double lds[(gemm_lds_bytes<T, X3>() + 7) / 8];
     const GemmTile t = tiles[blockIdx.x];
     if (t.g < 0) return;  // padding tile of the XCD-ordered list
     const GemmGroup g = groups[t.g];
     if (g.tm == 128)
-        gemm_tile_n<128, T, TO>(g, t, A, B, C, W, rowmap, lds);
+        gemm_tile_n<128, T, TO, X3>(g, t, A, B, C, W, rowmap, lds);
     else
-        gemm_tile_n<64, T, TO>(g, t, A, B, C, W, rowmap, lds);
+        gemm_tile_n<64, T, TO, X3>(g, t, A, B, C, W, rowmap, lds);
 }
 
 // C = sum over splits of the partial tiles, in split order (deterministic),
@@ -473,23 +558,24 @@ hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, cons
     return wapply_launch<float>(strips, packs, npacks, srcs, T, C, stage, s);
 }
 
-template <int STAGE, class T, class TO>
+template <int STAGE, class T, class TO, bool X3>
 static void gemm_stage(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const T *A, const T *B, TO *C,
                        T *W, const RowMap *rowmap, hipStream_t s) {
     if (ntiles > 0)
-        hipLaunchKernelGGL((k_gemm<STAGE, T, TO>), dim3(ntiles), dim3(256), 0, s, groups, tiles, A, B, C, W, rowmap);
+        hipLaunchKernelGGL((k_gemm<STAGE, T, TO, X3>), dim3(ntiles), dim3(256), 0, s, groups, tiles, A, B, C, W,
+                           rowmap);
 }
 
-template <class T, class TO>
+template <class T, class TO, bool X3>
 static hipError_t gemm_launch(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
                               int nsplit, const T *A, const T *B, TO *C, T *W, const RowMap *rowmap, int stage,
                               hipStream_t s) {
     switch (stage) {
-        case ST_SIGMA: gemm_stage<ST_SIGMA, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
-        case ST_SIGMA_OUT: gemm_stage<ST_SIGMA_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
-        case ST_STEP: gemm_stage<ST_STEP, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
-        case ST_STEP_OUT: gemm_stage<ST_STEP_OUT, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
-        default: gemm_stage<ST_MISC, T, TO>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        case ST_SIGMA: gemm_stage<ST_SIGMA, T, TO, X3>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        case ST_SIGMA_OUT: gemm_stage<ST_SIGMA_OUT, T, TO, X3>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        case ST_STEP: gemm_stage<ST_STEP, T, TO, X3>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        case ST_STEP_OUT: gemm_stage<ST_STEP_OUT, T, TO, X3>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
+        default: gemm_stage<ST_MISC, T, TO, X3>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
     }
     if (nsplit > 0)
         hipLaunchKernelGGL((k_split_reduce<T, TO>), dim3(64, nsplit), dim3(256), 0, s, groups, split_gids, W, C);
@@ -499,17 +585,25 @@ static hipError_t gemm_launch(const GemmGroup *groups, const GemmTile *tiles, in
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
                        const double *A, const double *B, double *C, double *W, const RowMap *rowmap, int stage,
                        hipStream_t s) {
-    return gemm_launch<double, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage, s);
+    return gemm_launch<double, double, false>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage,
+                                              s);
 }
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
                        const float *A, const float *B, float *C, float *W, const RowMap *rowmap, int stage,
-                       hipStream_t s) {
-    return gemm_launch<float, float>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage, s);
+                       hipStream_t s, bool bf16x3) {
+    if (bf16x3)
+        return gemm_launch<float, float, true>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage,
+                                               s);
+    return gemm_launch<float, float, false>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage, s);
 }
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
                        const float *A, const float *B, double *C, float *W, const RowMap *rowmap, int stage,
-                       hipStream_t s) {
-    return gemm_launch<float, double>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage, s);
+                       hipStream_t s, bool bf16x3) {
+    if (bf16x3)
+        return gemm_launch<float, double, true>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage,
+                                                s);
+    return gemm_launch<float, double, false>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage,
+                                             s);
 }
 
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,

@@ -21,17 +21,18 @@ hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, cons
                          double *C, int stage, hipStream_t s);
 hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const float *T, float *C,
                          int stage, hipStream_t s);
-// grouped GEMM over a plan's tile list (FP64; FP32 operands with FP32 or FP64 output);
+// grouped GEMM over a plan's tile list (FP64; FP32 operands with FP32 or FP64 output,
+// multiplied in FP32 or, with bf16x3, as three bf16 products accumulated in FP32);
 // rowmap: the row placements of groups with rowmap > 0 (may be null when none has)
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
                        const double *A, const double *B, double *C, double *W, const RowMap *rowmap, int stage,
                        hipStream_t s);
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
                        const float *A, const float *B, float *C, float *W, const RowMap *rowmap, int stage,
-                       hipStream_t s);
+                       hipStream_t s, bool bf16x3 = false);
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
                        const float *A, const float *B, double *C, float *W, const RowMap *rowmap, int stage,
-                       hipStream_t s);
+                       hipStream_t s, bool bf16x3 = false);
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,
                            double *partial, hipStream_t s);
 // out = (x - f u) / (theta - D), elementwise, leaving the elements where theta == D

@@ -58,6 +58,8 @@ def lib():
         L.mq_plan_rebind.argtypes = [vp]
         L.mq_gemm_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
                                   C.c_int, C.c_int]
+        L.mq_gemm_f32.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
+                                  C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.mq_step_out_desc.argtypes = [vp, C.POINTER(mq_boundary_desc)]
         L.mq_boundary_step.argtypes = [vp, vp, vp, vp]
         L.mq_jacobi_davidson.argtypes = [vp, vp, vp, vp, vp, f64p, C.c_int, C.c_double, C.c_int,
@@ -235,6 +237,22 @@ class GpuEngine:
                                      max(ldb, 1), _ptr(Cm), max(M, 1), kchunk), "mq_gemm_f64")
         return Cm.t()
 
+    def gemm32(self, a, b, transA=False, transB=False, kchunk=2048, tm=64, tn=64, bf16x3=False):
+        """a @ b (device float32 tensors) with the FP32-operand kernel of the mixed
+        forms, tile tm x tn, in FP32 or BF16x3 products."""
+        t = self.torch
+        M, K = a.shape
+        N = b.shape[1]
+        A = a.contiguous() if transA else a.t().contiguous()
+        B = b.contiguous() if transB else b.t().contiguous()
+        Cm = t.empty((N, M), dtype=t.float32, device=self.device)
+        t.cuda.synchronize()
+        lda = K if transA else M
+        ldb = N if transB else K
+        self.check(lib().mq_gemm_f32(self.ctx, int(transA), int(transB), M, N, K, _ptr(A), max(lda, 1), _ptr(B),
+                                     max(ldb, 1), _ptr(Cm), max(M, 1), kchunk, tm, tn, int(bf16x3)), "mq_gemm_f32")
+        return Cm.t()
+
     def seed_boundary(self, charge):
         return GpuBoundary(BoundaryLayout.seed(charge), self.to_dev(np.ones(1)))
 
@@ -314,7 +332,7 @@ class GpuEngine:
         problems precondition the corrections with the diagonal of H_eff."""
         sp = self.sigma_plan(symm, psis[0], L, R, W)
         try:
-            X0 = self.torch.as_tensor(np.stack([p.to_flat() for p in psis])).to(self.device)
+            X0 = self.torch.as_tensor(independent_starts([p.to_flat() for p in psis])).to(self.device)
             diag = sp.diagonal() if symm == "su2u1" else None
             thetas, U, its = sp.block_davidson(X0, max_iter, tol, diag=diag)
             self.flops += its * len(psis) * sp.ref_flops
@@ -332,6 +350,31 @@ class GpuEngine:
             return d.cpu().numpy()
         finally:
             sp.close()
+
+
+def independent_starts(vecs, seed=0):
+    """Orthonormal start vectors for a block solve: the given vectors by
+    Gram-Schmidt, each one that is (numerically) in the span of those before it
+    replaced by a random vector.  The roots' two-site tensors at the start of a
+    state-averaged sweep can be dependent (all built on one shared neighbour)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in vecs:
+        v = np.array(v, dtype=np.float64)
+        for _ in range(8):
+            nv = np.linalg.norm(v)
+            w = v.copy()
+            for q in out:
+                w -= (q @ w) * q
+            for q in out:
+                w -= (q @ w) * q
+            if nv > 0 and np.linalg.norm(w) > 1e-8 * nv:
+                break
+            v = rng.standard_normal(v.size)
+        else:
+            raise MQError(f"block solve: cannot find {len(vecs)} independent start vectors of length {v.size}")
+        out.append(w / np.linalg.norm(w))
+    return np.stack(out)
 
 
 class SigmaPlan:

@@ -142,3 +142,14 @@ def test_sa_truncated_roots_are_upper_bounds(oracle_engine):
         assert max(r["truncated_weight"] for r in d.records[-4:]) > 0
         Es.append(E)
     assert np.all(Es[1] < Es[0])
+
+
+def test_independent_starts_replaces_dependent_vectors():
+    from qcmaquis_amd.gpu import MQError, independent_starts
+    rng = np.random.default_rng(0)
+    a, b = rng.standard_normal(30), rng.standard_normal(30)
+    X = independent_starts([a, b, 2 * a - b, np.zeros(30)])
+    assert np.abs(X @ X.T - np.eye(4)).max() < 1e-12
+    assert abs(abs(X[0] @ a) - np.linalg.norm(a)) < 1e-12
+    with pytest.raises(MQError):
+        independent_starts([rng.standard_normal(2) for _ in range(3)])
