@@ -284,7 +284,7 @@ def main():
     def summary(r):
         return dict(tflops=sp.ref_flops / r["dt"] / 1e12, ms_per_sigma=r["dt"] * 1e3,
                     ms_per_davidson_iter=r["ms_jd"], stages_ms=dict(zip(STAGES, r["avg"])),
-                    theta=r["theta"] + ham.ecore)
+                    stage_algorithmic_bytes=dict(zip(STAGES[2:], r["work"][3:6])), theta=r["theta"] + ham.ecore)
 
     out = {
         "metric": METRIC,

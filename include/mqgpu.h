@@ -227,8 +227,9 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
 /* With prec != MQ_PREC_FP64 the iterations run in that form and the returned
  * eigenvalue is the FP64 Rayleigh quotient of the Ritz vector.  With tol > 0
  * those iterations stop at tol or at the form's rounding floor, whichever is
- * larger (|r| <= 1e-6 |theta| for MQ_PREC_FP32, 2e-5 |theta| for
- * MQ_PREC_BF16X3), and when the FP64 residual of the pair misses tol, the
+ * larger (|r| <= 4e-6 |theta| for MQ_PREC_FP32, 2e-5 |theta| for
+ * MQ_PREC_BF16X3), or when the largest open residual has not halved for 8
+ * iterations; when the FP64 residual of the pair then misses tol, the
  * solve continues in FP64 from the Ritz vector (up to max_iter more
  * iterations, counted in *iterations).  The same holds for mq_davidson and
  * mq_block_davidson. */

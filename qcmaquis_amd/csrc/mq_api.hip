@@ -888,13 +888,33 @@ void sym_eig(int n, std::vector<double> A, std::vector<double> &w, std::vector<d
 
 // Convergence threshold of the iterations in form prec: with tol > 0 the mixed
 // forms stop at their rounding floor (the residual relative to |theta| that
-// their sigma vectors cannot resolve: FP32 about 1e-6, BF16x3 about 2e-5) when
+// their sigma vectors cannot resolve: FP32 about 4e-6, BF16x3 about 2e-5) when
 // that is above tol, and the FP64 continuation finishes the solve; tol <= 0
 // (run max_iter iterations) is kept as given.
 static double phase_tol(double tol, int prec) {
     if (tol <= 0 || prec == MQ_PREC_FP64) return tol;
-    return std::max(tol, prec == MQ_PREC_BF16X3 ? 2e-5 : 1e-6);
+    return std::max(tol, prec == MQ_PREC_BF16X3 ? 2e-5 : 4e-6);
 }
+
+// The mixed iterations also end when they stagnate above that floor (a root
+// whose rounding floor is a little higher): the largest open residual, relative
+// to |theta|, has not halved for kStallIters iterations.
+struct Stall {
+    static constexpr int kStallIters = 8;
+    bool on;
+    double ref = -1.0;
+    int since = 0;
+    explicit Stall(bool active) : on(active) {}
+    bool update(double rel) {
+        if (!on) return false;
+        if (ref < 0 || rel < 0.5 * ref) {
+            ref = rel;
+            since = 0;
+            return false;
+        }
+        return ++since >= kStallIters;
+    }
+};
 
 extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R, const double *x0, double *u_out,
                                   double *theta_out, int max_iter, double tol, int prec, const double *const *ortho,
@@ -911,6 +931,7 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
     }
     try {
         const double ptol = phase_tol(tol, prec);
+        Stall stall(prec != MQ_PREC_FP64 && tol > 0);
         JD jd;
         jd.p = plan;
         jd.s = plan->ctx->stream;
@@ -1017,7 +1038,9 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             // |r|, r.u and u.u in one batch: convergence test and correction t = -r + (r.u)/(u.u) u
             std::vector<double> rr = jd.dots({uA, uA, u}, {uA, u, u});
             double rn = std::sqrt(rr[0]);
-            if (rn <= ptol * std::fabs(theta) || rn < ptol || it >= max_iter) break;
+            if (rn <= ptol * std::fabs(theta) || rn < ptol || it >= max_iter ||
+                stall.update(rn / std::max(std::fabs(theta), 1e-300)))
+                break;
             jd.lincomb({uA, u}, {-1.0, rr[1] / rr[2]}, 0.0, V(it));
         }
         bool refine = false;
@@ -1081,6 +1104,7 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
     }
     try {
         const double ptol = phase_tol(tol, prec);
+        Stall stall(prec != MQ_PREC_FP64 && tol > 0);
         JD jd;
         jd.p = plan;
         jd.s = plan->ctx->stream;
@@ -1198,7 +1222,9 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
             ++it;
             std::vector<double> rr = jd.dots({r, vc, u}, {r, r, vc});
             const double rn = std::sqrt(rr[0]);
-            if (rn <= ptol * std::fabs(theta) || rn < ptol || it >= max_iter) break;
+            if (rn <= ptol * std::fabs(theta) || rn < ptol || it >= max_iter ||
+                stall.update(rn / std::max(std::fabs(theta), 1e-300)))
+                break;
             if (dim >= kMaxDim) {
                 // restart with the two lowest Ritz vectors and their images
                 std::vector<double> y1(E.begin() + dim, E.begin() + 2 * dim);
@@ -1276,6 +1302,7 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
     const int cap = 60;  // basis capacity
     try {
         const double ptol = phase_tol(tol, prec);
+        Stall stall(prec != MQ_PREC_FP64 && tol > 0);
         JD jd;
         jd.p = plan;
         jd.s = plan->ctx->stream;
@@ -1400,11 +1427,15 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
             ++it;
             std::vector<double> rr = jd.dots(xs, ys);
             std::vector<int> open;
+            double worst = 0.0;
             for (int r = 0; r < k; ++r) {
                 const double rn = std::sqrt(rr[3 * r]);
-                if (!(rn <= ptol * std::fabs(theta[r]) || rn < ptol)) open.push_back(r);
+                if (!(rn <= ptol * std::fabs(theta[r]) || rn < ptol)) {
+                    open.push_back(r);
+                    worst = std::max(worst, rn / std::max(std::fabs(theta[r]), 1e-300));
+                }
             }
-            if (open.empty() || it >= max_iter) break;
+            if (open.empty() || it >= max_iter || stall.update(worst)) break;
             if (dim + (int)open.size() > cap) {
                 // thick restart: the lowest min(dim, 2 nroots) Ritz vectors and their images
                 const int keep = std::min(dim, 2 * k);
