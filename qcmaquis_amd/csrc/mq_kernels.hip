@@ -49,16 +49,26 @@ constexpr int WU = 4;
 // large strip runs the strip with all threads; a pack of several small strips
 // runs their elements as one index range, each element finding its strip with a
 // binary search over the pack's element offsets in LDS.
+// A large strip gives each thread WS = 8 elements per pass, so that a wave has
+// 2 KB (FP32) or 4 KB (FP64) of loads in flight per source: the W application is
+// bound by the bytes in flight per wave more than by instructions (measured at
+// the L=24 centre: FP32 1.06 -> 1.00 ms from 4 to 8 elements, FP64 1.77 -> 1.72 ms;
+// 16 FP32 elements gained nothing more).
+template <class T>
+constexpr int strip_units() {
+    return 8;
+}
 template <class T>
 __device__ __forceinline__ void wapply_strip(const Strip &st, const Src *__restrict__ srcs, const T *__restrict__ src,
                                              T *__restrict__ C) {
+    constexpr int WS = strip_units<T>();
     const int rows = st.rows, n = st.rows * st.cols;
     const float inv = 1.0f / (float)max(rows, 1);
-    for (int base = 0; base < n; base += 256 * WU) {
-        int ii[WU], jj[WU];
-        T acc[WU];
+    for (int base = 0; base < n; base += 256 * WS) {
+        int ii[WS], jj[WS];
+        T acc[WS];
 #pragma unroll
-        for (int u = 0; u < WU; ++u) {
+        for (int u = 0; u < WS; ++u) {
             const int e = base + threadIdx.x + 256 * u;
             int j = (int)((float)e * inv);
             if (j * rows > e) --j;
@@ -70,14 +80,14 @@ __device__ __forceinline__ void wapply_strip(const Strip &st, const Src *__restr
         for (int s = st.s0; s < st.s1; ++s) {
             const Src sr = srcs[s];
             const T c = (T)sr.coef;
-            T v[WU];
+            T v[WS];
 #pragma unroll
-            for (int u = 0; u < WU; ++u) v[u] = jj[u] >= 0 ? src[sr.off + ii[u] + (int64_t)jj[u] * sr.ld] : (T)0;
+            for (int u = 0; u < WS; ++u) v[u] = jj[u] >= 0 ? src[sr.off + ii[u] + (int64_t)jj[u] * sr.ld] : (T)0;
 #pragma unroll
-            for (int u = 0; u < WU; ++u) acc[u] += c * v[u];
+            for (int u = 0; u < WS; ++u) acc[u] += c * v[u];
         }
 #pragma unroll
-        for (int u = 0; u < WU; ++u)
+        for (int u = 0; u < WS; ++u)
             if (jj[u] >= 0) C[st.dst + ii[u] + (int64_t)jj[u] * st.ldd] = acc[u];
     }
 }
