@@ -132,8 +132,9 @@ def main():
     ham, psi, Lb, Rb, W, site = sp_.ham, sp_.psi, sp_.L, sp_.R, sp_.W, sp_.site
     log(f"setup {time.perf_counter() - t0:.1f}s: site {site}, psi {psi.to_flat().size}, W {W.row_dim}x{W.col_dim}")
     # schedule per form (fused-column rule, split-K chunk): tuned on this problem
+    FORMS = (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32), ("bf16x3", MQ_PREC_BF16X3))
     tuned = {}
-    for name, prec in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)):
+    for name, prec in FORMS:
         if a.no_autotune:
             tuned[name] = (DEFAULT_SCHEDULE, [])
         else:
@@ -144,7 +145,7 @@ def main():
                 f"in {time.perf_counter() - t0:.1f}s")
     plans = {}
     t_plan = 0.0
-    for name in ("fp64", "mixed"):
+    for name, _ in FORMS:
         eng.set_schedule(tuned[name][0])
         t0 = time.perf_counter()
         plans[name] = eng.sigma_plan(ham.symm, psi, Lb, Rb, W)
@@ -155,8 +156,8 @@ def main():
     x = eng.to_dev(psi.to_flat())
     y = torch.empty_like(x)
 
-    def measure(prec):
-        sp = plans["fp64" if prec == MQ_PREC_FP64 else "mixed"]
+    def measure(name, prec):
+        sp = plans[name]
         torch.cuda.synchronize()
         for _ in range(a.warmup):
             sp.apply(x, y, prec)
@@ -182,10 +183,9 @@ def main():
         ms_jd = (time.perf_counter() - t0) / max(its, 1) * 1e3
         return dict(dt=dt, avg=avg, work=work, theta=theta, its=its, ms_jd=ms_jd, y=ycopy)
 
-    FORMS = (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32), ("bf16x3", MQ_PREC_BF16X3))
     res = {}
     for name, prec in FORMS:
-        res[name] = measure(prec)
+        res[name] = measure(name, prec)
         log(f"{name} stages ms: " + ", ".join(f"{n} {v:.3f}" for n, v in zip(STAGES, res[name]["avg"])))
     r64, r32 = res["fp64"], res["mixed"]
     sig_err = {k: float(np.abs(res[k]["y"] - r64["y"]).max() / np.abs(r64["y"]).max()) for k in ("mixed", "bf16x3")}
@@ -202,7 +202,7 @@ def main():
                 break
         conv = {}
         for name, prec in FORMS:
-            th, _, its = plans["fp64" if name == "fp64" else "mixed"].jd(u, 10, 1e-10, prec=prec)
+            th, _, its = plans[name].jd(u, 10, 1e-10, prec=prec)
             conv[name] = dict(theta=th + ham.ecore, iterations=its)
         dE = {k: abs(conv[k]["theta"] - conv["fp64"]["theta"]) for k in ("mixed", "bf16x3")}
         log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it); "
@@ -224,7 +224,7 @@ def main():
         block = {"roots": k}
         ref = None
         for name, prec in FORMS:
-            bp = plans["fp64" if name == "fp64" else "mixed"]
+            bp = plans[name]
             diag = bp.diagonal() if ham.symm == "su2u1" else None
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -312,7 +312,7 @@ def main():
         "ms_per_davidson_iter": ch["ms_jd"],
         "davidson_iters": ch["its"],
         "flops_per_sigma": sp.ref_flops,
-        "executed_flops_per_sigma": plans["fp64" if prec_name == "fp64" else "mixed"].exec_flops,
+        "executed_flops_per_sigma": plans[prec_name].exec_flops,
         "plan_build_s": t_plan,
         "schedule": {k: asdict(v[0]) for k, v in tuned.items()},
         "autotune": None if a.no_autotune else {k: table_json(v[1][:5]) for k, v in tuned.items()},
