@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--symm", default="su2u1", choices=["su2u1", "2u1"])
     ap.add_argument("--decay", type=float, default=None, help="integral decay length (exp(-|p-q|/decay))")
     ap.add_argument("--roots", type=int, default=4, help="block Davidson roots to time (0: skip)")
+    ap.add_argument("--forms", default="fp64,mixed,bf16x3",
+                    help="comma-separated forms to measure (fp64 is always included)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-delta-e", action="store_true", help="skip the converged-solve energy comparison")
@@ -132,7 +134,9 @@ def main():
     ham, psi, Lb, Rb, W, site = sp_.ham, sp_.psi, sp_.L, sp_.R, sp_.W, sp_.site
     log(f"setup {time.perf_counter() - t0:.1f}s: site {site}, psi {psi.to_flat().size}, W {W.row_dim}x{W.col_dim}")
     # schedule per form (fused-column rule, split-K chunk): tuned on this problem
-    FORMS = (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32), ("bf16x3", MQ_PREC_BF16X3))
+    want = set(a.forms.split(",")) | {"fp64"}
+    FORMS = tuple((n, p) for n, p in (("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32), ("bf16x3", MQ_PREC_BF16X3))
+                  if n in want)
     tuned = {}
     for name, prec in FORMS:
         if a.no_autotune:
@@ -187,8 +191,9 @@ def main():
     for name, prec in FORMS:
         res[name] = measure(name, prec)
         log(f"{name} stages ms: " + ", ".join(f"{n} {v:.3f}" for n, v in zip(STAGES, res[name]["avg"])))
-    r64, r32 = res["fp64"], res["mixed"]
-    sig_err = {k: float(np.abs(res[k]["y"] - r64["y"]).max() / np.abs(r64["y"]).max()) for k in ("mixed", "bf16x3")}
+    r64 = res["fp64"]
+    LOW = [n for n, _ in FORMS if n != "fp64"]
+    sig_err = {k: float(np.abs(res[k]["y"] - r64["y"]).max() / np.abs(r64["y"]).max()) for k in LOW}
     # energy agreement at convergence, the regime of the solves that fix a sweep's
     # final energy: the lowest eigenvector is first converged in FP64 (restarted
     # solves of 40 iterations), then both forms run one more solve from it
@@ -204,11 +209,11 @@ def main():
         for name, prec in FORMS:
             th, _, its = plans[name].jd(u, 10, 1e-10, prec=prec)
             conv[name] = dict(theta=th + ham.ecore, iterations=its)
-        dE = {k: abs(conv[k]["theta"] - conv["fp64"]["theta"]) for k in ("mixed", "bf16x3")}
+        dE = {k: abs(conv[k]["theta"] - conv["fp64"]["theta"]) for k in LOW}
         log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it); "
             + "; ".join(f"{k} {conv[k]['theta']:.12f} ({conv[k]['iterations']} it) dE {dE[k]:.2e}, sigma rel err "
                         f"{sig_err[k]:.2e}" for k in dE))
-    ok = ["fp64"] + [k for k in ("mixed", "bf16x3") if dE is not None and dE[k] <= 1e-8]
+    ok = ["fp64"] + [k for k in LOW if dE is not None and dE[k] <= 1e-8]
     prec_name = min(ok, key=lambda k: res[k]["dt"])
     ch = res[prec_name]
     dt, avg, work = ch["dt"], ch["avg"], ch["work"]
@@ -226,6 +231,8 @@ def main():
         for name, prec in FORMS:
             bp = plans[name]
             diag = bp.diagonal() if ham.symm == "su2u1" else None
+            bp.block_davidson(X0, 1, 0.0, prec=prec, diag=diag)  # builds the plan's batched task lists
+            eng.sync()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             _, _, its = bp.block_davidson(X0, a.jd_iters, 0.0, prec=prec, diag=diag)
@@ -243,7 +250,7 @@ def main():
                                max_delta_E_vs_fp64=max(abs(x - y) for x, y in zip(th, ref)))
             log(f"block Davidson {k} roots {name}: {ms_it:.2f} ms/iteration; converged in {itc} iterations "
                 f"({t_conv:.2f} s), max dE vs fp64 {block[name]['max_delta_E_vs_fp64']:.2e}")
-        for name in ("mixed", "bf16x3"):
+        for name in LOW:
             block[name]["speedup_vs_fp64"] = block["fp64"]["ms_per_block_iter"] / block[name]["ms_per_block_iter"]
 
     # roofline of the dominant kernel of the reported form: the binding roof is the
@@ -317,15 +324,11 @@ def main():
         "schedule": {k: asdict(v[0]) for k, v in tuned.items()},
         "autotune": None if a.no_autotune else {k: table_json(v[1][:5]) for k, v in tuned.items()},
         "fp64": summary(r64),
-        "mixed": summary(r32),
-        "bf16x3": summary(res["bf16x3"]),
-        "mixed_speedup_vs_fp64": r64["dt"] / r32["dt"],
-        "bf16x3_speedup_vs_fp64": r64["dt"] / res["bf16x3"]["dt"],
-        "delta_E_mixed_vs_fp64_Eh": None if dE is None else dE["mixed"],
-        "delta_E_bf16x3_vs_fp64_Eh": None if dE is None else dE["bf16x3"],
+        **{k: summary(res[k]) for k in LOW},
+        **{f"{k}_speedup_vs_fp64": r64["dt"] / res[k]["dt"] for k in LOW},
+        **{f"delta_E_{k}_vs_fp64_Eh": None if dE is None else dE[k] for k in LOW},
         "converged_solves": conv,
-        "mixed_sigma_max_rel_err": sig_err["mixed"],
-        "bf16x3_sigma_max_rel_err": sig_err["bf16x3"],
+        **{f"{k}_sigma_max_rel_err": sig_err[k] for k in LOW},
         "block_davidson": block,
         "parity_max_rel_err_vs_oracle": err,
         "roofline": roof,
