@@ -154,9 +154,11 @@ const char *mq_last_error(const mq_ctx *ctx);
  * (1..4095, default 64).  MQ_OPT_KCHUNK: target K chunk of split-K GEMM
  * groups (256..65536, a multiple of 16, default 2048).  Results agree to
  * round-off.  MQ_OPT_GRAPH (0 default / 1): the Davidson drivers run their
- * sigma vectors through mq_sigma_graph's captured graph (same results). */
+ * sigma vectors through mq_sigma_graph's captured graph (same results).
+ * MQ_OPT_TILE_M: row tile of every GEMM group, 64 or 128, or 0 (default:
+ * 128 rows unless the group's M fits 64). */
 enum mq_option { MQ_OPT_FUSE = 0, MQ_OPT_FUSE_RATIO = 1, MQ_OPT_FUSE_MIN = 2, MQ_OPT_KCHUNK = 3,
-                 MQ_OPT_GRAPH = 4 };
+                 MQ_OPT_GRAPH = 4, MQ_OPT_TILE_M = 5 };
 int mq_ctx_set_option(mq_ctx *ctx, int option, int value);
 /* HIP stream (hipStream_t) the context launches on. */
 void *mq_ctx_stream(mq_ctx *ctx);

@@ -1,11 +1,13 @@
 """Schedule autotuner for the sigma-vector plans.
 
-A plan's schedule has two choices that the task lists depend on but the
+A plan's schedule has three choices that the task lists depend on but the
 results do not (they agree to round-off):
   - the fused-column rule: which MPO columns are summed on the left boundary
     first (mode 0 none, 1 by an entries/positions ratio and a minimum entry
     count, 2 all), trading W-application traffic for GEMM flops;
-  - the split-K chunk of the output GEMM groups.
+  - the split-K chunk of the output GEMM groups;
+  - the GEMM row tile (0: 128 rows unless a group fits 64; or 64 for every group,
+    twice the tiles with half the accumulators each).
 The tuner builds the plan of one site problem under each candidate schedule,
 times a few sigma vectors in the requested precision with HIP events on the
 library stream, and keeps the fastest.  Results are cached per site shape
@@ -25,11 +27,13 @@ class Schedule:
     ratio: int = 4
     min_entries: int = 64
     kchunk: int = 2048
+    tile_m: int = 0
 
     def label(self):
+        tm = f" tile_m={self.tile_m}" if self.tile_m else ""
         if self.fuse != 1:
-            return f"fuse={self.fuse} kchunk={self.kchunk}"
-        return f"fuse=1 ratio={self.ratio} min={self.min_entries} kchunk={self.kchunk}"
+            return f"fuse={self.fuse} kchunk={self.kchunk}{tm}"
+        return f"fuse=1 ratio={self.ratio} min={self.min_entries} kchunk={self.kchunk}{tm}"
 
 
 DEFAULT = Schedule()
@@ -37,7 +41,8 @@ DEFAULT = Schedule()
 
 def candidates():
     """Fused-column rules around the default (ratio x minimum entries), the two
-    extremes, and split-K chunks of 1024 / 2048 / 4096 for the default rule."""
+    extremes, split-K chunks of 1024 / 2048 / 4096 and the 64-row tile for the
+    default rule."""
     out = [DEFAULT, Schedule(fuse=0), Schedule(fuse=2)]
     for ratio in (2, 3, 6, 8):
         for mn in (1, 64):
@@ -46,6 +51,7 @@ def candidates():
     out.append(Schedule(1, 4, 16))
     for kc in (1024, 4096):
         out.append(Schedule(kchunk=kc))
+    out.append(Schedule(tile_m=64))
     return out
 
 

@@ -17,6 +17,7 @@ struct mq_ctx {
     std::string err;
     FuseRule fuse;  // MQ_OPT_FUSE, MQ_OPT_FUSE_RATIO, MQ_OPT_FUSE_MIN
     int kchunk = 2048;  // MQ_OPT_KCHUNK
+    int tile_m = 0;     // MQ_OPT_TILE_M (0: automatic)
     int graph = 0;      // MQ_OPT_GRAPH
 };
 
@@ -194,6 +195,10 @@ int mq_ctx_set_option(mq_ctx *ctx, int option, int value) {
         ctx->kchunk = value;
         return MQ_OK;
     }
+    if (option == MQ_OPT_TILE_M && (value == 0 || value == 64 || value == 128)) {
+        ctx->tile_m = value;
+        return MQ_OK;
+    }
     if (option == MQ_OPT_GRAPH && (value == 0 || value == 1)) {
         ctx->graph = value;
         return MQ_OK;
@@ -220,7 +225,7 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         check(hipSetDevice(ctx->device), "hipSetDevice");
         std::unique_ptr<mq_plan> p(new mq_plan());
         p->ctx = ctx;
-        const int tm = 64, tn = 64, kc = ctx->kchunk;
+        const int tm = ctx->tile_m, tn = 0, kc = ctx->kchunk;
         if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse);
         else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse);
         else if (kind == PLAN_RSTEP) p->hp = build_rstep_plan(*desc, tm, tn, kc);
@@ -631,7 +636,7 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const 
         g.transA = transA ? 1 : 0;
         g.transB = transB ? 1 : 0;
         GemmList gl;
-        tile_gemm(gl, g, 64, 64, kchunk);
+        tile_gemm(gl, g, 0, 0, kchunk);
         order_tiles(gl);
         DevGemm dg;
         dg.upload(gl, ctx->stream);
@@ -1542,7 +1547,7 @@ int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out) 
     if (!desc || !out) return MQ_ERR_ARG;
     try {
         std::unique_ptr<mq_host_plan> h(new mq_host_plan());
-        const int tm = 64, tn = 64, kc = 2048;
+        const int tm = 0, tn = 0, kc = 2048;
         const int fm = (kind >> 8) & 3, fr = (kind >> 10) & 255, fmin = (kind >> 18) & 4095, k = kind & 255;
         FuseRule fuse;
         if (fm) fuse.mode = fm - 1;

@@ -161,14 +161,15 @@ int column_tile(int N) {
 }
 
 void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
-    (void)tn;
     if (g.rowmap < 0 || (g.rowmap > 0 && g.rowmap - 1 + g.M > (int64_t)gl.rowmap.size()))
         throw std::runtime_error("GEMM row map out of range");
+    if ((tm != 0 && tm != 64 && tm != 128) || (tn != 0 && (tn < 16 || tn > 128 || tn % 16)))
+        throw std::runtime_error("GEMM tile shape out of range");
     g.split = 1;
     g.kchunk = std::max(g.K, 1);
-    g.tm = g.M > 64 ? 128 : 64;
-    g.tn = column_tile(g.N);
-    (void)tm;
+    // 0: automatic (128 rows unless M fits 64; the column tile of column_tile)
+    g.tm = tm ? tm : (g.M > 64 ? 128 : 64);
+    g.tn = tn ? tn : column_tile(g.N);
     if (g.M == 0 || g.N == 0) return;
     int mt = ceil_div(g.M, g.tm), nt = ceil_div(g.N, g.tn);
     // split K when the output is small and K long: chunks of about kchunk_target

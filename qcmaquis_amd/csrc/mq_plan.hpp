@@ -131,7 +131,10 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kch
 // combine LA columns into T, and g2 writes the diagonal = T * RB per row slice.
 HostPlan build_diag_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
 
-// One GEMM group split into tiles (and K chunks) as the plans do.
+// One GEMM group split into tiles (and K chunks) as the plans do.  tile_m: 64,
+// 128 or 0 (128 rows unless M fits 64); tile_n: a multiple of 16 up to 128, or 0
+// (the padding-minimising column tile).  The plan builders take the same tile_m
+// and tile_n for every group they add.
 void tile_gemm(GemmList &gl, GemmGroup g, int tile_m, int tile_n, int kchunk_target);
 // XCD-aware tile order (call once every group has been added).
 void order_tiles(GemmList &gl);

@@ -191,9 +191,10 @@ class GpuEngine:
 
     def set_schedule(self, rule):
         """Plan-construction schedule of the plans built afterwards: a
-        Schedule (fused-column rule and split-K chunk)."""
+        Schedule (fused-column rule, split-K chunk, GEMM row tile)."""
         self.set_fuse(rule.fuse, rule.ratio, rule.min_entries)
         self.check(lib().mq_ctx_set_option(self.ctx, 3, rule.kchunk), "mq_ctx_set_option")
+        self.check(lib().mq_ctx_set_option(self.ctx, 5, rule.tile_m), "mq_ctx_set_option")
         self.schedule = rule
 
     def close(self):

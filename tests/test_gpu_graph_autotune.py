@@ -82,6 +82,31 @@ def test_autotune_picks_a_schedule_with_the_same_sigma(fixtures, oracle_engine, 
     assert np.abs(ys[0] - ys[1]).max() <= 1e-13 * np.abs(ys[0]).max()
 
 
+def test_every_candidate_schedule_gives_the_same_sigma(fixtures, oracle_engine, gpu_engine):
+    """Each candidate schedule (fused-column rules, split-K chunks, the 64-row
+    tile) in FP64, FP32 and BF16x3 against the default schedule's FP64 sigma."""
+    from qcmaquis_amd.autotune import DEFAULT, candidates
+    from qcmaquis_amd.desc import MQ_PREC_BF16X3, MQ_PREC_FP32, MQ_PREC_FP64
+    ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
+    x = gpu_engine.to_dev(np.random.default_rng(4).standard_normal(psi.to_flat().size))
+    ref = None
+    try:
+        for sched in [DEFAULT] + candidates():
+            gpu_engine.set_schedule(sched)
+            sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+            try:
+                for prec, bar in ((MQ_PREC_FP64, 1e-13), (MQ_PREC_FP32, 1e-5), (MQ_PREC_BF16X3, 1e-4)):
+                    y = sp.apply(x, prec=prec).cpu().numpy()
+                    if ref is None:
+                        ref = y
+                    err = np.abs(y - ref).max() / np.abs(ref).max()
+                    assert err <= bar, (sched.label(), prec, err)
+            finally:
+                sp.close()
+    finally:
+        gpu_engine.set_schedule(DEFAULT)
+
+
 @pytest.mark.parametrize("nvec", [1, 3, 4])
 def test_sigma_batch_equals_single_vectors(fixtures, oracle_engine, gpu_engine, nvec):
     """mq_sigma_batch: each output row equals mq_sigma of its input row (the
