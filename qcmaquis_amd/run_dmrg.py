@@ -1,37 +1,42 @@
 """Two-site DMRG sweeps on one GPU.
 
     python -m qcmaquis_amd.run_dmrg --L 24 --ramp 64,128,256,512,512,512 \
-        [--fcidump FILE] [--prec fp64|mixed|both|compare] [--jd-maxiter 8] [--out result.json]
+        [--fcidump FILE] [--prec fp64|mixed|bf16x3|both|all|compare] [--jd-maxiter 8] [--out result.json]
 
 Without --fcidump the synthetic L-orbital half-filled singlet Hamiltonian
 (qcmaquis_amd.integrals.synthetic, seed 1000 + L) is used.  The sweep order,
 truncation and Davidson semantics follow the reference's two-site optimizer;
 each local problem is solved by the GPU Jacobi-Davidson (mixed: FP32 sigma
-vectors with one FP64 defect-correction sigma per solve).  With --prec both
-the two forms run from the same initial MPS and the energy differences per
-sweep are reported.  With --prec compare the sweep runs in FP64 and every
-local problem is solved a second time, in the mixed form, on the same plan
-from the same start vector: the largest |theta_mixed - theta_fp64| per sweep
-measures the mixed form's energy error on the sweep's own trajectory.
+vectors; bf16x3: the same with BF16x3 products; both with FP64 Rayleigh
+quotients and an FP64 continuation when the FP64 residual misses the
+tolerance).  With --prec both (fp64 and mixed) or all (fp64, mixed and bf16x3)
+the forms run from the same initial MPS and the energy differences per sweep
+are reported.  With --prec compare the sweep runs in FP64 and every local
+problem is solved again in the mixed and BF16x3 forms, on the same plan from the
+same start vector: the largest |theta_form - theta_fp64| per sweep measures
+each form's energy error on the sweep's own trajectory.
 """
 import argparse
 import json
 import sys
 import time
 
-from .desc import MQ_PREC_FP32, MQ_PREC_FP64
+from .desc import MQ_PREC_BF16X3, MQ_PREC_FP32, MQ_PREC_FP64
 from .integrals import read_fcidump, synthetic
 from .sweep import Hamiltonian, TwoSiteDMRG
 
 
+LOW_FORMS = (("mixed", MQ_PREC_FP32), ("bf16x3", MQ_PREC_BF16X3))
+
+
 class _CompareEngine:
     """GPU engine whose local solves run in FP64 (driving the sweep) and, on the
-    same plan and start vector, in the mixed form; records |theta_mixed -
-    theta_fp64| per solve."""
+    same plan and start vector, in the mixed and BF16x3 forms; records
+    |theta_form - theta_fp64| per solve."""
 
     def __init__(self, eng):
         self.eng = eng
-        self.diffs = []
+        self.diffs = {n: [] for n, _ in LOW_FORMS}
 
     def __getattr__(self, k):
         return getattr(self.eng, k)
@@ -41,15 +46,16 @@ class _CompareEngine:
         try:
             x = self.eng.to_dev(psi.to_flat())
             t64, u, its = sp.jd(x, max_iter, tol, ortho, prec=MQ_PREC_FP64)
-            t32, _, _ = sp.jd(x, max_iter, tol, ortho, prec=MQ_PREC_FP32)
-            self.diffs.append(abs(t32 - t64))
+            for name, prec in LOW_FORMS:
+                t, _, _ = sp.jd(x, max_iter, tol, ortho, prec=prec)
+                self.diffs[name].append(abs(t - t64))
             return t64, u.cpu().numpy(), its
         finally:
             sp.close()
 
 
 def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print):
-    """FP64 sweeps; every local problem is also solved in the mixed form."""
+    """FP64 sweeps; every local problem is also solved in the mixed and BF16x3 forms."""
     from .gpu import GpuEngine
     eng = _CompareEngine(GpuEngine(0, prec=MQ_PREC_FP64))
     try:
@@ -58,15 +64,22 @@ def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print):
         t0 = time.perf_counter()
         for s, m in enumerate(ramp):
             d.m = m
-            eng.diffs = []
+            eng.diffs = {n: [] for n, _ in LOW_FORMS}
             E = d.sweep(s)
-            out.append(dict(sweep=s, m=m, energy=E, max_abs_dtheta=max(eng.diffs), mean_abs_dtheta=sum(eng.diffs) /
-                            len(eng.diffs), solves=len(eng.diffs), seconds=time.perf_counter() - t0))
-            log(f"  sweep {s} m {m} E(fp64) {E:.12f} max |theta_mixed - theta_fp64| {max(eng.diffs):.3e} Eh "
-                f"over {len(eng.diffs)} solves, t {time.perf_counter() - t0:.1f}s")
+            rec = dict(sweep=s, m=m, energy=E, solves=len(eng.diffs["mixed"]), seconds=time.perf_counter() - t0)
+            for n, _ in LOW_FORMS:
+                rec[f"max_abs_dtheta_{n}"] = max(eng.diffs[n])
+                rec[f"mean_abs_dtheta_{n}"] = sum(eng.diffs[n]) / len(eng.diffs[n])
+            out.append(rec)
+            log(f"  sweep {s} m {m} E(fp64) {E:.12f} max |theta - theta_fp64|: "
+                + ", ".join(f"{n} {rec[f'max_abs_dtheta_{n}']:.3e}" for n, _ in LOW_FORMS)
+                + f" Eh over {rec['solves']} solves, t {time.perf_counter() - t0:.1f}s")
         return out
     finally:
         eng.eng.close()
+
+
+NAMES = {MQ_PREC_FP64: "fp64", MQ_PREC_FP32: "mixed", MQ_PREC_BF16X3: "bf16x3"}
 
 
 def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd"):
@@ -82,7 +95,7 @@ def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd"):
             recs = [r for r in d.records if r["sweep"] == s]
             out.append(dict(sweep=s, m=m, energy=E, max_bond=max(r["bond"] for r in recs),
                             jd_seconds=sum(r["t_jd"] for r in recs), seconds=time.perf_counter() - t0))
-            log(f"  {'fp64' if prec == MQ_PREC_FP64 else 'mixed'} sweep {s} m {m} E {E:.12f} "
+            log(f"  {NAMES[prec]} sweep {s} m {m} E {E:.12f} "
                 f"t {time.perf_counter() - t0:.1f}s")
         return out
     finally:
@@ -94,7 +107,7 @@ def main(argv=None):
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--fcidump")
     ap.add_argument("--ramp", default="64,64,64,64")
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "both", "compare"])
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "bf16x3", "both", "all", "compare"])
     ap.add_argument("--jd-maxiter", type=int, default=8)
     ap.add_argument("--jd-tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=42)
@@ -116,13 +129,16 @@ def main(argv=None):
                 json.dump(rec, f, indent=1)
         return rec
     precs = {"fp64": [("fp64", MQ_PREC_FP64)], "mixed": [("mixed", MQ_PREC_FP32)],
-             "both": [("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)]}[a.prec]
+             "bf16x3": [("bf16x3", MQ_PREC_BF16X3)],
+             "both": [("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)],
+             "all": [("fp64", MQ_PREC_FP64)] + list(LOW_FORMS)}[a.prec]
     res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver) for name, p in precs}
     rec = dict(L=ham.L, ramp=ramp, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, eigensolver=a.eigensolver, results=res)
-    if "fp64" in res and "mixed" in res:
-        rec["delta_E"] = [m["energy"] - f["energy"] for f, m in zip(res["fp64"], res["mixed"])]
-        for s, dE in enumerate(rec["delta_E"]):
-            print(f"sweep {s}: E(mixed) - E(fp64) = {dE:.3e} Eh")
+    for name, _ in LOW_FORMS:
+        if "fp64" in res and name in res:
+            rec[f"delta_E_{name}"] = [m["energy"] - f["energy"] for f, m in zip(res["fp64"], res[name])]
+            for s, dE in enumerate(rec[f"delta_E_{name}"]):
+                print(f"sweep {s}: E({name}) - E(fp64) = {dE:.3e} Eh")
     if a.out:
         with open(a.out, "w") as f:
             json.dump(rec, f, indent=1)
