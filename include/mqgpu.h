@@ -156,9 +156,12 @@ const char *mq_last_error(const mq_ctx *ctx);
  * round-off.  MQ_OPT_GRAPH (0 default / 1): the Davidson drivers run their
  * sigma vectors through mq_sigma_graph's captured graph (same results).
  * MQ_OPT_TILE_M: row tile of every GEMM group, 64 or 128, or 0 (default:
- * 128 rows unless the group's M fits 64). */
+ * 128 rows unless the group's M fits 64).  MQ_OPT_REF_FLOPS (1 default / 0):
+ * count the reference schedule's FLOPs for mq_plan_flops; with 0 sigma and
+ * left-step plans skip that count (it walks the reference schedule once more)
+ * and mq_plan_flops reports -1. */
 enum mq_option { MQ_OPT_FUSE = 0, MQ_OPT_FUSE_RATIO = 1, MQ_OPT_FUSE_MIN = 2, MQ_OPT_KCHUNK = 3,
-                 MQ_OPT_GRAPH = 4, MQ_OPT_TILE_M = 5 };
+                 MQ_OPT_GRAPH = 4, MQ_OPT_TILE_M = 5, MQ_OPT_REF_FLOPS = 6 };
 int mq_ctx_set_option(mq_ctx *ctx, int option, int value);
 /* HIP stream (hipStream_t) the context launches on. */
 void *mq_ctx_stream(mq_ctx *ctx);

@@ -18,6 +18,7 @@ struct mq_ctx {
     FuseRule fuse;  // MQ_OPT_FUSE, MQ_OPT_FUSE_RATIO, MQ_OPT_FUSE_MIN
     int kchunk = 2048;  // MQ_OPT_KCHUNK
     int tile_m = 0;     // MQ_OPT_TILE_M (0: automatic)
+    int ref_flops = 1;  // MQ_OPT_REF_FLOPS
     int graph = 0;      // MQ_OPT_GRAPH
 };
 
@@ -195,6 +196,10 @@ int mq_ctx_set_option(mq_ctx *ctx, int option, int value) {
         ctx->kchunk = value;
         return MQ_OK;
     }
+    if (option == MQ_OPT_REF_FLOPS && (value == 0 || value == 1)) {
+        ctx->ref_flops = value;
+        return MQ_OK;
+    }
     if (option == MQ_OPT_TILE_M && (value == 0 || value == 64 || value == 128)) {
         ctx->tile_m = value;
         return MQ_OK;
@@ -226,8 +231,8 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         std::unique_ptr<mq_plan> p(new mq_plan());
         p->ctx = ctx;
         const int tm = ctx->tile_m, tn = 0, kc = ctx->kchunk;
-        if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse);
-        else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse);
+        if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse, ctx->ref_flops != 0);
+        else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse, ctx->ref_flops != 0);
         else if (kind == PLAN_RSTEP) p->hp = build_rstep_plan(*desc, tm, tn, kc);
         else p->hp = build_diag_plan(*desc, tm, tn, kc);
         hipStream_t s = ctx->stream;

@@ -948,7 +948,7 @@ GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step
 }
 
 // ============================================================ sigma
-HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse) {
+HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse, bool ref_count) {
     Site s(d);
     BLayout L = to_layout(d.L), R = to_layout(d.R);
     if (L.aux != s.W.rows || R.aux != s.W.cols) throw std::runtime_error("boundary aux_dim does not match the MPO");
@@ -967,7 +967,8 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     std::vector<int> strip_al;
     std::vector<Raw> raw;
     // reference schedule (site_hamil.hpp:35): when it is lbtm its FLOPs are counted in this pass
-    const bool ref_lbtm = s.symm != MQ_SYMM_SU2U1 || (s.W.rows - s.W.one_rows) < (s.W.cols - s.W.one_cols);
+    const bool ref_lbtm =
+        ref_count && (s.symm != MQ_SYMM_SU2U1 || (s.W.rows - s.W.one_rows) < (s.W.cols - s.W.one_cols));
     TLay tall;
     if (ref_lbtm) tall = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, tm, tn, kc);
     double ref_axpy = 0, ref_trim = 0;
@@ -1023,7 +1024,9 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     }
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
-    if (ref_lbtm) {
+    if (!ref_count) {
+        P.ref_flops = -1;
+    } else if (ref_lbtm) {
         P.ref_flops = tall.gemm_flops + ref_axpy + ref_trim;
     } else {
         P.ref_flops = reference_sigma_flops(d);
@@ -1037,7 +1040,7 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
 }
 
 // ============================================================ left step
-HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse) {
+HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse, bool ref_count) {
     Site s(d);
     BLayout L = to_layout(d.L);
     if (L.aux != s.W.rows) throw std::runtime_error("left boundary aux_dim does not match the MPO");
@@ -1049,7 +1052,8 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     P.n_prp = prp.total;
     ColProg cp = column_program(s, true, fuse);
     TLay t = t_layout(s, L, prp, &cp, &P.arrL, &P.combL, &P.g1, tm, tn, kc);
-    TLay tall = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, tm, tn, kc);
+    TLay tall;
+    if (ref_count) tall = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, tm, tn, kc);
     P.n_la = t.la_total;
     P.n_t = t.t_total;
     double axf = 0;
@@ -1060,7 +1064,7 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     for (int b2 = 0; b2 < s.W.cols; ++b2) {
         if (s.W.rskip(b2)) continue;
         lbtm_column(s, t, &cp, cp.col[b2], b2, raw);
-        axf += reference_axpy_flops(s, tall, b2);
+        if (ref_count) axf += reference_axpy_flops(s, tall, b2);
         size_t i = 0;
         while (i < raw.size()) {
             size_t j = i;
@@ -1138,7 +1142,7 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
     for (auto &st : P.combL.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
-    P.ref_flops = tall.gemm_flops + axf + gf;
+    P.ref_flops = ref_count ? tall.gemm_flops + axf + gf : -1;
     P.min_bytes = 8.0 * (L.total + off + s.X.total);
     order_tiles(P.g1);
     order_tiles(P.g2);

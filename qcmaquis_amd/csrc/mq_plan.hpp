@@ -123,8 +123,13 @@ struct FuseRule {
     int ratio = 4;
     int min_entries = 64;
 };
-HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {});
-HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {});
+// ref_count: also count the reference schedule's FLOPs (ref_flops; -1 when not
+// counted).  The count walks the reference schedule once more and is only needed
+// for measurement.
+HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {},
+                          bool ref_count = true);
+HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {},
+                          bool ref_count = true);
 HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);
 // Diagonal of H_eff (SU2 only): arrL extracts the diagonals of the L[b1](lc,lc)
 // blocks into LA, arrR those of R[b2](rc,rc) into RB, the W-application strips

@@ -185,6 +185,12 @@ class GpuEngine:
     def leave(self):
         self.torch.cuda.current_stream(self.device).wait_stream(self.stream)
 
+    def set_ref_flops(self, on):
+        """Count the reference schedule's FLOPs in the plans built afterwards
+        (MQ_OPT_REF_FLOPS); without the count a plan's ref_flops is -1 and the
+        engine's running total uses the executed FLOPs."""
+        self.check(lib().mq_ctx_set_option(self.ctx, 6, 1 if on else 0), "mq_ctx_set_option")
+
     def set_graph(self, on):
         """Davidson drivers run their sigma vectors through captured HIP graphs (MQ_OPT_GRAPH)."""
         self.check(lib().mq_ctx_set_option(self.ctx, 4, 1 if on else 0), "mq_ctx_set_option")
@@ -282,7 +288,7 @@ class GpuEngine:
             self.sync()
             fl = C.c_double()
             lib().mq_plan_flops(p, C.byref(fl), None)
-            self.flops += fl.value
+            self.flops += fl.value if fl.value >= 0 else lib().mq_plan_exec_flops(p)
         finally:
             lib().mq_plan_destroy(p)
         return GpuBoundary(lay, out)
@@ -312,7 +318,7 @@ class GpuEngine:
         sp = self.sigma_plan(symm, psi, L, R, W)
         try:
             theta, u, its = sp.jd(self.to_dev(psi.to_flat() if x0 is None else x0), max_iter, tol, ortho)
-            self.flops += its * sp.ref_flops
+            self.flops += its * sp.work_flops
             return theta, u.cpu().numpy(), its
         finally:
             sp.close()
@@ -322,7 +328,7 @@ class GpuEngine:
         sp = self.sigma_plan(symm, psi, L, R, W)
         try:
             theta, u, its = sp.davidson(self.to_dev(psi.to_flat() if x0 is None else x0), max_iter, tol, ortho)
-            self.flops += its * sp.ref_flops
+            self.flops += its * sp.work_flops
             return theta, u.cpu().numpy(), its
         finally:
             sp.close()
@@ -336,7 +342,7 @@ class GpuEngine:
             X0 = self.torch.as_tensor(independent_starts([p.to_flat() for p in psis])).to(self.device)
             diag = sp.diagonal() if symm == "su2u1" else None
             thetas, U, its = sp.block_davidson(X0, max_iter, tol, diag=diag)
-            self.flops += its * len(psis) * sp.ref_flops
+            self.flops += its * len(psis) * sp.work_flops
             Uh = U.cpu().numpy()
             return thetas, [Uh[r] for r in range(len(psis))], its
         finally:
@@ -389,6 +395,9 @@ class SigmaPlan:
         self.ref_flops = fl.value
         self.min_bytes = by.value
         self.exec_flops = lib().mq_plan_exec_flops(self.p)
+        # FLOPs per sigma for running totals: the reference count, or the executed one
+        # when the plan was built without it (MQ_OPT_REF_FLOPS = 0)
+        self.work_flops = self.ref_flops if self.ref_flops >= 0 else self.exec_flops
         self.n = lib().mq_plan_vec_len(self.p)
         self.workspace_bytes = lib().mq_plan_workspace_bytes(self.p)
 

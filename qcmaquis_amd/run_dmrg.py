@@ -58,6 +58,7 @@ def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print):
     """FP64 sweeps; every local problem is also solved in the mixed and BF16x3 forms."""
     from .gpu import GpuEngine
     eng = _CompareEngine(GpuEngine(0, prec=MQ_PREC_FP64))
+    eng.set_ref_flops(False)
     try:
         d = TwoSiteDMRG(ham, eng, ramp[0], jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed)
         out = []
@@ -85,6 +86,7 @@ NAMES = {MQ_PREC_FP64: "fp64", MQ_PREC_FP32: "mixed", MQ_PREC_BF16X3: "bf16x3"}
 def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd"):
     from .gpu import GpuEngine
     eng = GpuEngine(0, prec=prec)
+    eng.set_ref_flops(False)  # the sweep needs no reference FLOP counts
     try:
         d = TwoSiteDMRG(ham, eng, ramp[0], jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed, eigensolver=eigensolver)
         out = []

@@ -124,3 +124,23 @@ def test_sigma_batch_equals_single_vectors(fixtures, oracle_engine, gpu_engine, 
                 assert np.array_equal(Y[j], y), (prec, j, np.abs(Y[j] - y).max())
     finally:
         sp.close()
+
+
+def test_plans_without_the_reference_count(fixtures, oracle_engine, gpu_engine):
+    """MQ_OPT_REF_FLOPS = 0 skips the reference FLOP count (ref_flops -1) and
+    changes nothing else: same sigma, same executed FLOPs."""
+    ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
+    x = gpu_engine.to_dev(np.random.default_rng(5).standard_normal(psi.to_flat().size))
+    out = []
+    try:
+        for on in (True, False):
+            gpu_engine.set_ref_flops(on)
+            sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+            try:
+                out.append((sp.ref_flops, sp.exec_flops, sp.apply(x).cpu().numpy()))
+            finally:
+                sp.close()
+    finally:
+        gpu_engine.set_ref_flops(True)
+    assert out[0][0] > 0 and out[1][0] == -1 and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
