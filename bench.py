@@ -248,6 +248,7 @@ def main():
                 ref = th
             block[name] = dict(ms_per_block_iter=ms_it, converged_iterations=itc, converge_s=t_conv, energies=th,
                                max_delta_E_vs_fp64=max(abs(x - y) for x, y in zip(th, ref)))
+            bp.release()  # the batched arenas of one form at a time
             log(f"block Davidson {k} roots {name}: {ms_it:.2f} ms/iteration; converged in {itc} iterations "
                 f"({t_conv:.2f} s), max dE vs fp64 {block[name]['max_delta_E_vs_fp64']:.2e}")
         for name in LOW:
@@ -259,7 +260,7 @@ def main():
     dom = max(range(2, 5), key=lambda i: avg[i])
     tname = "double" if prec_name == "fp64" else "float"
     x3 = ", true" if prec_name == "bf16x3" else ", false"
-    kname = {2: f"k_gemm<0, {tname}, {tname}{x3}>", 3: f"k_wapply<0, {tname}>",
+    kname = {2: f"k_gemm<0, {tname}, {tname}{x3}>", 3: f"k_wapply<0, {tname}, false>",
              4: f"k_gemm<1, {tname}, double{x3}>"}[dom]
     flops = {2: work[0], 3: work[1], 4: work[2]}[dom]
     nbytes = {2: work[3], 3: work[4], 4: work[5]}[dom]

@@ -289,6 +289,12 @@ int mq_block_davidson(mq_plan *plan, const double *L, const double *R,
                       int prec, const double *const *ortho, int n_ortho,
                       int *iterations);
 
+/* Free the work arrays a plan keeps between calls (batched task lists and
+ * arenas, Davidson vectors, the captured sigma graph); they are rebuilt by the
+ * next call that needs them.  No reference counterpart: for callers that hold
+ * many plans at once and need the device memory back. */
+int mq_plan_release(mq_plan *plan);
+
 /* -- diagnostics (no reference counterpart) ------------------------------ */
 /* C(M x N, ldc) = op(A) op(B) with the library's grouped FP64 GEMM kernel on
  * device arrays (column-major; transA: A stored K x M, transB: B stored

@@ -28,6 +28,9 @@ template <class T>
 struct DevArray {
     T *p = nullptr;
     size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray &) = delete;  // owns its device memory
+    DevArray &operator=(const DevArray &) = delete;
     void alloc(size_t count) {
         free_();
         n = count;
@@ -118,12 +121,9 @@ struct mq_plan {
     const double **h_ptrs = nullptr;
     // several vectors per launch (mq_sigma_batch): replicated task lists per batch
     // size, and per-vector copies of the psi, T and C arenas for the largest size used
-    struct BatchLists {
-        int64_t nstrips = 0, npacks = 0, ncopies = 0, ws = 0;
+    struct BatchLists {  // the W application reuses the plan's own strips for every vector
+        int64_t ncopies = 0, ws = 0;
         DevArray<Copy2D> reshape;
-        DevArray<Strip> strips;
-        DevArray<int> packs;
-        DevArray<Src> srcs;
         DevGemm g1, g2;
     };
     std::map<int, std::unique_ptr<BatchLists>> blists;
@@ -400,13 +400,6 @@ static mq_plan::BatchLists &batch_setup(mq_plan *p, int nvec, bool f32) {
         std::unique_ptr<mq_plan::BatchLists> B(new mq_plan::BatchLists());
         B->reshape.upload(batch_copies(h.reshape, nvec, h.vec_len, h.n_prp), s);
         B->ncopies = (int64_t)h.reshape.size() * nvec;
-        StripList sl = batch_strips(h.wapply, nvec, h.n_t, h.n_c);
-        check_packs(sl);
-        B->strips.upload(sl.strips, s);
-        B->srcs.upload(sl.srcs, s);
-        B->packs.upload(sl.packs, s);
-        B->nstrips = (int64_t)sl.strips.size();
-        B->npacks = npacks(sl);
         GemmList g1 = batch_gemm(h.g1, nvec, 0, h.n_prp, h.n_t);
         GemmList g2 = batch_gemm(h.g2, nvec, h.n_c, 0, h.vec_len);
         B->g1.upload(g1, s);
@@ -448,21 +441,20 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
     auto &A = p->barena;
     struct {
         DevArray<Copy2D> &reshape;
-        DevArray<Strip> &strips;
-        DevArray<Src> &srcs;
-        DevArray<int> &packs;
         DevGemm &g1, &g2;
-        int64_t ncopies, nstrips, npacks;
+        int64_t ncopies;
         DevArray<double> &prp, &t, &c, &ws;
         DevArray<float> &prp32, &t32, &c32, &ws32;
-    } B{Bl.reshape, Bl.strips, Bl.srcs, Bl.packs, Bl.g1, Bl.g2, Bl.ncopies, Bl.nstrips, Bl.npacks,
-        A.prp, A.t, A.c, A.ws64, A.prp32, A.t32, A.c32, A.ws32};
+    } B{Bl.reshape, Bl.g1, Bl.g2, Bl.ncopies, A.prp, A.t, A.c, A.ws64, A.prp32, A.t32, A.c32, A.ws32};
+    HostPlan &h = p->hp;
     if (prec == MQ_PREC_FP64) {
         check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la.p, B.prp.p,
                           B.t.p, B.ws.p, B.g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
-        check(launch_wapply(B.strips.p, B.packs.p, (int)B.npacks, B.srcs.p, B.t.p, B.c.p, ST_SIGMA, s), "W apply");
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, B.t.p, B.c.p, ST_SIGMA, s, nvec,
+                            h.n_t, h.n_c),
+              "W apply");
         check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c.p, p->rb.p, Y,
                           B.ws.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
@@ -471,7 +463,9 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la32.p,
                           B.prp32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s, prec == MQ_PREC_BF16X3),
               "gemm T");
-        check(launch_wapply(B.strips.p, B.packs.p, (int)B.npacks, B.srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s), "W apply");
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s,
+                            nvec, h.n_t, h.n_c),
+              "W apply");
         check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c32.p,
                           p->rb32.p, Y, B.ws32.p, B.g2.rowmap.p, ST_SIGMA_OUT, s, prec == MQ_PREC_BF16X3),
               "gemm R");
@@ -698,6 +692,41 @@ int mq_plan_rebind(mq_plan *plan) {
     plan->bound_L = plan->bound_R = nullptr;
     plan->gL = plan->gR = nullptr;
     return MQ_OK;
+}
+
+// Free the work arrays a plan keeps between calls: the batched task lists and
+// arenas of mq_sigma_batch / mq_block_davidson, the Davidson vectors and the
+// captured graph with its staging vectors.  They are rebuilt on the next call
+// that needs them.  The FP64 and FP32 panels stay.
+int mq_plan_release(mq_plan *plan) {
+    if (!plan) return MQ_ERR_ARG;
+    try {
+        check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(plan->ctx->stream), "release sync");
+        plan->blists.clear();
+        auto &A = plan->barena;
+        A.prp.free_();
+        A.t.free_();
+        A.c.free_();
+        A.ws64.free_();
+        A.prp32.free_();
+        A.t32.free_();
+        A.c32.free_();
+        A.ws32.free_();
+        A.cap = A.cap32 = 0;
+        A.ws = 0;
+        plan->jd_vecs.free_();
+        plan->jd_cap = 0;
+        if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);
+        plan->gexec = nullptr;
+        plan->gprec = -1;
+        plan->gL = plan->gR = nullptr;
+        plan->gx.free_();
+        plan->gy.free_();
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
 }
 
 int mq_step_out_desc(const mq_plan *plan, mq_boundary_desc *out) {

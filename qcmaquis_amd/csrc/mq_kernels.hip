@@ -94,13 +94,19 @@ __device__ __forceinline__ void wapply_strip(const Strip &st, const Src *__restr
 
 static_assert(kPackElems <= 256 * WU, "a pack of small strips is run in one pass");
 
-template <int STAGE, class T>
+// nvec vectors share one task list: vector v reads src + v * sstride and writes
+// C + v * dstride (the batched sigma), so the strip and source records are read
+// once per workgroup for all of them.
+// (MULTI false: one vector, compiled without the vector loop.)
+template <int STAGE, class T, bool MULTI>
 __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips, const int *__restrict__ packs,
                                                 const Src *__restrict__ srcs, const T *__restrict__ src,
-                                                T *__restrict__ C) {
+                                                T *__restrict__ C, int nvec_arg, int64_t sstride, int64_t dstride) {
+    const int nvec = MULTI ? nvec_arg : 1;
     const int p0 = packs[blockIdx.x], ns = packs[blockIdx.x + 1] - p0;
     if (ns == 1) {
-        wapply_strip<T>(strips[p0], srcs, src, C);
+        const Strip st = strips[p0];
+        for (int v = 0; v < nvec; ++v) wapply_strip<T>(st, srcs, src + v * sstride, C + v * dstride);
         return;
     }
     __shared__ Strip sst[kPackStrips];
@@ -144,27 +150,31 @@ __global__ void __launch_bounds__(256) k_wapply(const Strip *__restrict__ strips
         }
     }
     // the elements' sources in lock step, so their loads are issued together
-    T acc[WU];
+    for (int vec = 0; vec < nvec; ++vec) {
+        const T *sv = src + vec * sstride;
+        T *Cv = C + vec * dstride;
+        T acc[WU];
 #pragma unroll
-    for (int u = 0; u < WU; ++u) acc[u] = 0;
-    for (int q = 0; q < nsrc; ++q) {
-        T v[WU], c[WU];
+        for (int u = 0; u < WU; ++u) acc[u] = 0;
+        for (int q = 0; q < nsrc; ++q) {
+            T v[WU], c[WU];
 #pragma unroll
-        for (int u = 0; u < WU; ++u) {
-            v[u] = 0;
-            c[u] = 0;
-            if (s0[u] + q < s1[u]) {
-                const Src sr = srcs[s0[u] + q];
-                c[u] = (T)sr.coef;
-                v[u] = src[sr.off + ii[u] + (int64_t)jj[u] * sr.ld];
+            for (int u = 0; u < WU; ++u) {
+                v[u] = 0;
+                c[u] = 0;
+                if (s0[u] + q < s1[u]) {
+                    const Src sr = srcs[s0[u] + q];
+                    c[u] = (T)sr.coef;
+                    v[u] = sv[sr.off + ii[u] + (int64_t)jj[u] * sr.ld];
+                }
             }
+#pragma unroll
+            for (int u = 0; u < WU; ++u) acc[u] += c[u] * v[u];
         }
 #pragma unroll
-        for (int u = 0; u < WU; ++u) acc[u] += c[u] * v[u];
+        for (int u = 0; u < WU; ++u)
+            if (dst[u] >= 0) Cv[dst[u]] = acc[u];
     }
-#pragma unroll
-    for (int u = 0; u < WU; ++u)
-        if (dst[u] >= 0) C[dst[u]] = acc[u];
 }
 
 // ------------------------------------------------------------------ grouped GEMM (MFMA)
@@ -546,26 +556,42 @@ hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s
     return hipGetLastError();
 }
 
-template <class T>
-static hipError_t wapply_launch(const Strip *strips, const int *packs, int npacks, const Src *srcs, const T *src,
-                                T *C, int stage, hipStream_t s) {
-    if (npacks == 0) return hipSuccess;
+template <class T, bool MULTI>
+static void wapply_stage(const Strip *strips, const int *packs, int npacks, const Src *srcs, const T *src, T *C,
+                         int stage, hipStream_t s, int nvec, int64_t ss, int64_t ds) {
     const dim3 g(npacks), b(256);
     switch (stage) {
-        case ST_SIGMA: hipLaunchKernelGGL((k_wapply<ST_SIGMA, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
-        case ST_FUSE: hipLaunchKernelGGL((k_wapply<ST_FUSE, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
-        case ST_MISC: hipLaunchKernelGGL((k_wapply<ST_MISC, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
-        default: hipLaunchKernelGGL((k_wapply<ST_STEP, T>), g, b, 0, s, strips, packs, srcs, src, C); break;
+        case ST_SIGMA:
+            hipLaunchKernelGGL((k_wapply<ST_SIGMA, T, MULTI>), g, b, 0, s, strips, packs, srcs, src, C, nvec, ss, ds);
+            break;
+        case ST_FUSE:
+            hipLaunchKernelGGL((k_wapply<ST_FUSE, T, MULTI>), g, b, 0, s, strips, packs, srcs, src, C, nvec, ss, ds);
+            break;
+        case ST_MISC:
+            hipLaunchKernelGGL((k_wapply<ST_MISC, T, MULTI>), g, b, 0, s, strips, packs, srcs, src, C, nvec, ss, ds);
+            break;
+        default:
+            hipLaunchKernelGGL((k_wapply<ST_STEP, T, MULTI>), g, b, 0, s, strips, packs, srcs, src, C, nvec, ss, ds);
+            break;
     }
+}
+template <class T>
+static hipError_t wapply_launch(const Strip *strips, const int *packs, int npacks, const Src *srcs, const T *src,
+                                T *C, int stage, hipStream_t s, int nvec, int64_t ss, int64_t ds) {
+    if (npacks == 0 || nvec < 1) return hipSuccess;
+    if (nvec == 1)
+        wapply_stage<T, false>(strips, packs, npacks, srcs, src, C, stage, s, 1, 0, 0);
+    else
+        wapply_stage<T, true>(strips, packs, npacks, srcs, src, C, stage, s, nvec, ss, ds);
     return hipGetLastError();
 }
 hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const double *T,
-                         double *C, int stage, hipStream_t s) {
-    return wapply_launch<double>(strips, packs, npacks, srcs, T, C, stage, s);
+                         double *C, int stage, hipStream_t s, int nvec, int64_t src_stride, int64_t dst_stride) {
+    return wapply_launch<double>(strips, packs, npacks, srcs, T, C, stage, s, nvec, src_stride, dst_stride);
 }
 hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const float *T, float *C,
-                         int stage, hipStream_t s) {
-    return wapply_launch<float>(strips, packs, npacks, srcs, T, C, stage, s);
+                         int stage, hipStream_t s, int nvec, int64_t src_stride, int64_t dst_stride) {
+    return wapply_launch<float>(strips, packs, npacks, srcs, T, C, stage, s, nvec, src_stride, dst_stride);
 }
 
 template <int STAGE, class T, class TO, bool X3>

@@ -17,10 +17,13 @@ hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *
 hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s);
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s);
 // one workgroup per pack of strips (StripList::packs, npacks = packs.size() - 1)
+// nvec > 1: the same task list for nvec vectors, vector v at T + v * src_stride and
+// C + v * dst_stride
 hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const double *T,
-                         double *C, int stage, hipStream_t s);
+                         double *C, int stage, hipStream_t s, int nvec = 1, int64_t src_stride = 0,
+                         int64_t dst_stride = 0);
 hipError_t launch_wapply(const Strip *strips, const int *packs, int npacks, const Src *srcs, const float *T, float *C,
-                         int stage, hipStream_t s);
+                         int stage, hipStream_t s, int nvec = 1, int64_t src_stride = 0, int64_t dst_stride = 0);
 // grouped GEMM over a plan's tile list (FP64; FP32 operands with FP32 or FP64 output,
 // multiplied in FP32 or, with bf16x3, as three bf16 products accumulated in FP32);
 // rowmap: the row placements of groups with rowmap > 0 (may be null when none has)

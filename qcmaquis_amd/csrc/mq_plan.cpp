@@ -898,27 +898,6 @@ std::vector<Copy2D> batch_copies(const std::vector<Copy2D> &v, int nvec, int64_t
     return out;
 }
 
-StripList batch_strips(const StripList &sl, int nvec, int64_t src_step, int64_t dst_step) {
-    StripList out;
-    out.strips.reserve(sl.strips.size() * nvec);
-    out.srcs.reserve(sl.srcs.size() * nvec);
-    for (int j = 0; j < nvec; ++j) {
-        const int base = (int)out.srcs.size();
-        for (Src x : sl.srcs) {
-            x.off += j * src_step;
-            out.srcs.push_back(x);
-        }
-        for (Strip st : sl.strips) {
-            st.dst += j * dst_step;
-            st.s0 += base;
-            st.s1 += base;
-            out.strips.push_back(st);
-        }
-    }
-    make_packs(out);
-    return out;
-}
-
 GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step, int64_t c_step) {
     GemmList out;
     const int G = (int)gl.groups.size();

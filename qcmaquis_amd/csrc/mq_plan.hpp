@@ -146,9 +146,10 @@ void order_tiles(GemmList &gl);
 
 // Task lists for nvec vectors in one launch per stage: vector j's copies are the
 // single-vector tasks with their per-vector arena offsets advanced by j steps
-// (a step of 0 marks an operand all vectors share).
+// (a step of 0 marks an operand all vectors share).  The W application needs no
+// copies: its kernel runs the plan's one strip list for every vector, with
+// per-vector strides.
 std::vector<Copy2D> batch_copies(const std::vector<Copy2D> &v, int nvec, int64_t src_step, int64_t dst_step);
-StripList batch_strips(const StripList &sl, int nvec, int64_t src_step, int64_t dst_step);
 GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step, int64_t c_step);
 
 // FLOPs of the reference schedule (site_hamil.hpp:28-204 for sigma, move_boundary.hpp for steps)

@@ -56,6 +56,7 @@ def lib():
         L.mq_block_davidson.argtypes = [vp, vp, vp, vp, vp, vp, f64p, C.c_int, C.c_int, C.c_double, C.c_int,
                                         C.POINTER(vp), C.c_int, C.POINTER(C.c_int)]
         L.mq_plan_rebind.argtypes = [vp]
+        L.mq_plan_release.argtypes = [vp]
         L.mq_gemm_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
                                   C.c_int, C.c_int]
         L.mq_gemm_f32.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
@@ -400,6 +401,11 @@ class SigmaPlan:
         self.work_flops = self.ref_flops if self.ref_flops >= 0 else self.exec_flops
         self.n = lib().mq_plan_vec_len(self.p)
         self.workspace_bytes = lib().mq_plan_workspace_bytes(self.p)
+
+    def release(self):
+        """Free the plan's cached work arrays (batched lists and arenas, Davidson
+        vectors, captured graph); the next call that needs them rebuilds them."""
+        self.eng.check(lib().mq_plan_release(self.p), "mq_plan_release")
 
     def close(self):
         if getattr(self, "_dp", None):

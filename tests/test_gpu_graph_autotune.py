@@ -144,3 +144,27 @@ def test_plans_without_the_reference_count(fixtures, oracle_engine, gpu_engine):
         gpu_engine.set_ref_flops(True)
     assert out[0][0] > 0 and out[1][0] == -1 and out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_release_frees_caches_and_calls_rebuild_them(fixtures, oracle_engine, gpu_engine):
+    """mq_plan_release drops the batched lists and arenas, the Davidson vectors and
+    the captured graph; the next batched sigma, graph replay and block Davidson
+    rebuild them and give the same results."""
+    from qcmaquis_amd.desc import MQ_PREC_FP32, MQ_PREC_FP64
+    ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
+    sp = gpu_engine.sigma_plan(ham.symm, psi, Lg, Rg, W)
+    try:
+        X = gpu_engine.torch.as_tensor(np.random.default_rng(6).standard_normal((3, sp.n))).to(gpu_engine.device)
+        before = []
+        for rep in range(2):
+            out = [sp.apply_batch(X, prec=MQ_PREC_FP64).cpu().numpy(),
+                   sp.apply_batch(X, prec=MQ_PREC_FP32).cpu().numpy(),
+                   sp.apply_graph(X[0].contiguous(), prec=MQ_PREC_FP64).cpu().numpy()]
+            th, _, its = sp.block_davidson(X, 100, 1e-10, prec=MQ_PREC_FP64)
+            out.append(np.array(th))
+            sp.release()
+            before.append(out)
+        for a, b in zip(*before):
+            assert np.array_equal(a, b)
+    finally:
+        sp.close()
