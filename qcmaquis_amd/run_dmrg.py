@@ -54,20 +54,52 @@ class _CompareEngine:
             sp.close()
 
 
-def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print):
+    def block_solve(self, symm, psis, L, R, W, max_iter=10, tol=1e-8):
+        """The roots in FP64 (driving the sweep) and, on the same plan from the same
+        start vectors, in the mixed and BF16x3 forms; records the largest
+        |theta_form - theta_fp64| over the roots."""
+        import numpy as np
+        from .gpu import independent_starts
+        sp = self.eng.sigma_plan(symm, psis[0], L, R, W)
+        try:
+            X0 = self.eng.torch.as_tensor(independent_starts([p.to_flat() for p in psis])).to(self.eng.device)
+            diag = sp.diagonal() if symm == "su2u1" else None
+            t64, U, its = sp.block_davidson(X0, max_iter, tol, prec=MQ_PREC_FP64, diag=diag)
+            for name, prec in LOW_FORMS:
+                t, _, _ = sp.block_davidson(X0, max_iter, tol, prec=prec, diag=diag)
+                self.diffs[name].append(float(np.max(np.abs(np.array(t) - np.array(t64)))))
+            Uh = U.cpu().numpy()
+            return t64, [Uh[r] for r in range(len(psis))], its
+        finally:
+            sp.close()
+
+
+def _driver(ham, eng, m, roots, jd_maxiter, jd_tol, seed, eigensolver="jd"):
+    from .sweep import StateAveragedDMRG
+    if roots > 1:
+        return StateAveragedDMRG(ham, eng, m, roots, jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed)
+    return TwoSiteDMRG(ham, eng, m, jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed, eigensolver=eigensolver)
+
+
+def _energies(d, E):
+    return list(d.root_energies[-1]) if hasattr(d, "root_energies") else [E]
+
+
+def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print, roots=1):
     """FP64 sweeps; every local problem is also solved in the mixed and BF16x3 forms."""
     from .gpu import GpuEngine
     eng = _CompareEngine(GpuEngine(0, prec=MQ_PREC_FP64))
     eng.set_ref_flops(False)
     try:
-        d = TwoSiteDMRG(ham, eng, ramp[0], jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed)
+        d = _driver(ham, eng, ramp[0], roots, jd_maxiter, jd_tol, seed)
         out = []
         t0 = time.perf_counter()
         for s, m in enumerate(ramp):
             d.m = m
             eng.diffs = {n: [] for n, _ in LOW_FORMS}
             E = d.sweep(s)
-            rec = dict(sweep=s, m=m, energy=E, solves=len(eng.diffs["mixed"]), seconds=time.perf_counter() - t0)
+            rec = dict(sweep=s, m=m, energy=E, energies=_energies(d, E), solves=len(eng.diffs["mixed"]),
+                       seconds=time.perf_counter() - t0)
             for n, _ in LOW_FORMS:
                 rec[f"max_abs_dtheta_{n}"] = max(eng.diffs[n])
                 rec[f"mean_abs_dtheta_{n}"] = sum(eng.diffs[n]) / len(eng.diffs[n])
@@ -83,20 +115,21 @@ def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print):
 NAMES = {MQ_PREC_FP64: "fp64", MQ_PREC_FP32: "mixed", MQ_PREC_BF16X3: "bf16x3"}
 
 
-def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd"):
+def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd", roots=1):
     from .gpu import GpuEngine
     eng = GpuEngine(0, prec=prec)
     eng.set_ref_flops(False)  # the sweep needs no reference FLOP counts
     try:
-        d = TwoSiteDMRG(ham, eng, ramp[0], jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed, eigensolver=eigensolver)
+        d = _driver(ham, eng, ramp[0], roots, jd_maxiter, jd_tol, seed, eigensolver)
         out = []
         t0 = time.perf_counter()
         for s, m in enumerate(ramp):
             d.m = m
             E = d.sweep(s)
             recs = [r for r in d.records if r["sweep"] == s]
-            out.append(dict(sweep=s, m=m, energy=E, max_bond=max(r["bond"] for r in recs),
-                            jd_seconds=sum(r["t_jd"] for r in recs), seconds=time.perf_counter() - t0))
+            out.append(dict(sweep=s, m=m, energy=E, energies=_energies(d, E), max_bond=max(r["bond"] for r in recs),
+                            jd_seconds=sum(r.get("t_jd", r.get("t_solve", 0.0)) for r in recs),
+                            seconds=time.perf_counter() - t0))
             log(f"  {NAMES[prec]} sweep {s} m {m} E {E:.12f} "
                 f"t {time.perf_counter() - t0:.1f}s")
         return out
@@ -115,6 +148,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--eigensolver", default="jd", choices=["jd", "davidson"],
                     help="IETL_JCD (default) or IETL_DAVIDSON (SU2 only)")
+    ap.add_argument("--roots", type=int, default=1,
+                    help="states optimised together (state-averaged sweeps with the block Davidson)")
     ap.add_argument("--out")
     a = ap.parse_args(argv)
     if a.fcidump:
@@ -124,8 +159,9 @@ def main(argv=None):
         ham = Hamiltonian(a.L, a.L, 0, synthetic(a.L, seed=1000 + a.L))
     ramp = [int(v) for v in a.ramp.split(",")]
     if a.prec == "compare":
-        res = run_compare(ham, ramp, a.jd_maxiter, a.jd_tol, a.seed)
-        rec = dict(L=ham.L, ramp=ramp, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, mode="compare", results=res)
+        res = run_compare(ham, ramp, a.jd_maxiter, a.jd_tol, a.seed, roots=a.roots)
+        rec = dict(L=ham.L, ramp=ramp, roots=a.roots, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, mode="compare",
+                   results=res)
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(rec, f, indent=1)
@@ -134,8 +170,10 @@ def main(argv=None):
              "bf16x3": [("bf16x3", MQ_PREC_BF16X3)],
              "both": [("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)],
              "all": [("fp64", MQ_PREC_FP64)] + list(LOW_FORMS)}[a.prec]
-    res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver) for name, p in precs}
-    rec = dict(L=ham.L, ramp=ramp, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, eigensolver=a.eigensolver, results=res)
+    res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver, roots=a.roots)
+           for name, p in precs}
+    rec = dict(L=ham.L, ramp=ramp, roots=a.roots, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol,
+               eigensolver=a.eigensolver, results=res)
     for name, _ in LOW_FORMS:
         if "fp64" in res and name in res:
             rec[f"delta_E_{name}"] = [m["energy"] - f["energy"] for f, m in zip(res["fp64"], res[name])]
