@@ -38,3 +38,16 @@ def test_gpu_sa_twou1_matches_oracle_sweep(oracle_engine, gpu_engine):
         Es.append(np.array(d.run(3)))
     assert np.abs(Es[0] - e0[:3]).max() < 1e-9
     assert np.abs(Es[1] - e0[:3]).max() < 1e-9, (Es[1], e0[:3])
+
+
+def test_gpu_sa_compare_sweep_matches_fci(oracle_engine):
+    """The sweep CLI's compare mode with 3 roots: FP64 state-averaged sweeps reach the
+    FCI singlets, and every block solve repeated in FP32 and BF16x3 agrees with FP64."""
+    from qcmaquis_amd.run_dmrg import run_compare
+    from qcmaquis_amd.sweep import Hamiltonian
+    ints, e0, singlets = _fci_roots(8, 1008, 3)
+    print("compare sweeps L=8, 3 roots", flush=True)
+    res = run_compare(Hamiltonian(8, 8, 0, ints), [400, 400, 400], 60, 1e-10, 42, roots=3)
+    last = res[-1]
+    assert np.abs(np.array(last["energies"]) - singlets[:3]).max() < 1e-9, (last["energies"], singlets[:3])
+    assert last["max_abs_dtheta_mixed"] < 1e-9 and last["max_abs_dtheta_bf16x3"] < 1e-9, last
