@@ -250,7 +250,21 @@ struct GemmStage {
     T a[TM * kstep<T>() / 256], b[TN * kstep<T>() / 256];
 };
 
-template <int TM, int TN, class T>
+// Row and k of A-tile slot r of this thread, column-major A (transA false).
+// QUADS: the slots of a thread form whole groups of four consecutive k of one
+// row (group kq + (256 / TM) * (r / 4), kq = thread / TM), so the BF16x3 stash
+// writes one 16-byte LDS quad per group; the lanes of one load still read
+// consecutive rows of one column.  Otherwise k = (thread + 256 r) / TM.
+template <int TM, bool QUADS>
+__device__ __forceinline__ void a_slot(int r, int &i, int &k) {
+    i = threadIdx.x % TM;
+    if (QUADS)
+        k = 4 * ((int)threadIdx.x / TM + (256 / TM) * (r >> 2)) + (r & 3);
+    else
+        k = ((int)threadIdx.x + 256 * r) / TM;
+}
+
+template <int TM, int TN, class T, bool QUADS = false>
 __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restrict__ A, const T *__restrict__ B,
                                            int m0, int n0, int k0, int kend, GemmStage<TM, TN, T> &st) {
     constexpr int TK = kstep<T>();
@@ -259,8 +273,7 @@ __device__ __forceinline__ void gemm_fetch(const GemmGroup &g, const T *__restri
         const int e = threadIdx.x + 256 * r;
         int i, k;
         if (!g.transA) {
-            i = e % TM;
-            k = e / TM;
+            a_slot<TM, QUADS>(r, i, k);
         } else {
             k = e % TK;
             i = e / TK;
@@ -315,14 +328,32 @@ template <int TM, int TN>
 __device__ __forceinline__ void gemm_stash_x3(const GemmGroup &g, const GemmStage<TM, TN, float> &st,
                                               Bf16Quad (*Ax)[X3_QUADS], Bf16Quad (*Bx)[X3_QUADS]) {
     constexpr int TK = kstep<float>();
+    if (!g.transA) {
+        // whole quads (the QUADS slots of gemm_fetch): one 16-byte write each
 #pragma unroll
-    for (int r = 0; r < TM * TK / 256; ++r) {
-        const int e = threadIdx.x + 256 * r;
-        const int i = !g.transA ? e % TM : e / TK, k = !g.transA ? e / TM : e % TK;
-        short h, l;
-        bf16_split(st.a[r], h, l);
-        Ax[i][k >> 2].hi[k & 3] = h;
-        Ax[i][k >> 2].lo[k & 3] = l;
+        for (int q = 0; q < TM * TK / 1024; ++q) {
+            int i, k;
+            a_slot<TM, true>(4 * q, i, k);
+            Bf16Quad v;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                short h, l;
+                bf16_split(st.a[4 * q + u], h, l);
+                v.hi[u] = h;
+                v.lo[u] = l;
+            }
+            Ax[i][k >> 2] = v;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < TM * TK / 256; ++r) {
+            const int e = threadIdx.x + 256 * r;
+            const int i = e / TK, k = e % TK;
+            short h, l;
+            bf16_split(st.a[r], h, l);
+            Ax[i][k >> 2].hi[k & 3] = h;
+            Ax[i][k >> 2].lo[k & 3] = l;
+        }
     }
 #pragma unroll
     for (int r = 0; r < TN * TK / 256; ++r) {
@@ -365,7 +396,7 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
     GemmStage<TM, TN, T> st;
     int buf = 0;
     if (kbeg < kend) {
-        gemm_fetch<TM, TN, T>(g, A, B, m0, n0, kbeg, kend, st);
+        gemm_fetch<TM, TN, T, X3>(g, A, B, m0, n0, kbeg, kend, st);
         if constexpr (X3)
             gemm_stash_x3<TM, TN>(g, st, Ax[0], Bx[0]);
         else
@@ -374,7 +405,7 @@ __device__ __forceinline__ void gemm_tile(const GemmGroup &g, const GemmTile &t,
     __syncthreads();
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
         const bool more = k0 + TK < kend;
-        if (more) gemm_fetch<TM, TN, T>(g, A, B, m0, n0, k0 + TK, kend, st);
+        if (more) gemm_fetch<TM, TN, T, X3>(g, A, B, m0, n0, k0 + TK, kend, st);
         if constexpr (X3) {
             // one 16x16x16 step covers the K tile: lane group lq holds k = 4 lq .. 4 lq + 3
             Bf16Quad av[MI];
