@@ -12,15 +12,22 @@ value = algorithmic FLOPs of the reference schedule / wall time per step.
 Three forms are measured: the plain FP64 sigma, the mixed form (FP32 panels,
 intermediates and MFMA, FP64 output; Davidson in FP64 with an FP64 defect
 correction per solve) and the BF16x3 form (the mixed form with every GEMM
-product as three bf16 MFMA products accumulated in FP32).  The value is the
-fastest form whose Davidson eigenvalue agrees with the FP64 one to 1e-8 Eh at
-convergence (all solves started from the FP64-converged eigenvector).
+product as bf16 hi/lo MFMA products accumulated in FP32).  Every form solves
+the site problem from the same start vector (the MPS guess) to a residual of
+1e-8 |theta|; the value is the fastest form whose eigenvalue agrees with the
+FP64 one to 1e-8 Eh.  The record of each solve gives the iterations in the
+form, the FP64 continuation iterations and the form's own Ritz value.
 The 4-root block Davidson (one batched sigma of the block per iteration) is
-timed in each form on the same problem, with its converged roots compared
-with the FP64 ones.
-The JSON line also carries ms per Jacobi-Davidson iteration, per-stage HIP
-event timings on the library stream, the roofline of the dominant kernel and
-the CPU oracle timed on a bounded sample of the same sigma.
+timed in each form from the MPS guess plus random vectors restricted to the
+coordinates H_eff couples, with its lowest root compared with the single-root
+eigenvalue.
+The JSON line also carries ms per Jacobi-Davidson iteration, the cost of one
+site solve (boundary binding plus the solve), per-stage HIP event timings on
+the library stream, the roofline of the sigma (F and the minimum bytes B_min
+of the reference schedule, the binding roof and the fraction of the form's
+MFMA peak reached), the measured HBM traffic of the run's schedule when a
+counter profile of it is committed, and the CPU oracle timed on a bounded
+sample of the same sigma.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -68,6 +75,36 @@ def cpu_baseline(ham, psi, Lb, Rb, W, y_gpu, seconds):
     return dict(value=f0 / per / 1e12, unit="TFLOP/s", cores=int(orclib().orc_num_threads()), kind="port",
                 sample=f"{n} FP64 oracle sigma(s) of the same centre-bond problem, {per * 1e3:.1f} ms each, "
                        f"host BLAS {blas_name()}"), err
+
+
+def sigma_kernels(form):
+    """Kernel names (as rocprofv3 reports them) of the stages of one sigma in a form."""
+    t = "double" if form == "fp64" else "float"
+    x3 = "true" if form == "bf16x3" else "false"
+    return {"reshape": f"k_copy2d<0, double, {t}>", "gemm_T": f"k_gemm<0, {t}, {t}, {x3}>",
+            "w_apply": f"k_wapply<0, {t}, false>", "gemm_out": f"k_gemm<1, {t}, double, {x3}>",
+            "split_reduce": f"k_split_reduce<1, {t}, double>"}
+
+
+def plan_key(form, plan):
+    """Identifies a plan's task lists for a form: schedules that build the same
+    lists (e.g. fused-column ratios that fuse the same columns) share it."""
+    return f"{form} | executed {plan.exec_flops:.0f} | workspace {plan.workspace_bytes}"
+
+
+def traffic_profile(a, sec_src, form, plan):
+    """Counter-measured HBM bytes per launch of the sigma kernels for this
+    workload, form and plan, from the committed profile (None when the run
+    builds a plan that has not been profiled)."""
+    if not (sec_src == "tests/golden/sectors_L24_m512.json" and a.L == 24 and a.symm == "su2u1"):
+        return None
+    path = os.path.join(ROOT, "profiles", "sigma_L24_m512_traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get(plan_key(form, plan))
+    if rec is None:
+        return None
+    return dict(rec, file=os.path.relpath(path, ROOT))
 
 
 def main():
@@ -167,6 +204,19 @@ def main():
             sp.apply(x, y, prec)
         eng.sync()
         torch.cuda.synchronize()
+        # per-site cost of binding the boundaries: arrangement into panels, the fused
+        # rows and (mixed forms) the FP32 copies, measured as bind + sigma - sigma
+        t0 = time.perf_counter()
+        for _ in range(3):
+            sp.apply(x, y, prec)
+        eng.sync()
+        t_sig = (time.perf_counter() - t0) / 3
+        t0 = time.perf_counter()
+        for _ in range(3):
+            sp.bind()
+            sp.apply(x, y, prec)
+        eng.sync()
+        bind_ms = max((time.perf_counter() - t0) / 3 - t_sig, 0.0) * 1e3
         tot = [0.0] * 5
         work = None
         t0 = time.perf_counter()
@@ -185,7 +235,7 @@ def main():
         eng.sync()
         torch.cuda.synchronize()
         ms_jd = (time.perf_counter() - t0) / max(its, 1) * 1e3
-        return dict(dt=dt, avg=avg, work=work, theta=theta, its=its, ms_jd=ms_jd, y=ycopy)
+        return dict(dt=dt, avg=avg, work=work, theta=theta, its=its, ms_jd=ms_jd, y=ycopy, bind_ms=bind_ms)
 
     res = {}
     for name, prec in FORMS:
@@ -194,25 +244,46 @@ def main():
     r64 = res["fp64"]
     LOW = [n for n, _ in FORMS if n != "fp64"]
     sig_err = {k: float(np.abs(res[k]["y"] - r64["y"]).max() / np.abs(r64["y"]).max()) for k in LOW}
-    # energy agreement at convergence, the regime of the solves that fix a sweep's
-    # final energy: the lowest eigenvector is first converged in FP64 (restarted
-    # solves of 40 iterations), then both forms run one more solve from it
+    # energy agreement: every form solves the site problem from the same start
+    # vector (the MPS guess) to |r| <= 1e-8 |theta| (the reference's default
+    # ietl_jcd_tol, DmrgParameters.h:53; the eigenvalue error is then about
+    # |r|^2 / gap, far below 1e-8 Eh), with restarted solves of JD_SOLVE
+    # iterations; each record keeps the iterations in the form, the FP64
+    # continuation iterations, the form's own last Ritz value and the wall time
+    JD_SOLVE, RESTARTS, TOL = 50, 8, 1e-8
     conv, dE = None, None
     if not a.no_delta_e:
-        # restarted FP64 solves of 40 iterations until one converges (at most 12)
-        u = x
-        for _ in range(12):
-            _, u, its = sp.jd(u, 40, 1e-11, prec=MQ_PREC_FP64)
-            if its < 40:
-                break
         conv = {}
         for name, prec in FORMS:
-            th, _, its = plans[name].jd(u, 10, 1e-10, prec=prec)
-            conv[name] = dict(theta=th + ham.ecore, iterations=its)
+            p_ = plans[name]
+            u, rec = x, dict(form_iterations=0, fp64_iterations=0, restarts=0, converged=False)
+            eng.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for r_ in range(RESTARTS):
+                th, u, its = p_.jd(u, JD_SOLVE, TOL, prec=prec)
+                info = p_.solve_info()
+                if r_ == 0:
+                    rec["theta_form_first_solve"] = info["theta_form"] + ham.ecore
+                    rec["theta_rayleigh_fp64_first_solve"] = info["theta_rayleigh_fp64"] + ham.ecore
+                rec["form_iterations"] += info["form_iterations"]
+                rec["fp64_iterations"] += info["fp64_iterations"]
+                rec["restarts"] = r_
+                done = info["fp64_iterations"] < JD_SOLVE if prec != MQ_PREC_FP64 and info["fp64_iterations"] > 0 \
+                    else info["form_iterations"] < JD_SOLVE
+                if done:
+                    rec["converged"] = True
+                    break
+            eng.sync()
+            rec["solve_s"] = time.perf_counter() - t0
+            rec["theta"] = th + ham.ecore
+            conv[name] = rec
         dE = {k: abs(conv[k]["theta"] - conv["fp64"]["theta"]) for k in LOW}
-        log(f"converged theta fp64 {conv['fp64']['theta']:.12f} ({conv['fp64']['iterations']} it); "
-            + "; ".join(f"{k} {conv[k]['theta']:.12f} ({conv[k]['iterations']} it) dE {dE[k]:.2e}, sigma rel err "
-                        f"{sig_err[k]:.2e}" for k in dE))
+        for name, _ in FORMS:
+            c_ = conv[name]
+            log(f"solve {name}: theta {c_['theta']:.12f}, {c_['form_iterations']} iterations in the form + "
+                f"{c_['fp64_iterations']} FP64, {c_['solve_s']:.2f} s, converged {c_['converged']}"
+                + (f", dE vs fp64 {dE[name]:.2e}, sigma rel err {sig_err[name]:.2e}" if name in dE else ""))
     ok = ["fp64"] + [k for k in LOW if dE is not None and dE[k] <= 1e-8]
     prec_name = min(ok, key=lambda k: res[k]["dt"])
     ch = res[prec_name]
@@ -222,11 +293,16 @@ def main():
     # the block) in each form, and the converged roots against the FP64 ones
     block = None
     if a.roots > 0:
+        from qcmaquis_amd.gpu import independent_starts
         k = a.roots
         rng = np.random.default_rng(11)
-        X0 = torch.as_tensor(np.stack([psi.to_flat()] + [rng.standard_normal(sp.n) for _ in range(k - 1)]))
-        X0 = X0.to(eng.device)
-        block = {"roots": k}
+        # the MPS guess and random vectors on the coordinates H_eff couples (the
+        # structurally zero columns of H_eff have eigenvalue 0, below every physical
+        # root of this Hamiltonian)
+        live = sp.live_mask()
+        X0 = independent_starts([psi.to_flat()] + [rng.standard_normal(sp.n) for _ in range(k - 1)], live=live)
+        X0 = torch.as_tensor(X0).to(eng.device)
+        block = {"roots": k, "dead_coordinates": int((~live).sum())}
         ref = None
         for name, prec in FORMS:
             bp = plans[name]
@@ -246,53 +322,76 @@ def main():
             th = [t + ham.ecore for t in th]
             if ref is None:
                 ref = th
+            info = bp.solve_info()
             block[name] = dict(ms_per_block_iter=ms_it, converged_iterations=itc, converge_s=t_conv, energies=th,
+                               form_iterations=info["form_iterations"], fp64_iterations=info["fp64_iterations"],
                                max_delta_E_vs_fp64=max(abs(x - y) for x, y in zip(th, ref)))
+            if conv is not None:
+                block[name]["root0_minus_single_root_theta"] = th[0] - conv["fp64"]["theta"]
             bp.release()  # the batched arenas of one form at a time
             log(f"block Davidson {k} roots {name}: {ms_it:.2f} ms/iteration; converged in {itc} iterations "
-                f"({t_conv:.2f} s), max dE vs fp64 {block[name]['max_delta_E_vs_fp64']:.2e}")
+                f"({t_conv:.2f} s), roots {', '.join(f'{e:.9f}' for e in th)}, max dE vs fp64 "
+                f"{block[name]['max_delta_E_vs_fp64']:.2e}")
         for name in LOW:
             block[name]["speedup_vs_fp64"] = block["fp64"]["ms_per_block_iter"] / block[name]["ms_per_block_iter"]
 
-    # roofline of the dominant kernel of the reported form: the binding roof is the
-    # one with the larger lower bound on the time (algorithmic flops / MFMA peak vs
-    # algorithmic bytes / HBM peak)
-    dom = max(range(2, 5), key=lambda i: avg[i])
-    tname = "double" if prec_name == "fp64" else "float"
-    x3 = ", true" if prec_name == "bf16x3" else ", false"
-    kname = {2: f"k_gemm<0, {tname}, {tname}{x3}>", 3: f"k_wapply<0, {tname}, false>",
-             4: f"k_gemm<1, {tname}, double{x3}>"}[dom]
-    flops = {2: work[0], 3: work[1], 4: work[2]}[dom]
-    nbytes = {2: work[3], 3: work[4], 4: work[5]}[dom]
+    # roofline of the sigma on the reference schedule's terms: F (algorithmic
+    # flops) and B_min = |L| + |R| + |psi| + |sigma| in FP64 (the bytes any
+    # schedule must move); the binding roof is the one with the larger lower bound
+    # on the time, F / P_MFMA(form) or B_min / BW_HBM, and achieved = F / t with t
+    # the kernel time of one sigma from HIP events on the library stream
     peak_f = {"fp64": PEAK_FP64_TFLOPS, "mixed": PEAK_FP32_TFLOPS, "bf16x3": PEAK_BF16X3_TFLOPS}[prec_name]
-    t_s = avg[dom] * 1e-3
+    F, Bmin = sp.ref_flops, sp.min_bytes
+    t_sig = sum(avg[1:]) * 1e-3
+    kern = sigma_kernels(prec_name)
+    prof = traffic_profile(a, sec_src, prec_name, plans[prec_name])
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "sigma_L24_m512_traffic.json")
-    if sec_src == "tests/golden/sectors_L24_m512.json" and a.L == 24 and os.path.exists(tfile):
-        rec = json.load(open(tfile)).get(kname)
-        if rec:
-            traffic = rec["traffic_bytes"]
-    if flops / (peak_f * 1e12) >= nbytes / (PEAK_HBM_GBS * 1e9) and dom != 3:
-        ach = flops / t_s / 1e12
-        roof = dict(bound="mfma", kernel=kname, achieved=ach, peak=peak_f, unit="TFLOP/s", frac=ach / peak_f,
-                    traffic=traffic)
+    if prof is not None and all(k in prof["kernels"] for k in kern.values()):
+        traffic = sum(prof["kernels"][k]["traffic_bytes"] * prof["kernels"][k].get("per_sigma", 1.0)
+                      for k in kern.values())
+    mfma_bound = F / (peak_f * 1e12) >= Bmin / (PEAK_HBM_GBS * 1e9)
+    if mfma_bound:
+        ach = F / t_sig / 1e12
+        roof = dict(bound="mfma", achieved=ach, peak=peak_f, unit="TFLOP/s", frac=ach / peak_f, traffic=traffic)
     else:
-        ach = nbytes / t_s / 1e9
-        roof = dict(bound="hbm", kernel=kname, achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=ach / PEAK_HBM_GBS, traffic=traffic)
-    roof["flops"] = flops
-    roof["algorithmic_bytes"] = nbytes
-    roof["avg_launch_ms"] = avg[dom]
+        ach = Bmin / t_sig / 1e9
+        roof = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                    traffic=traffic)
+    roof.update(kernel="sigma: " + ", ".join(kern.values()), flops=F, algorithmic_bytes=Bmin,
+                intensity_flop_per_byte=F / Bmin, avg_launch_ms=t_sig * 1e3,
+                traffic_over_algorithmic=None if traffic is None else traffic / Bmin,
+                traffic_profile=None if prof is None else prof["file"])
+    # the slowest stage kernel on its own: executed flops and the implementation's
+    # own bytes (its operands and intermediates) per launch
+    dom = max(range(2, 5), key=lambda i: avg[i])
+    kname = {2: kern["gemm_T"], 3: kern["w_apply"], 4: kern["gemm_out"]}[dom]
+    t_s = avg[dom] * 1e-3
+    ex_f = {2: work[0], 3: work[1], 4: work[2]}[dom]
+    ex_b = {2: work[3], 3: work[4], 4: work[5]}[dom]
+    roof["dominant_kernel"] = dict(
+        kernel=kname, avg_launch_ms=avg[dom], executed_flops=ex_f, tflops=ex_f / t_s / 1e12,
+        frac_of_mfma_peak=ex_f / t_s / 1e12 / peak_f, implementation_bytes=ex_b, gbs=ex_b / t_s / 1e9,
+        traffic=None if prof is None or kname not in prof["kernels"] else prof["kernels"][kname]["traffic_bytes"])
 
     cpu, err = None, None
     if not a.no_cpu:
         cpu, err = cpu_baseline(ham, psi, Lb, Rb, W, r64["y"], a.cpu_seconds)
         log(f"cpu baseline {cpu['value'] * 1e3:.2f} GFLOP/s; GPU FP64 vs oracle max rel err {err:.2e}")
 
-    def summary(r):
-        return dict(tflops=sp.ref_flops / r["dt"] / 1e12, ms_per_sigma=r["dt"] * 1e3,
+    def summary(name):
+        r = res[name]
+        pk = {"fp64": PEAK_FP64_TFLOPS, "mixed": PEAK_FP32_TFLOPS, "bf16x3": PEAK_BF16X3_TFLOPS}[name]
+        out_ = dict(tflops=sp.ref_flops / r["dt"] / 1e12, ms_per_sigma=r["dt"] * 1e3,
+                    frac_of_mfma_peak=sp.ref_flops / (sum(r["avg"][1:]) * 1e-3) / 1e12 / pk,
                     ms_per_davidson_iter=r["ms_jd"], stages_ms=dict(zip(STAGES, r["avg"])),
-                    stage_algorithmic_bytes=dict(zip(STAGES[2:], r["work"][3:6])), theta=r["theta"] + ham.ecore)
+                    stage_implementation_bytes=dict(zip(STAGES[2:], r["work"][3:6])),
+                    theta_after_jd_iters=r["theta"] + ham.ecore, bind_ms=r["bind_ms"])
+        if conv is not None:
+            c_ = conv[name]
+            out_["solve"] = c_
+            # one site solve: binding the boundaries, then the solve from the MPS guess
+            out_["site_solve_ms"] = r["bind_ms"] + c_["solve_s"] * 1e3
+        return out_
 
     out = {
         "metric": METRIC,
@@ -323,12 +422,13 @@ def main():
         "executed_flops_per_sigma": plans[prec_name].exec_flops,
         "plan_build_s": t_plan,
         "schedule": {k: asdict(v[0]) for k, v in tuned.items()},
+        "plan_keys": {k: plan_key(k, p_) for k, p_ in plans.items()},
         "autotune": None if a.no_autotune else {k: table_json(v[1][:5]) for k, v in tuned.items()},
-        "fp64": summary(r64),
-        **{k: summary(res[k]) for k in LOW},
+        "fp64": summary("fp64"),
+        **{k: summary(k) for k in LOW},
         **{f"{k}_speedup_vs_fp64": r64["dt"] / res[k]["dt"] for k in LOW},
         **{f"delta_E_{k}_vs_fp64_Eh": None if dE is None else dE[k] for k in LOW},
-        "converged_solves": conv,
+        "min_bytes_per_sigma": sp.min_bytes,
         **{f"{k}_sigma_max_rel_err": sig_err[k] for k in LOW},
         "block_davidson": block,
         "parity_max_rel_err_vs_oracle": err,

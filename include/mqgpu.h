@@ -159,9 +159,15 @@ const char *mq_last_error(const mq_ctx *ctx);
  * 128 rows unless the group's M fits 64).  MQ_OPT_REF_FLOPS (1 default / 0):
  * count the reference schedule's FLOPs for mq_plan_flops; with 0 sigma and
  * left-step plans skip that count (it walks the reference schedule once more)
- * and mq_plan_flops reports -1. */
+ * and mq_plan_flops reports -1.  MQ_OPT_SCHEDULE: how sigma plans contract,
+ * MQ_SCHED_LBTM (0, default) the lbtm form with fused columns at every site,
+ * MQ_SCHED_REFERENCE (1) the reference's choice per site
+ * (non-abelian/site_hamil.hpp:35): the rbtm form of site_hamil_rbtm
+ * (site_hamil.hpp:149-204) where (rows - one_rows) >= (cols - one_cols) for
+ * SU2U1, lbtm elsewhere. */
 enum mq_option { MQ_OPT_FUSE = 0, MQ_OPT_FUSE_RATIO = 1, MQ_OPT_FUSE_MIN = 2, MQ_OPT_KCHUNK = 3,
-                 MQ_OPT_GRAPH = 4, MQ_OPT_TILE_M = 5, MQ_OPT_REF_FLOPS = 6 };
+                 MQ_OPT_GRAPH = 4, MQ_OPT_TILE_M = 5, MQ_OPT_REF_FLOPS = 6, MQ_OPT_SCHEDULE = 7 };
+enum mq_schedule { MQ_SCHED_LBTM = 0, MQ_SCHED_REFERENCE = 1 };
 int mq_ctx_set_option(mq_ctx *ctx, int option, int value);
 /* HIP stream (hipStream_t) the context launches on. */
 void *mq_ctx_stream(mq_ctx *ctx);
@@ -173,6 +179,8 @@ int mq_sigma_plan_create(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out);
 int mq_plan_destroy(mq_plan *plan);
 /* Elements of one MPS vector (canonical left-paired layout). */
 int64_t mq_plan_vec_len(const mq_plan *plan);
+/* Contraction form of a sigma plan: 0 lbtm, 1 rbtm (see MQ_OPT_SCHEDULE). */
+int mq_plan_schedule(const mq_plan *plan);
 /* Algorithmic FLOPs of the reference schedule (2mnk per block GEMM + 2 len
  * per axpy, lbtm/rbtm chosen as site_hamil.hpp:35) and minimum HBM bytes. */
 int mq_plan_flops(const mq_plan *plan, double *flops, double *bytes);
@@ -203,6 +211,23 @@ int mq_sigma_graph(mq_plan *plan, const double *L, const double *R,
                    const double *x, double *y, int prec);
 /* Drop the rearranged boundaries (call after changing L or R in place). */
 int mq_plan_rebind(mq_plan *plan);
+/* Rearrange the boundaries L, R into the plan's GEMM panels now and make them
+ * the current binding (asynchronous on the plan's stream).  The implicit
+ * binding of mq_sigma and the solvers is keyed on the (L, R) pointer pair
+ * only, so a caller that frees a boundary and allocates a new one (which may
+ * land at the same address) calls mq_plan_bind (or mq_plan_rebind) before the
+ * next sigma.  Stands in for the boundary arguments of site_hamil2
+ * (non-abelian/engine.hpp:198), which the reference reads afresh every call. */
+int mq_plan_bind(mq_plan *plan, const double *L, const double *R);
+/* mask (host, vec_len bytes, written): 1 for the coordinates of the
+ * left-paired vector that H_eff couples, 0 for those whose row and column of
+ * H_eff are structurally zero (no MPO term reaches that slot of the reduced
+ * two-site tensor: e.g. the spin-1 pair of two singly occupied sites next to a
+ * spin-0 bond state, which ts_reduction.h:103-105 never fills and
+ * apply_op.hpp:75 filters).  Start vectors of an eigensolver that are not
+ * built from an MPS must vanish there, or the zero eigenvalues of those
+ * coordinates become roots.  Sigma plans only. */
+int mq_plan_live_mask(const mq_plan *plan, uint8_t *mask);
 /* Instrumented mq_sigma (no reference counterpart): HIP events around each
  * stage on the plan's stream.  ms[0] boundary arrangement, ms[1] reshape,
  * ms[2] T GEMM, ms[3] W application, ms[4] output GEMM.  work[0..2] are the
@@ -289,6 +314,15 @@ int mq_block_davidson(mq_plan *plan, const double *L, const double *R,
                       int prec, const double *const *ortho, int n_ortho,
                       int *iterations);
 
+/* The last eigensolver call on this plan (no reference counterpart: the
+ * record of a precision form's two phases): info[0] iterations in the
+ * requested form, info[1] FP64 continuation iterations that followed them,
+ * info[2] the form's last Ritz value (lowest root of a block solve), info[3]
+ * the FP64 Rayleigh quotient of that Ritz vector (the defect-corrected value
+ * before any continuation).  For MQ_PREC_FP64 calls info[1] = 0 and
+ * info[2] = info[3] = the returned eigenvalue. */
+int mq_plan_solve_info(const mq_plan *plan, double info[4]);
+
 /* Free the work arrays a plan keeps between calls (batched task lists and
  * arenas, Davidson vectors, the captured sigma graph); they are rebuilt by the
  * next call that needs them.  No reference counterpart: for callers that hold
@@ -316,16 +350,18 @@ int mq_gemm_f32(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
  * task lists checked) on machines without a GPU.  kind: 0 sigma, 1 left
  * step, 2 right step, 3 diagonal, plus MQ_HOST_FUSE(v), MQ_HOST_FUSE_RATIO(r)
  * and MQ_HOST_FUSE_MIN(m) to build with those MQ_OPT_FUSE* values (0 keeps a
- * default).  Arrays stay owned by the handle. */
+ * default), and MQ_HOST_SCHED_REFERENCE for the reference's per-site schedule.  Arrays stay owned by the handle. */
 #define MQ_HOST_FUSE(v) (((v) + 1) << 8)
 #define MQ_HOST_FUSE_RATIO(r) ((r) << 10) /* 1..255 */
 #define MQ_HOST_FUSE_MIN(m) ((m) << 18)   /* 1..4095 */
+#define MQ_HOST_SCHED_REFERENCE (1 << 30) /* sigma plans with MQ_SCHED_REFERENCE */
 typedef struct mq_host_plan mq_host_plan;
 enum mq_plan_part {
     MQ_PART_ARR_L = 0, MQ_PART_ARR_R = 1, MQ_PART_RESHAPE = 2, MQ_PART_G1_GROUPS = 3,
     MQ_PART_G1_TILES = 4, MQ_PART_STRIPS = 5, MQ_PART_SRCS = 6, MQ_PART_G2_GROUPS = 7,
     MQ_PART_G2_TILES = 8, MQ_PART_SCATTER = 9, MQ_PART_COMB_STRIPS = 10, MQ_PART_COMB_SRCS = 11,
-    MQ_PART_G1_ROWMAP = 12, MQ_PART_G2_ROWMAP = 13
+    MQ_PART_G1_ROWMAP = 12, MQ_PART_G2_ROWMAP = 13,
+    MQ_PART_LIVE = 14 /* sigma plans: the mq_plan_live_mask bytes */
 };
 int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out);
 int mq_host_plan_part(const mq_host_plan *hp, int part, const void **ptr, int64_t *count, int64_t *elem_bytes);
@@ -333,6 +369,8 @@ int mq_host_plan_part(const mq_host_plan *hp, int part, const void **ptr, int64_
 int mq_host_plan_sizes(const mq_host_plan *hp, int64_t sizes[10]);
 int mq_host_plan_flops(const mq_host_plan *hp, double *ref_flops, double *exec_flops, double *min_bytes);
 int mq_host_plan_out_desc(const mq_host_plan *hp, mq_boundary_desc *out);
+/* 0 lbtm, 1 rbtm (sigma plans; see MQ_OPT_SCHEDULE). */
+int mq_host_plan_schedule(const mq_host_plan *hp);
 int mq_host_plan_destroy(mq_host_plan *hp);
 const char *mq_host_last_error(void);
 

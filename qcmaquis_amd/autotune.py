@@ -28,12 +28,14 @@ class Schedule:
     min_entries: int = 64
     kchunk: int = 2048
     tile_m: int = 0
+    sched: int = 0  # MQ_OPT_SCHEDULE: 0 lbtm everywhere, 1 the reference's lbtm/rbtm choice
 
     def label(self):
         tm = f" tile_m={self.tile_m}" if self.tile_m else ""
+        rule = " sched=reference" if self.sched else ""
         if self.fuse != 1:
-            return f"fuse={self.fuse} kchunk={self.kchunk}{tm}"
-        return f"fuse=1 ratio={self.ratio} min={self.min_entries} kchunk={self.kchunk}{tm}"
+            return f"fuse={self.fuse} kchunk={self.kchunk}{tm}{rule}"
+        return f"fuse=1 ratio={self.ratio} min={self.min_entries} kchunk={self.kchunk}{tm}{rule}"
 
 
 DEFAULT = Schedule()
@@ -81,9 +83,12 @@ def time_schedule(engine, symm, psi, L, R, W, sched, x, prec, reps=5, warmup=2):
         sp.close()
 
 
-def tune(engine, symm, psi, L, R, W, prec=MQ_PREC_FP64, cands=None, reps=5, log=None):
+def tune(engine, symm, psi, L, R, W, prec=MQ_PREC_FP64, cands=None, reps=5, log=None, margin=0.05):
     """Fastest schedule for this site problem in precision `prec`, with the
-    timing table [(schedule, ms, exec_flops, workspace_bytes)] sorted by time."""
+    timing table [(schedule, ms, exec_flops, workspace_bytes)] sorted by time.
+    The default schedule is kept unless another is faster by more than
+    `margin` (a fraction of the default's time): candidates within timing
+    noise of it would otherwise change the pick from run to run."""
     cache = engine.__dict__.setdefault("tuned", {})
     key = (site_key(symm, psi, L, R, W), prec)
     if key in cache and cands is None:
@@ -96,7 +101,11 @@ def tune(engine, symm, psi, L, R, W, prec=MQ_PREC_FP64, cands=None, reps=5, log=
         if log:
             log(f"  schedule {s.label()}: {ms:.3f} ms, executed {ex / 1e9:.1f} GF, workspace {ws / 1e9:.2f} GB")
     table.sort(key=lambda r: r[1])
-    res = (table[0][0], table)
+    best = table[0][0]
+    dflt = [r for r in table if r[0] == DEFAULT]
+    if dflt and table[0][1] > (1.0 - margin) * dflt[0][1]:
+        best = DEFAULT
+    res = (best, table)
     cache[key] = res
     return res
 

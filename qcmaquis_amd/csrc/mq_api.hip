@@ -20,6 +20,7 @@ struct mq_ctx {
     int tile_m = 0;     // MQ_OPT_TILE_M (0: automatic)
     int ref_flops = 1;  // MQ_OPT_REF_FLOPS
     int graph = 0;      // MQ_OPT_GRAPH
+    int schedule = SCHED_LBTM;  // MQ_OPT_SCHEDULE
 };
 
 namespace {
@@ -122,15 +123,15 @@ struct mq_plan {
     // several vectors per launch (mq_sigma_batch): replicated task lists per batch
     // size, and per-vector copies of the psi, T and C arenas for the largest size used
     struct BatchLists {  // the W application reuses the plan's own strips for every vector
-        int64_t ncopies = 0, ws = 0;
-        DevArray<Copy2D> reshape;
+        int64_t ncopies = 0, nscatter = 0, ws = 0;
+        DevArray<Copy2D> reshape, scatter;  // scatter: rbtm plans' right-to-left reshape
         DevGemm g1, g2;
     };
     std::map<int, std::unique_ptr<BatchLists>> blists;
     struct BatchArenas {
         int cap = 0;
         int64_t ws = 0;
-        DevArray<double> prp, t, c, ws64;
+        DevArray<double> prp, t, c, ws64, tmp;  // tmp: rbtm plans' right-paired sigma
         DevArray<float> prp32, t32, c32, ws32;
         int cap32 = 0;
     } barena;
@@ -139,6 +140,10 @@ struct mq_plan {
     int gprec = -1;
     const double *gL = nullptr, *gR = nullptr;
     DevArray<double> gx, gy;
+    // last eigensolver call (mq_plan_solve_info): iterations in the requested form,
+    // FP64 continuation iterations, the form's last Ritz value (lowest root) and the
+    // FP64 Rayleigh quotient of its Ritz vector
+    double solve_info[4] = {0, 0, 0, 0};
     ~mq_plan() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (h_coef) (void)hipHostFree(h_coef);
@@ -208,6 +213,10 @@ int mq_ctx_set_option(mq_ctx *ctx, int option, int value) {
         ctx->graph = value;
         return MQ_OK;
     }
+    if (option == MQ_OPT_SCHEDULE && (value == SCHED_LBTM || value == SCHED_REFERENCE)) {
+        ctx->schedule = value;
+        return MQ_OK;
+    }
     ctx->err = "mq_ctx_set_option: unknown option or value";
     return MQ_ERR_ARG;
 }
@@ -231,7 +240,8 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         std::unique_ptr<mq_plan> p(new mq_plan());
         p->ctx = ctx;
         const int tm = ctx->tile_m, tn = 0, kc = ctx->kchunk;
-        if (kind == PLAN_SIGMA) p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse, ctx->ref_flops != 0);
+        if (kind == PLAN_SIGMA)
+            p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse, ctx->ref_flops != 0, ctx->schedule);
         else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse, ctx->ref_flops != 0);
         else if (kind == PLAN_RSTEP) p->hp = build_rstep_plan(*desc, tm, tn, kc);
         else p->hp = build_diag_plan(*desc, tm, tn, kc);
@@ -353,8 +363,42 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
     auto mark = [&](int i) {
         if (ev) check(hipEventRecord(ev[i], s), "record");
     };
+    if (h.rbtm) {
+        // rbtm schedule: t = X RC, W application into BUF, S = LA^T BUF, S -> y
+        if (prec == MQ_PREC_FP64) {
+            check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s, true), "copy psi");
+            mark(2);
+            check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->prp.p,
+                              p->rb.p, p->t.p, p->ws.p, p->g1.rowmap.p, ST_SIGMA, s),
+                  "gemm t");
+            mark(3);
+            check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, p->t.p, p->c.p, ST_SIGMA, s),
+                  "W apply");
+            mark(4);
+            check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->la.p,
+                              p->c.p, p->tmp.p, p->ws.p, p->g2.rowmap.p, ST_SIGMA_OUT, s),
+                  "gemm L");
+        } else {
+            check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp32.p, s, true), "copy psi");
+            mark(2);
+            check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->prp32.p,
+                              p->rb32.p, p->t32.p, p->ws32.p, p->g1.rowmap.p, ST_SIGMA, s, x3),
+                  "gemm t");
+            mark(3);
+            check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, p->t32.p, p->c32.p, ST_SIGMA,
+                                s),
+                  "W apply");
+            mark(4);
+            check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->la32.p,
+                              p->c32.p, p->tmp.p, p->ws32.p, p->g2.rowmap.p, ST_SIGMA_OUT, s, x3),
+                  "gemm L");
+        }
+        check(launch_copy2d(p->scatter.p, (int)h.scatter.size(), p->tmp.p, y, s, true), "reshape sigma");
+        mark(5);
+        return;
+    }
     if (prec == MQ_PREC_FP64) {
-        check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s), "reshape");
+        check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s, true), "reshape");
         mark(2);
         check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la.p,
                           p->prp.p, p->t.p, p->ws.p, p->g1.rowmap.p, ST_SIGMA, s),
@@ -367,7 +411,7 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
                           p->rb.p, y, p->ws.p, p->g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
     } else {
-        check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp32.p, s), "reshape");
+        check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp32.p, s, true), "reshape");
         mark(2);
         check(launch_gemm(p->g1.groups.p, p->g1.tiles.p, p->g1.ntiles, p->g1.split.p, p->g1.nsplit, p->la32.p,
                           p->prp32.p, p->t32.p, p->ws32.p, p->g1.rowmap.p, ST_SIGMA, s, x3),
@@ -400,8 +444,13 @@ static mq_plan::BatchLists &batch_setup(mq_plan *p, int nvec, bool f32) {
         std::unique_ptr<mq_plan::BatchLists> B(new mq_plan::BatchLists());
         B->reshape.upload(batch_copies(h.reshape, nvec, h.vec_len, h.n_prp), s);
         B->ncopies = (int64_t)h.reshape.size() * nvec;
-        GemmList g1 = batch_gemm(h.g1, nvec, 0, h.n_prp, h.n_t);
-        GemmList g2 = batch_gemm(h.g2, nvec, h.n_c, 0, h.vec_len);
+        // per-vector operands advance by their arena size; shared ones (panels) by 0
+        GemmList g1 = h.rbtm ? batch_gemm(h.g1, nvec, h.n_prp, 0, h.n_t) : batch_gemm(h.g1, nvec, 0, h.n_prp, h.n_t);
+        GemmList g2 = h.rbtm ? batch_gemm(h.g2, nvec, 0, h.n_c, h.n_tmp) : batch_gemm(h.g2, nvec, h.n_c, 0, h.vec_len);
+        if (h.rbtm) {
+            B->scatter.upload(batch_copies(h.scatter, nvec, h.n_tmp, h.vec_len), s);
+            B->nscatter = (int64_t)h.scatter.size() * nvec;
+        }
         B->g1.upload(g1, s);
         B->g2.upload(g2, s);
         B->ws = std::max(g1.ws, g2.ws);
@@ -415,6 +464,7 @@ static mq_plan::BatchLists &batch_setup(mq_plan *p, int nvec, bool f32) {
         A.prp.alloc((size_t)h.n_prp * cap);
         A.t.alloc((size_t)h.n_t * cap);
         A.c.alloc((size_t)h.n_c * cap);
+        A.tmp.alloc((size_t)h.n_tmp * cap);
         A.ws = std::max(A.ws, slot->ws);
         A.ws64.alloc((size_t)A.ws);
         A.cap = cap;
@@ -447,8 +497,36 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
         DevArray<float> &prp32, &t32, &c32, &ws32;
     } B{Bl.reshape, Bl.g1, Bl.g2, Bl.ncopies, A.prp, A.t, A.c, A.ws64, A.prp32, A.t32, A.c32, A.ws32};
     HostPlan &h = p->hp;
+    if (h.rbtm) {
+        const bool x3 = prec == MQ_PREC_BF16X3;
+        if (prec == MQ_PREC_FP64) {
+            check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s, true), "copy psi");
+            check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, B.prp.p, p->rb.p,
+                              B.t.p, B.ws.p, B.g1.rowmap.p, ST_SIGMA, s),
+                  "gemm t");
+            check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, B.t.p, B.c.p, ST_SIGMA, s, nvec,
+                                h.n_t, h.n_c),
+                  "W apply");
+            check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, p->la.p, B.c.p,
+                              A.tmp.p, B.ws.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
+                  "gemm L");
+        } else {
+            check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp32.p, s, true), "copy psi");
+            check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, B.prp32.p,
+                              p->rb32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s, x3),
+                  "gemm t");
+            check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s,
+                                nvec, h.n_t, h.n_c),
+                  "W apply");
+            check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, p->la32.p,
+                              B.c32.p, A.tmp.p, B.ws32.p, B.g2.rowmap.p, ST_SIGMA_OUT, s, x3),
+                  "gemm L");
+        }
+        check(launch_copy2d(Bl.scatter.p, (int)Bl.nscatter, A.tmp.p, Y, s, true), "reshape sigma");
+        return;
+    }
     if (prec == MQ_PREC_FP64) {
-        check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s), "reshape");
+        check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp.p, s, true), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la.p, B.prp.p,
                           B.t.p, B.ws.p, B.g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
@@ -459,7 +537,7 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
                           B.ws.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
               "gemm R");
     } else {
-        check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp32.p, s), "reshape");
+        check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp32.p, s, true), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la32.p,
                           B.prp32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s, prec == MQ_PREC_BF16X3),
               "gemm T");
@@ -590,8 +668,15 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
     try {
         hipStream_t s = plan->ctx->stream;
         HostPlan &h = plan->hp;
-        hipEvent_t ev[6];
-        for (auto &e : ev) check(hipEventCreate(&e), "event");
+        struct Events {  // destroyed on every exit path
+            hipEvent_t e[6] = {};
+            ~Events() {
+                for (auto &x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } evs;
+        hipEvent_t *ev = evs.e;
+        for (int i = 0; i < 6; ++i) check(hipEventCreate(&ev[i]), "event");
         check(hipEventRecord(ev[0], s), "record");
         bind(plan, L, R, s);
         if (prec != MQ_PREC_FP64) bind32(plan, s);
@@ -599,7 +684,6 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
         sigma_stages(plan, x, y, prec, ev);
         check(hipEventSynchronize(ev[5]), "sync");
         for (int i = 0; i < 5; ++i) check(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]), "elapsed");
-        for (auto &e : ev) (void)hipEventDestroy(e);
         if (work) {
             const double es = prec == MQ_PREC_FP64 ? 8.0 : 4.0;  // bytes per stored element
             const double srcel = plan->w_src_el, dstel = plan->w_dst_el;
@@ -609,9 +693,14 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
             // bytes each stage must move: T GEMM reads LA + PRP and writes T; the W
             // application reads every source slice and writes C; the output GEMM
             // reads C + RB and writes y (FP64)
-            work[3] = es * (h.n_la + h.n_prp + h.n_t);
             work[4] = es * (srcel + dstel);
-            work[5] = es * (h.n_c + h.n_rb) + 8.0 * h.vec_len;
+            if (h.rbtm) {  // t GEMM reads psi + RB; L GEMM reads LA + BUF and writes S (FP64), then S -> y
+                work[3] = es * (h.n_prp + h.n_rb + h.n_t);
+                work[5] = es * (h.n_c + h.n_la) + 8.0 * (2.0 * h.n_tmp + h.vec_len);
+            } else {
+                work[3] = es * (h.n_la + h.n_prp + h.n_t);
+                work[5] = es * (h.n_c + h.n_rb) + 8.0 * h.vec_len;
+            }
         }
         return MQ_OK;
     } catch (std::exception &e) {
@@ -691,6 +780,40 @@ int mq_plan_rebind(mq_plan *plan) {
     if (!plan) return MQ_ERR_ARG;
     plan->bound_L = plan->bound_R = nullptr;
     plan->gL = plan->gR = nullptr;
+    return MQ_OK;
+}
+
+// Arrange L and R into the plan's panels now, whatever was bound before (a new
+// boundary may sit at the address of a freed one), and make them the binding
+// that later calls with the same pointers reuse.
+int mq_plan_bind(mq_plan *plan, const double *L, const double *R) {
+    if (!plan || !L || !R) return MQ_ERR_ARG;
+    if (plan->hp.kind != PLAN_SIGMA) {
+        plan->ctx->err = "mq_plan_bind: plan is not a sigma plan";
+        return MQ_ERR_ARG;
+    }
+    try {
+        check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+        plan->bound_L = plan->bound_R = nullptr;
+        plan->gL = plan->gR = nullptr;
+        bind(plan, L, R, plan->ctx->stream);
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(plan->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+int mq_plan_schedule(const mq_plan *plan) { return plan ? plan->hp.rbtm : MQ_ERR_ARG; }
+
+int mq_plan_solve_info(const mq_plan *plan, double info[4]) {
+    if (!plan || !info) return MQ_ERR_ARG;
+    for (int i = 0; i < 4; ++i) info[i] = plan->solve_info[i];
+    return MQ_OK;
+}
+
+int mq_plan_live_mask(const mq_plan *plan, uint8_t *mask) {
+    if (!plan || !mask || plan->hp.kind != PLAN_SIGMA) return MQ_ERR_ARG;
+    std::copy(plan->hp.live.begin(), plan->hp.live.end(), mask);
     return MQ_OK;
 }
 
@@ -1082,6 +1205,8 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
                 break;
             jd.lincomb({uA, u}, {-1.0, rr[1] / rr[2]}, 0.0, V(it));
         }
+        const double theta_form = theta;
+        const int its_form = it;
         bool refine = false;
         if (prec != MQ_PREC_FP64) {
             // FP64 defect correction: the eigenvalue is the FP64 Rayleigh quotient of the
@@ -1098,6 +1223,7 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
                 refine = !(rn <= tol * std::fabs(theta) || rn < tol);
             }
         }
+        const double theta_rq = theta;
         check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
         jd.sync();
         if (refine) {
@@ -1107,6 +1233,10 @@ extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *
             if (rc != MQ_OK) return rc;
             it += its2;
         }
+        plan->solve_info[0] = its_form;
+        plan->solve_info[1] = it - its_form;
+        plan->solve_info[2] = theta_form;
+        plan->solve_info[3] = theta_rq;
         *theta_out = theta;
         if (iterations) *iterations = it;
         return MQ_OK;
@@ -1283,6 +1413,8 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
             // t = (r - (Vcpy.r)/(u.Vcpy) u) / (theta - D), normalised by the next pass
             check(launch_diag_scale(r, u, rr[1] / rr[2], diag, theta, V(dim), n, jd.s), "precondition");
         }
+        const double theta_form = theta;
+        const int its_form = it;
         bool refine = false;
         if (prec != MQ_PREC_FP64) {
             // FP64 defect correction and, when the FP64 residual misses the tolerance,
@@ -1296,6 +1428,7 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
                 refine = !(rn <= tol * std::fabs(theta) || rn < tol);
             }
         }
+        const double theta_rq = theta;
         check(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, jd.s), "u out");
         jd.sync();
         if (refine) {
@@ -1305,6 +1438,10 @@ extern "C" int mq_davidson(mq_plan *plan, const double *L, const double *R, cons
             if (rc != MQ_OK) return rc;
             it += its2;
         }
+        plan->solve_info[0] = its_form;
+        plan->solve_info[1] = it - its_form;
+        plan->solve_info[2] = theta_form;
+        plan->solve_info[3] = theta_rq;
         *theta_out = theta;
         if (iterations) *iterations = it;
         return MQ_OK;
@@ -1523,6 +1660,8 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
                 }
             }
         }
+        const double theta_form = theta[0];
+        const int its_form = it;
         bool refine = false;
         if (prec != MQ_PREC_FP64) {
             // FP64 defect correction per root: Rayleigh quotients from one batched FP64
@@ -1549,6 +1688,7 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
                 }
             }
         }
+        const double theta_rq = theta[0];
         check(hipMemcpyAsync(U, Ur(0), sizeof(double) * n * k, hipMemcpyDeviceToDevice, jd.s), "U out");
         jd.sync();
         if (refine) {
@@ -1558,6 +1698,10 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
             if (rc != MQ_OK) return rc;
             it += its2;
         }
+        plan->solve_info[0] = its_form;
+        plan->solve_info[1] = it - its_form;
+        plan->solve_info[2] = theta_form;
+        plan->solve_info[3] = theta_rq;
         for (int r = 0; r < k; ++r) theta_out[r] = theta[r];
         if (iterations) *iterations = it;
         return MQ_OK;
@@ -1587,7 +1731,8 @@ int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out) 
         if (fm) fuse.mode = fm - 1;
         if (fr) fuse.ratio = fr;
         if (fmin) fuse.min_entries = fmin;
-        if (k == PLAN_SIGMA) h->hp = build_sigma_plan(*desc, tm, tn, kc, fuse);
+        const int sched = (kind >> 30) & 1;
+        if (k == PLAN_SIGMA) h->hp = build_sigma_plan(*desc, tm, tn, kc, fuse, true, sched);
         else if (k == PLAN_LSTEP) h->hp = build_lstep_plan(*desc, tm, tn, kc, fuse);
         else if (k == PLAN_RSTEP) h->hp = build_rstep_plan(*desc, tm, tn, kc);
         else if (k == PLAN_DIAG) h->hp = build_diag_plan(*desc, tm, tn, kc);
@@ -1618,6 +1763,7 @@ int mq_host_plan_part(const mq_host_plan *h, int part, const void **ptr, int64_t
         case MQ_PART_COMB_SRCS: *ptr = p.combL.srcs.data(); *count = p.combL.srcs.size(); *elem = sizeof(Src); break;
         case MQ_PART_G1_ROWMAP: *ptr = p.g1.rowmap.data(); *count = p.g1.rowmap.size(); *elem = sizeof(RowMap); break;
         case MQ_PART_G2_ROWMAP: *ptr = p.g2.rowmap.data(); *count = p.g2.rowmap.size(); *elem = sizeof(RowMap); break;
+        case MQ_PART_LIVE: *ptr = p.live.data(); *count = p.live.size(); *elem = 1; break;
         default: return MQ_ERR_ARG;
     }
     return MQ_OK;
@@ -1653,6 +1799,8 @@ int mq_host_plan_out_desc(const mq_host_plan *h, mq_boundary_desc *out) {
     out->total = p.n_out;
     return MQ_OK;
 }
+
+int mq_host_plan_schedule(const mq_host_plan *h) { return h ? h->hp.rbtm : MQ_ERR_ARG; }
 
 int mq_host_plan_destroy(mq_host_plan *h) {
     delete h;

@@ -13,7 +13,7 @@ namespace mq {
 // ------------------------------------------------------------------ copy / scatter
 // One workgroup per task; element e -> (i = e % rows, j = e / rows).  The
 // destination may be of a narrower type (FP64 -> FP32 conversion).
-template <class TS, class TD>
+template <int STAGE, class TS, class TD>
 __global__ void __launch_bounds__(256) k_copy2d(const Copy2D *__restrict__ tasks, const TS *__restrict__ src,
                                                 TD *__restrict__ dst) {
     const Copy2D t = tasks[blockIdx.x];
@@ -527,7 +527,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(X3 ? 3
 
 // C = sum over splits of the partial tiles, in split order (deterministic),
 // accumulated in FP64.
-template <class T, class TO>
+template <int STAGE, class T, class TO>
 __global__ void __launch_bounds__(256) k_split_reduce(const GemmGroup *__restrict__ groups,
                                                       const int *__restrict__ gids, const T *__restrict__ W,
                                                       TO *__restrict__ C) {
@@ -573,15 +573,20 @@ __global__ void __launch_bounds__(256) k_lincomb(const double *const *__restrict
 }
 
 // ------------------------------------------------------------------ launchers
-hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *dst, hipStream_t s) {
+template <class TD>
+static hipError_t copy2d_launch(const Copy2D *tasks, int n, const double *src, TD *dst, hipStream_t s, bool sigma) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_copy2d<double, double>), dim3(n), dim3(256), 0, s, tasks, src, dst);
+    if (sigma)
+        hipLaunchKernelGGL((k_copy2d<ST_SIGMA, double, TD>), dim3(n), dim3(256), 0, s, tasks, src, dst);
+    else
+        hipLaunchKernelGGL((k_copy2d<ST_MISC, double, TD>), dim3(n), dim3(256), 0, s, tasks, src, dst);
     return hipGetLastError();
 }
-hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_copy2d<double, float>), dim3(n), dim3(256), 0, s, tasks, src, dst);
-    return hipGetLastError();
+hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *dst, hipStream_t s, bool sigma) {
+    return copy2d_launch(tasks, n, src, dst, s, sigma);
+}
+hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s, bool sigma) {
+    return copy2d_launch(tasks, n, src, dst, s, sigma);
 }
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
@@ -647,8 +652,15 @@ static hipError_t gemm_launch(const GemmGroup *groups, const GemmTile *tiles, in
         case ST_STEP_OUT: gemm_stage<ST_STEP_OUT, T, TO, X3>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
         default: gemm_stage<ST_MISC, T, TO, X3>(groups, tiles, ntiles, A, B, C, W, rowmap, s); break;
     }
-    if (nsplit > 0)
-        hipLaunchKernelGGL((k_split_reduce<T, TO>), dim3(64, nsplit), dim3(256), 0, s, groups, split_gids, W, C);
+    if (nsplit > 0) {
+        // the sigma's reduction has its own instantiation (profiles attribute it per stage)
+        if (stage == ST_SIGMA_OUT)
+            hipLaunchKernelGGL((k_split_reduce<ST_SIGMA_OUT, T, TO>), dim3(64, nsplit), dim3(256), 0, s, groups,
+                               split_gids, W, C);
+        else
+            hipLaunchKernelGGL((k_split_reduce<ST_MISC, T, TO>), dim3(64, nsplit), dim3(256), 0, s, groups,
+                               split_gids, W, C);
+    }
     return hipGetLastError();
 }
 

@@ -13,8 +13,11 @@ namespace mq {
 // that profiles attribute time per stage.
 enum Stage { ST_SIGMA = 0, ST_SIGMA_OUT = 1, ST_STEP = 2, ST_STEP_OUT = 3, ST_FUSE = 4, ST_MISC = 5 };
 
-hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *dst, hipStream_t s);
-hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s);
+// sigma: the instantiation of the sigma's psi reshape (its own kernel name in profiles)
+hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, double *dst, hipStream_t s,
+                         bool sigma = false);
+hipError_t launch_copy2d(const Copy2D *tasks, int n, const double *src, float *dst, hipStream_t s,
+                         bool sigma = false);
 hipError_t launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s);
 // one workgroup per pack of strips (StripList::packs, npacks = packs.size() - 1)
 // nvec > 1: the same task list for nvec vectors, vector v at T + v * src_stride and

@@ -927,7 +927,12 @@ GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step
 }
 
 // ============================================================ sigma
-HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse, bool ref_count) {
+HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRule fuse, bool ref_count,
+                          int schedule) {
+    if (schedule == SCHED_REFERENCE && d.symm == MQ_SYMM_SU2U1) {
+        Mpo w = to_mpo(d.mpo);
+        if ((w.rows - w.one_rows) >= (w.cols - w.one_cols)) return build_sigma_plan_rbtm(d, tm, tn, kc, ref_count);
+    }
     Site s(d);
     BLayout L = to_layout(d.L), R = to_layout(d.R);
     if (L.aux != s.W.rows || R.aux != s.W.cols) throw std::runtime_error("boundary aux_dim does not match the MPO");
@@ -977,6 +982,20 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
         ct += (int64_t)xb.rows * K[r];
         rb_off[r] = rbt;
         rbt += (int64_t)K[r] * xb.cols;
+    }
+    // the coordinates H_eff couples: every row that a strip with a nonzero source
+    // writes, in every column of its output block (strip dst is still relative to
+    // its al block here: koff * rows + row0)
+    P.live.assign((size_t)s.X.total, 0);
+    for (size_t i = 0; i < P.wapply.strips.size(); ++i) {
+        const Strip &st = P.wapply.strips[i];
+        bool any = false;
+        for (int k = st.s0; k < st.s1 && !any; ++k) any = P.wapply.srcs[k].coef != 0.0;
+        if (!any) continue;
+        const VBlk &xb = s.X.blk[s.x_of_r[strip_al[i]]];
+        const int64_t row0 = st.dst % xb.rows;
+        for (int c = 0; c < xb.cols; ++c)
+            std::fill_n(P.live.begin() + xb.off + (int64_t)c * xb.rows + row0, st.rows, (uint8_t)1);
     }
     patch_strips(P.wapply, strip_al, c_off);
     P.n_c = ct;
@@ -1068,6 +1087,20 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
         tmp_off[r] = tt;
         tt += (int64_t)K[r] * xb.cols;
     }
+    // the coordinates H_eff couples: every row that a strip with a nonzero source
+    // writes, in every column of its output block (strip dst is still relative to
+    // its al block here: koff * rows + row0)
+    P.live.assign((size_t)s.X.total, 0);
+    for (size_t i = 0; i < P.wapply.strips.size(); ++i) {
+        const Strip &st = P.wapply.strips[i];
+        bool any = false;
+        for (int k = st.s0; k < st.s1 && !any; ++k) any = P.wapply.srcs[k].coef != 0.0;
+        if (!any) continue;
+        const VBlk &xb = s.X.blk[s.x_of_r[strip_al[i]]];
+        const int64_t row0 = st.dst % xb.rows;
+        for (int c = 0; c < xb.cols; ++c)
+            std::fill_n(P.live.begin() + xb.off + (int64_t)c * xb.rows + row0, st.rows, (uint8_t)1);
+    }
     patch_strips(P.wapply, strip_al, c_off);
     P.n_c = ct;
     P.n_tmp = tt;
@@ -1130,27 +1163,36 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     return P;
 }
 
-// ============================================================ right step
+// ============================================================ rbtm core
+// Shared by the right step and the rbtm sigma: the MPSBoundaryProduct GEMMs
+// t[b2] = X * Rop[b2] (boundary_times_mps.hpp:226-293) and the rbtm W
+// application (apply_op.hpp:114-209) into BUF_or, which stacks the rows (b1, ol)
+// over the right-paired columns of the output left charge or.
 namespace {
 struct RTB {  // block (c, c') of t[b2] = X_c Rop[b2](c, c')
     int c, cp;      // right-index ids
     int64_t off;    // offset of column 0 inside the t arena
     int ld;         // rows of X_c
 };
-}  // namespace
+struct RowBlk {
+    int b1, ol, orr, row;
+};
+struct RbtmCore {
+    std::vector<int> Mr;           // [l] rows of BUF_l
+    std::vector<RowBlk> rowblks;   // in BUF row order per l
+    std::vector<int64_t> buf_off;  // [l] offset of BUF_l in the C arena (-1: none)
+    double axf = 0;                // axpy FLOPs of the reference's tasks
+};
 
-HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
-    Site s(d);
-    BLayout R = to_layout(d.R);
-    if (R.aux != s.W.cols) throw std::runtime_error("right boundary aux_dim does not match the MPO");
-    HostPlan P;
-    P.kind = PLAN_RSTEP;
-    P.symm = s.symm;
-    P.vec_len = s.X.total;
-    PRP prp = prp_layout(s, P.reshape);
-    P.n_prp = prp.total;
-    // MPSBoundaryProduct (boundary_times_mps.hpp:226-293): t[b2] = X * Rop[b2] right-trimmed to X's
-    // right charges; RC_c stacks Rop[b2](c, c') horizontally over (b2, c').
+// g1 reads the X blocks at their canonical offsets in the A arena (psi itself for
+// the right step, its copy for the sigma).  sigma: the sigma's rbtm (every MPO
+// row, hermitian-partner rows included; a (b1, ol, or) group only when L[b1] has
+// the block (ket ol, bra or), as rbtm_axpy_gemm skips the others); otherwise the
+// right step (rows whose output boundary slice is stored).
+RbtmCore rbtm_core(const Site &s, const BLayout &R, const BLayout *Lsig, HostPlan &P, int tm, int tn, int kc) {
+    RbtmCore out;
+    // MPSBoundaryProduct: t[b2] = X * Rop[b2] right-trimmed to X's right charges;
+    // RC_c stacks Rop[b2](c, c') horizontally over (b2, c').
     struct Rop {
         int b2, c, cp, col;
         const BBlk *blk;
@@ -1216,16 +1258,20 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
         int ld;
         double coef;
     };
-    struct RowBlk {
-        int b1, ol, orr, row;
-    };
-    std::vector<int> Mr(s.nl, 0);
-    std::vector<RowBlk> rowblks;
-    std::vector<int> strip_or;
+    out.Mr.assign(s.nl, 0);
     std::vector<RR> raw;
-    double axf = 0;
     for (int b1 = 0; b1 < s.W.rows; ++b1) {
-        if (s.W.lskip(b1)) continue;
+        const bool lskip = s.W.lskip(b1);
+        if (!Lsig && lskip) continue;
+        // (bra, ket) left ids of the L blocks the sigma's charge_gemm can use
+        std::unordered_map<uint64_t, char> lblk;
+        if (Lsig)
+            for (auto &x : Lsig->b[lskip ? s.W.lherm[b1] : b1]) {
+                int p = s.li.get(x.lc), q = s.li.get(x.rc);
+                if (p < 0 || q < 0) continue;
+                int bra = lskip ? p : q, ket = lskip ? q : p;
+                lblk[((uint64_t)bra << 32) | (uint32_t)ket] = 1;
+            }
         raw.clear();
         int a = s.W.lspin[b1];
         for (int b2 : s.W.row_b2[b1]) {
@@ -1243,6 +1289,7 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
                         if (orr < 0 || s.pbr_sz_l[orr] == 0) continue;  // out_right_i
                         int j = spin2(s.left.q[ol]), i = spin2(s.left.q[orr]);
                         if (s.su2 && !tri(j, a, i)) continue;
+                        if (Lsig && !lblk.count(((uint64_t)orr << 32) | (uint32_t)ol)) continue;
                         int lsz = s.lsize(ol), rsz = s.rsize(t.cp);
                         static const std::array<double, 4> one{1.0, 1.0, 1.0, 1.0};
                         const std::array<double, 4> &cpl =
@@ -1252,7 +1299,7 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
                         for (auto &e : wb.e) {
                             raw.push_back({orr, ol, oro + e.col * rsz, t.off + ilo + e.row * lsz, t.ld,
                                            e.coeff * term.scale * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : 1.0)});
-                            axf += 2.0 * lsz * rsz;
+                            out.axf += 2.0 * lsz * rsz;
                         }
                     }
                 }
@@ -1278,11 +1325,12 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
             size_t j = i;
             while (j < raw.size() && raw[j].orr == raw[i].orr && raw[j].ol == raw[i].ol) ++j;
             int orr = raw[i].orr, ol = raw[i].ol;
-            // spin filter of SU2::gemm(ret[b1], transpose(bra_conj), tmp, spin(b1)) (move_boundary.hpp:224)
+            // spin filter of SU2::gemm(ret[b1], transpose(bra_conj), tmp, spin(b1)) (move_boundary.hpp:224);
+            // the sigma's tasks passed the same triangle test (apply_op.hpp:163)
             if (!s.su2 || tri(spin2(s.left.q[ol]), a, spin2(s.left.q[orr]))) {
-                RowBlk rb{b1, ol, orr, Mr[orr]};
-                Mr[orr] += s.lsize(ol);
-                rowblks.push_back(rb);
+                RowBlk rb{b1, ol, orr, out.Mr[orr]};
+                out.Mr[orr] += s.lsize(ol);
+                out.rowblks.push_back(rb);
                 int lsz = s.lsize(ol);
                 Q orq = s.left.q[orr];
                 for (int p = 0; p < s.np; ++p) {
@@ -1292,7 +1340,7 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
                     for (int ss = 0; ss < s.phys.s[p]; ++ss) {
                         int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
                         Strip st;
-                        st.dst = rb.row + (int64_t)col0;  // patched: + buf_off + col0*(M-1)
+                        st.dst = 0;  // set below, once every BUF height is known
                         st.rows = lsz;
                         st.cols = rsz;
                         st.ldd = 0;
@@ -1301,51 +1349,66 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
                             if (raw[q].col0 == col0) P.wapply.srcs.push_back({raw[q].off, raw[q].ld, raw[q].coef});
                         st.s1 = (int)P.wapply.srcs.size();
                         P.wapply.strips.push_back(st);
-                        strip_or.push_back(orr);
                     }
                 }
             }
             i = j;
         }
     }
-    std::vector<int64_t> buf_off(s.nl, -1), tmp_off(s.nl, -1);
-    int64_t bt = 0, mt = 0;
+    out.buf_off.assign(s.nl, -1);
+    int64_t bt = 0;
+    for (int l = 0; l < s.nl; ++l) {
+        if (out.Mr[l] == 0) continue;
+        out.buf_off[l] = bt;
+        bt += (int64_t)out.Mr[l] * s.pbr_sz_l[l];
+    }
+    P.n_c = bt;
+    // strip destinations: row block rb.row of BUF_or, column col0 (leading dimension Mr[or]),
+    // in the order the strips were emitted
+    size_t k = 0;
+    for (auto &rb : out.rowblks) {
+        Q orq = s.left.q[rb.orr];
+        for (int p = 0; p < s.np; ++p) {
+            int r = s.ri.get(orq + s.phys.q[p]);
+            if (r < 0) continue;
+            int rsz = s.rsize(r);
+            for (int ss = 0; ss < s.phys.s[p]; ++ss) {
+                int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
+                Strip &st = P.wapply.strips[k++];
+                st.dst = out.buf_off[rb.orr] + rb.row + (int64_t)col0 * out.Mr[rb.orr];
+                st.ldd = out.Mr[rb.orr];
+            }
+        }
+    }
+    if (k != P.wapply.strips.size()) throw std::runtime_error("rbtm strip bookkeeping");
+    return out;
+}
+}  // namespace
+
+// ============================================================ right step
+HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
+    Site s(d);
+    BLayout R = to_layout(d.R);
+    if (R.aux != s.W.cols) throw std::runtime_error("right boundary aux_dim does not match the MPO");
+    HostPlan P;
+    P.kind = PLAN_RSTEP;
+    P.symm = s.symm;
+    P.vec_len = s.X.total;
+    PRP prp = prp_layout(s, P.reshape);
+    P.n_prp = prp.total;
+    RbtmCore core = rbtm_core(s, R, nullptr, P, tm, tn, kc);
+    const std::vector<int> &Mr = core.Mr;
+    const std::vector<RowBlk> &rowblks = core.rowblks;
+    const std::vector<int64_t> &buf_off = core.buf_off;
+    const double axf = core.axf;
+    std::vector<int64_t> tmp_off(s.nl, -1);
+    int64_t mt = 0;
     for (int l = 0; l < s.nl; ++l) {
         if (Mr[l] == 0) continue;
-        buf_off[l] = bt;
-        bt += (int64_t)Mr[l] * s.pbr_sz_l[l];
         tmp_off[l] = mt;
         mt += (int64_t)Mr[l] * s.lsize(l);
     }
-    P.n_c = bt;
     P.n_tmp = mt;
-    for (size_t k = 0; k < P.wapply.strips.size(); ++k) {
-        Strip &st = P.wapply.strips[k];
-        int orr = strip_or[k];
-        int64_t row = 0, col0 = 0;
-        // st.dst currently row + col0 with row < Mr: recover by construction order
-        (void)row;
-        (void)col0;
-        st.ldd = Mr[orr];
-    }
-    // recompute strip destinations exactly (row offset and column offset were packed additively)
-    {
-        size_t k = 0;
-        for (auto &rb : rowblks) {
-            Q orq = s.left.q[rb.orr];
-            for (int p = 0; p < s.np; ++p) {
-                int r = s.ri.get(orq + s.phys.q[p]);
-                if (r < 0) continue;
-                int rsz = s.rsize(r);
-                for (int ss = 0; ss < s.phys.s[p]; ++ss) {
-                    int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
-                    P.wapply.strips[k].dst = buf_off[rb.orr] + rb.row + (int64_t)col0 * Mr[rb.orr];
-                    ++k;
-                }
-            }
-        }
-        if (k != P.wapply.strips.size()) throw std::runtime_error("rstep strip bookkeeping");
-    }
     for (int l = 0; l < s.nl; ++l) {
         if (Mr[l] == 0) continue;
         GemmGroup g{};
@@ -1399,6 +1462,132 @@ HostPlan build_rstep_plan(const mq_site_desc &d, int tm, int tn, int kc) {
     order_tiles(P.g2);
     make_packs(P.wapply);
     make_packs(P.combL);
+    return P;
+}
+
+
+// ============================================================ sigma, rbtm schedule
+// site_hamil_rbtm (non-abelian/site_hamil.hpp:149-204; abelian/site_hamil.hpp for
+// TwoU1), the reference's schedule where (rows - one_rows) >= (cols - one_cols):
+//   PRP  = psi copied as is (the FP32 forms read it in their operand type)
+//   t    = X_c * RC_c              one GEMM per ket right charge (the right step's g1)
+//   BUF  = rbtm W application      rows (b1, ol) per output left charge or
+//   S_or = LA_or^T * BUF_or        LA_or stacks L[b1](ol, or) over the same rows
+//                                  (or the conjugate of the stored partner)
+//   y    = S reshaped from right-paired to left-paired (reshape_right_to_left_new)
+HostPlan build_sigma_plan_rbtm(const mq_site_desc &d, int tm, int tn, int kc, bool ref_count) {
+    Site s(d);
+    BLayout L = to_layout(d.L), R = to_layout(d.R);
+    if (L.aux != s.W.rows || R.aux != s.W.cols) throw std::runtime_error("boundary aux_dim does not match the MPO");
+    HostPlan P;
+    P.kind = PLAN_SIGMA;
+    P.rbtm = 1;
+    P.symm = s.symm;
+    P.vec_len = s.X.total;
+    for (auto &xb : s.X.blk) P.reshape.push_back(mk_copy(xb.off, xb.off, xb.rows, xb.cols, xb.rows, xb.rows, false, 1.0));
+    P.n_prp = s.X.total;
+    RbtmCore core = rbtm_core(s, R, &L, P, tm, tn, kc);
+    // LA_or (K_or x |or|, leading dimension K_or = Mr[or]): row block rb.row holds
+    // L[b1](ol, or) (ket rows, bra columns), or for a hermitian-partner row the
+    // stored block (or, ol) of L[conj(b1)] transposed, scaled by the conjugate
+    // correction and phase (rbtm_axpy_gemm with conjugate(left[left_conj(b1)]))
+    std::vector<int64_t> la_off(s.nl, -1), s_off(s.nl, -1);
+    int64_t lat = 0, st = 0;
+    for (int l = 0; l < s.nl; ++l) {
+        if (s.pbr_sz_l[l] == 0) continue;
+        s_off[l] = st;
+        st += (int64_t)s.lsize(l) * s.pbr_sz_l[l];
+        if (core.Mr[l] == 0) continue;
+        la_off[l] = lat;
+        lat += (int64_t)core.Mr[l] * s.lsize(l);
+    }
+    P.n_la = lat;
+    P.n_tmp = st;
+    for (auto &rb : core.rowblks) {
+        const bool skip = s.W.lskip(rb.b1);
+        const int b = skip ? s.W.lherm[rb.b1] : rb.b1;
+        const int ols = s.lsize(rb.ol), ors = s.lsize(rb.orr);
+        const BBlk *x = skip ? L.find(b, s.left.q[rb.orr], s.left.q[rb.ol]) : L.find(b, s.left.q[rb.ol], s.left.q[rb.orr]);
+        if (!x) throw std::runtime_error("rbtm row block without its boundary block");
+        if (x->rows * x->cols != ols * ors) throw std::runtime_error("boundary block size mismatch");
+        const double sc = (skip && s.su2) ? conj_correction(x->lc, x->rc, s.W.lspin[rb.b1]) * s.W.lphase[rb.b1] : 1.0;
+        P.arrL.push_back(mk_copy(la_off[rb.orr] + rb.row, x->off, ols, ors, core.Mr[rb.orr], skip ? ors : ols, skip, sc));
+    }
+    // S_or = LA_or^T BUF_or; output charges no row reaches get K = 0 (zeros)
+    for (int l = 0; l < s.nl; ++l) {
+        if (s.pbr_sz_l[l] == 0) continue;
+        GemmGroup g{};
+        g.a = std::max<int64_t>(la_off[l], 0);
+        g.b = std::max<int64_t>(core.buf_off[l], 0);
+        g.c = s_off[l];
+        g.M = s.lsize(l);
+        g.N = s.pbr_sz_l[l];
+        g.K = core.Mr[l];
+        g.lda = std::max(core.Mr[l], 1);
+        g.ldb = std::max(core.Mr[l], 1);
+        g.ldc = s.lsize(l);
+        g.transA = 1;
+        add_gemm(P.g2, g, tm, tn, kc);
+    }
+    // right-paired S -> left-paired y: the inverse of the psi reshape (prp_layout)
+    for (auto &xb : s.X.blk) {
+        int r = s.ri.get(xb.c);
+        for (int pp = 0; pp < s.np; ++pp) {
+            int l = s.li.get(xb.c - s.phys.q[pp]);
+            if (l < 0 || s_off[l] < 0) continue;
+            int ld = s.lsize(l);
+            for (int ss = 0; ss < s.phys.s[pp]; ++ss)
+                P.scatter.push_back(mk_copy(xb.off + s.pbl[(size_t)pp * s.nl + l] + ss * ld,
+                                            s_off[l] + (int64_t)(s.pbr[(size_t)pp * s.nr + r] + ss * xb.cols) * ld, ld,
+                                            xb.cols, xb.rows, ld, false, 1.0));
+        }
+    }
+    P.exec_flops = P.g1.flops() + P.g2.flops();
+    for (auto &sp : P.wapply.strips) P.exec_flops += 2.0 * sp.rows * sp.cols * (sp.s1 - sp.s0);
+    P.ref_flops = ref_count ? reference_sigma_flops(d) : -1;
+    P.min_bytes = 8.0 * (L.total + R.total + 2.0 * s.X.total);
+    // coordinates H_eff couples: the rows of y any nonzero strip source reaches go
+    // through S_or = LA^T BUF, so they are the (or, phys, r) slots whose BUF rows
+    // carry a source
+    P.live.assign((size_t)s.X.total, 0);
+    {
+        std::vector<std::vector<char>> col_live(s.nl);
+        for (int l = 0; l < s.nl; ++l) col_live[l].assign(s.pbr_sz_l[l], 0);
+        size_t k = 0;
+        for (auto &rb : core.rowblks) {
+            Q orq = s.left.q[rb.orr];
+            for (int p = 0; p < s.np; ++p) {
+                int r = s.ri.get(orq + s.phys.q[p]);
+                if (r < 0) continue;
+                int rsz = s.rsize(r);
+                for (int ss = 0; ss < s.phys.s[p]; ++ss) {
+                    const Strip &sp = P.wapply.strips[k++];
+                    bool any = false;
+                    for (int q = sp.s0; q < sp.s1 && !any; ++q) any = P.wapply.srcs[q].coef != 0.0;
+                    if (!any) continue;
+                    int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
+                    for (int c = 0; c < rsz; ++c) col_live[rb.orr][col0 + c] = 1;
+                }
+            }
+        }
+        for (auto &xb : s.X.blk) {
+            int r = s.ri.get(xb.c);
+            for (int pp = 0; pp < s.np; ++pp) {
+                int l = s.li.get(xb.c - s.phys.q[pp]);
+                if (l < 0 || s_off[l] < 0) continue;
+                int ld = s.lsize(l);
+                for (int ss = 0; ss < s.phys.s[pp]; ++ss)
+                    for (int c = 0; c < xb.cols; ++c)
+                        if (col_live[l][s.pbr[(size_t)pp * s.nr + r] + ss * xb.cols + c])
+                            std::fill_n(P.live.begin() + xb.off + s.pbl[(size_t)pp * s.nl + l] + ss * ld +
+                                            (int64_t)c * xb.rows,
+                                        ld, (uint8_t)1);
+            }
+        }
+    }
+    order_tiles(P.g1);
+    order_tiles(P.g2);
+    make_packs(P.wapply);
     return P;
 }
 

@@ -92,6 +92,7 @@ enum PlanKind { PLAN_SIGMA = 0, PLAN_LSTEP = 1, PLAN_RSTEP = 2, PLAN_DIAG = 3 };
 // Host-side plan; the API layer uploads the task arrays.
 struct HostPlan {
     int kind = PLAN_SIGMA;
+    int rbtm = 0;  // sigma plans: 1 for the rbtm schedule (build_sigma_plan_rbtm)
     int symm = MQ_SYMM_SU2U1;
     int64_t vec_len = 0;
     // arena sizes (elements)
@@ -109,6 +110,10 @@ struct HostPlan {
     int out_aux = 0;
     std::vector<int32_t> out_bptr, out_lc, out_rc, out_rows, out_cols;
     std::vector<int64_t> out_off;
+    // sigma plans: live[i] = 1 where H_eff couples coordinate i of the vector, 0
+    // where row i and column i of H_eff are structurally zero (no operator term
+    // reaches that (phys, left, right) slot of the reduced two-site tensor)
+    std::vector<uint8_t> live;
     // algorithmic accounting (reference schedule)
     double ref_flops = 0, min_bytes = 0;
     // executed FLOPs of this implementation (for reporting)
@@ -126,8 +131,17 @@ struct FuseRule {
 // ref_count: also count the reference schedule's FLOPs (ref_flops; -1 when not
 // counted).  The count walks the reference schedule once more and is only needed
 // for measurement.
+// Schedule of a sigma plan: SCHED_LBTM the lbtm form (with fused columns) at
+// every site; SCHED_REFERENCE the reference's choice per site (site_hamil.hpp:35):
+// rbtm (build_sigma_plan_rbtm, no fused rows) where (rows - one_rows) >=
+// (cols - one_cols) for SU2U1, lbtm elsewhere.
+enum SigmaSchedule { SCHED_LBTM = 0, SCHED_REFERENCE = 1 };
 HostPlan build_sigma_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {},
-                          bool ref_count = true);
+                          bool ref_count = true, int schedule = SCHED_LBTM);
+// The rbtm schedule (site_hamil.hpp:149-204) on every site: t = X * RC, the rbtm
+// W application into BUF, S = LA^T * BUF, y = S reshaped to the left-paired layout.
+HostPlan build_sigma_plan_rbtm(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target,
+                               bool ref_count = true);
 HostPlan build_lstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target, FuseRule fuse = {},
                           bool ref_count = true);
 HostPlan build_rstep_plan(const mq_site_desc &d, int tile_m, int tile_n, int kchunk_target);

@@ -56,6 +56,9 @@ def lib():
         L.mq_block_davidson.argtypes = [vp, vp, vp, vp, vp, vp, f64p, C.c_int, C.c_int, C.c_double, C.c_int,
                                         C.POINTER(vp), C.c_int, C.POINTER(C.c_int)]
         L.mq_plan_rebind.argtypes = [vp]
+        L.mq_plan_bind.argtypes = [vp, vp, vp]
+        L.mq_plan_live_mask.argtypes = [vp, vp]
+        L.mq_plan_solve_info.argtypes = [vp, f64p]
         L.mq_plan_release.argtypes = [vp]
         L.mq_gemm_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp,
                                   C.c_int, C.c_int]
@@ -74,6 +77,8 @@ def lib():
         L.mq_host_plan_flops.argtypes = [vp, f64p, f64p, f64p]
         L.mq_host_plan_out_desc.argtypes = [vp, C.POINTER(mq_boundary_desc)]
         L.mq_host_plan_destroy.argtypes = [vp]
+        L.mq_host_plan_schedule.argtypes = [vp]
+        L.mq_plan_schedule.argtypes = [vp]
         L.mq_host_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -92,19 +97,21 @@ STRIP = np.dtype([("dst", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("ldd", "<i4
                   ("s1", "<i4")], align=True)
 SRC = np.dtype([("off", "<i8"), ("ld", "<i4"), ("coef", "<f8")], align=True)
 PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D, STRIP, SRC,
-               ROWMAP, ROWMAP]
+               ROWMAP, ROWMAP, np.dtype(np.uint8)]
 PART_NAMES = ["arrL", "arrR", "reshape", "g1_groups", "g1_tiles", "strips", "srcs", "g2_groups", "g2_tiles",
-              "scatter", "comb_strips", "comb_srcs", "g1_rowmap", "g2_rowmap"]
+              "scatter", "comb_strips", "comb_srcs", "g1_rowmap", "g2_rowmap", "live"]
 KIND = {"sigma": 0, "lstep": 1, "rstep": 2, "diag": 3}
 
 
 class HostPlan:
     """Task lists of a plan, built on the host (no GPU needed)."""
 
-    def __init__(self, site: SiteDesc, kind: str, fuse=1, fuse_ratio=0, fuse_min=0):
+    def __init__(self, site: SiteDesc, kind: str, fuse=1, fuse_ratio=0, fuse_min=0, schedule=0):
+        """schedule 1: sigma plans take the reference's per-site choice of lbtm or
+        rbtm (MQ_HOST_SCHED_REFERENCE); self.rbtm tells which was built."""
         L = lib()
         h = C.c_void_p()
-        code = KIND[kind] | ((fuse + 1) << 8) | (fuse_ratio << 10) | (fuse_min << 18)
+        code = KIND[kind] | ((fuse + 1) << 8) | (fuse_ratio << 10) | (fuse_min << 18) | ((schedule & 1) << 30)
         if L.mq_host_plan_create(site.ptr(), code, C.byref(h)) != 0:
             raise MQError(L.mq_host_last_error().decode())
         try:
@@ -125,6 +132,7 @@ class HostPlan:
             a, b, c = C.c_double(), C.c_double(), C.c_double()
             L.mq_host_plan_flops(h, C.byref(a), C.byref(b), C.byref(c))
             self.ref_flops, self.exec_flops, self.min_bytes = a.value, b.value, c.value
+            self.rbtm = L.mq_host_plan_schedule(h) == 1
             self.out_layout = None
             if kind in ("lstep", "rstep"):
                 d = mq_boundary_desc()
@@ -202,6 +210,7 @@ class GpuEngine:
         self.set_fuse(rule.fuse, rule.ratio, rule.min_entries)
         self.check(lib().mq_ctx_set_option(self.ctx, 3, rule.kchunk), "mq_ctx_set_option")
         self.check(lib().mq_ctx_set_option(self.ctx, 5, rule.tile_m), "mq_ctx_set_option")
+        self.check(lib().mq_ctx_set_option(self.ctx, 7, rule.sched), "mq_ctx_set_option")
         self.schedule = rule
 
     def close(self):
@@ -340,7 +349,8 @@ class GpuEngine:
         problems precondition the corrections with the diagonal of H_eff."""
         sp = self.sigma_plan(symm, psis[0], L, R, W)
         try:
-            X0 = self.torch.as_tensor(independent_starts([p.to_flat() for p in psis])).to(self.device)
+            X0 = self.torch.as_tensor(independent_starts([p.to_flat() for p in psis],
+                                                         live=sp.live_mask())).to(self.device)
             diag = sp.diagonal() if symm == "su2u1" else None
             thetas, U, its = sp.block_davidson(X0, max_iter, tol, diag=diag)
             self.flops += its * len(psis) * sp.work_flops
@@ -360,16 +370,23 @@ class GpuEngine:
             sp.close()
 
 
-def independent_starts(vecs, seed=0):
+def independent_starts(vecs, seed=0, live=None):
     """Orthonormal start vectors for a block solve: the given vectors by
     Gram-Schmidt, each one that is (numerically) in the span of those before it
     replaced by a random vector.  The roots' two-site tensors at the start of a
-    state-averaged sweep can be dependent (all built on one shared neighbour)."""
+    state-averaged sweep can be dependent (all built on one shared neighbour).
+    live (the plan's live mask, SigmaPlan.live_mask()): every vector is
+    restricted to the coordinates H_eff couples, so that the structurally zero
+    columns of H_eff, whose eigenvalue 0 may lie below every physical root,
+    cannot enter the block."""
     rng = np.random.default_rng(seed)
+    keep = None if live is None else np.asarray(live, dtype=bool)
     out = []
     for v in vecs:
         v = np.array(v, dtype=np.float64)
         for _ in range(8):
+            if keep is not None:
+                v = np.where(keep, v, 0.0)
             nv = np.linalg.norm(v)
             w = v.copy()
             for q in out:
@@ -401,6 +418,33 @@ class SigmaPlan:
         self.work_flops = self.ref_flops if self.ref_flops >= 0 else self.exec_flops
         self.n = lib().mq_plan_vec_len(self.p)
         self.workspace_bytes = lib().mq_plan_workspace_bytes(self.p)
+        self.rbtm = lib().mq_plan_schedule(self.p) == 1
+        self.bind()
+
+    def bind(self, L: GpuBoundary = None, R: GpuBoundary = None):
+        """Arrange the boundaries (by default the plan's own) into the plan's
+        panels now (mq_plan_bind); later calls use them.  Call it after
+        writing new values into a boundary tensor, or to switch boundaries."""
+        if L is not None:
+            self.L = L
+        if R is not None:
+            self.R = R
+        self.eng.enter()
+        self.eng.check(lib().mq_plan_bind(self.p, _ptr(self.L.data), _ptr(self.R.data)), "mq_plan_bind")
+
+    def solve_info(self):
+        """The last solver call: dict(form_iterations, fp64_iterations,
+        theta_form, theta_rayleigh_fp64) (mq_plan_solve_info)."""
+        info = (C.c_double * 4)()
+        self.eng.check(lib().mq_plan_solve_info(self.p, info), "mq_plan_solve_info")
+        return dict(form_iterations=int(info[0]), fp64_iterations=int(info[1]), theta_form=info[2],
+                    theta_rayleigh_fp64=info[3])
+
+    def live_mask(self):
+        """Host bool array: the vector coordinates H_eff couples (mq_plan_live_mask)."""
+        m = np.zeros(self.n, dtype=np.uint8)
+        self.eng.check(lib().mq_plan_live_mask(self.p, m.ctypes.data_as(C.c_void_p)), "mq_plan_live_mask")
+        return m.astype(bool)
 
     def release(self):
         """Free the plan's cached work arrays (batched lists and arenas, Davidson
@@ -471,6 +515,7 @@ class SigmaPlan:
         self.eng.enter()
         self.eng.check(lib().mq_sigma_timed(self.p, _ptr(self.L.data), _ptr(self.R.data), _ptr(x), _ptr(y), prec,
                                             ms, work), "mq_sigma_timed")
+        self.eng.leave()
         return list(ms), list(work)
 
     def diagonal(self):

@@ -62,7 +62,8 @@ class _CompareEngine:
         from .gpu import independent_starts
         sp = self.eng.sigma_plan(symm, psis[0], L, R, W)
         try:
-            X0 = self.eng.torch.as_tensor(independent_starts([p.to_flat() for p in psis])).to(self.eng.device)
+            X0 = self.eng.torch.as_tensor(independent_starts([p.to_flat() for p in psis],
+                                                             live=sp.live_mask())).to(self.eng.device)
             diag = sp.diagonal() if symm == "su2u1" else None
             t64, U, its = sp.block_davidson(X0, max_iter, tol, prec=MQ_PREC_FP64, diag=diag)
             for name, prec in LOW_FORMS:

@@ -89,6 +89,18 @@ def run_sigma(hp, Ldata, Rdata, x):
     la, rb, prp = np.zeros(max(s["la"], 1)), np.zeros(max(s["rb"], 1)), np.zeros(max(s["prp"], 1))
     T, Cb = np.zeros(max(s["t"], 1)), np.zeros(max(s["c"], 1))
     y = np.full(s["vec"], np.nan)
+    if getattr(hp, "rbtm", False):
+        # rbtm: t = psi RC, BUF = W applied to t, S = LA^T BUF, y = S right -> left paired
+        tmp = np.full(max(s["tmp"], 1), np.nan)
+        copy2d(P["arrL"], Ldata, la)
+        copy2d(P["arrR"], Rdata, rb)
+        copy2d(P["reshape"], x, prp)
+        gemm(P["g1_groups"], P["g1_tiles"], prp, rb, T, P["g1_rowmap"])
+        wapply(P["strips"], P["srcs"], T, Cb)
+        gemm(P["g2_groups"], P["g2_tiles"], la, Cb, tmp)
+        copy2d(P["scatter"], tmp, y)
+        assert not np.isnan(y).any(), "sigma element not written"
+        return y
     copy2d(P["arrL"], Ldata, la)
     wapply(P["comb_strips"], P["comb_srcs"], la, la)
     copy2d(P["arrR"], Rdata, rb)

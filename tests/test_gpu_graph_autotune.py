@@ -67,7 +67,10 @@ def test_autotune_picks_a_schedule_with_the_same_sigma(fixtures, oracle_engine, 
     from qcmaquis_amd.desc import MQ_PREC_FP64
     ham, psi, Lg, Rg, W = _benzene(oracle_engine, gpu_engine, fixtures)
     best, table = tune(gpu_engine, ham.symm, psi, Lg, Rg, W, prec=MQ_PREC_FP64, reps=2)
-    assert len(table) == len(candidates()) and table[0][0] == best
+    assert len(table) == len(candidates())
+    # the fastest candidate, or the default when that is within 5% of the fastest
+    dms = [r[1] for r in table if r[0] == DEFAULT][0]
+    assert best == table[0][0] or (best == DEFAULT and table[0][1] > 0.95 * dms)
     assert all(table[i][1] <= table[i + 1][1] for i in range(len(table) - 1))
     x = gpu_engine.to_dev(np.random.default_rng(3).standard_normal(psi.to_flat().size))
     ys = []

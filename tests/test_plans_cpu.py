@@ -87,3 +87,37 @@ def test_plan_rejects_mismatched_boundary(fixtures, oracle_engine):
     bad = BoundaryLayout(W.row_dim + 1, [0] * (W.row_dim + 2), [], [], [], [], [], 0)
     with pytest.raises(MQError):
         HostPlan(SiteDesc(ham.symm, psi.phys, psi.left, psi.right, W, bad, Rb.layout), "sigma")
+
+
+def test_rbtm_plans_match_oracle(fixtures, oracle_engine):
+    """The reference's per-site schedule (MQ_HOST_SCHED_REFERENCE): at the sites
+    where site_hamil.hpp:35 picks rbtm, the rbtm plan (t = psi R, rbtm W
+    application, S = L^T BUF, right-to-left reshape) executed by the emulator
+    equals the oracle's sigma, executes the reference's FLOPs to within 5%, and
+    has the same live mask as the lbtm plan."""
+    import emulate
+    from qcmaquis_amd.desc import SiteDesc
+    from qcmaquis_amd.gpu import HostPlan
+    from qcmaquis_amd.integrals import synthetic
+    from qcmaquis_amd.sweep import Hamiltonian, TwoSiteDMRG
+    n_rbtm = 0
+    for ham in (ham_from_fixture(fixtures["benzene"]), Hamiltonian(8, 8, 0, synthetic(8, seed=1008))):
+        d = TwoSiteDMRG(ham, oracle_engine, 48)
+        d.sweep(0)
+        for site in range(ham.L - 1):
+            psi, Lb, Rb, W = d.site_problem(site)
+            sd = SiteDesc(ham.symm, psi.phys, psi.left, psi.right, W, Lb.layout, Rb.layout)
+            h1 = HostPlan(sd, "sigma", schedule=1)
+            h0 = HostPlan(sd, "sigma")
+            assert not h0.rbtm
+            x = np.random.default_rng(site).standard_normal(psi.to_flat().size)
+            y, fl = oracle_engine.sigma(ham.symm, psi, Lb, Rb, W, x=x)
+            ye = emulate.run_sigma(h1, Lb.data, Rb.data, x)
+            assert np.abs(y - ye).max() <= 1e-13 * max(1.0, np.abs(y).max()), site
+            assert h1.ref_flops == fl
+            assert np.array_equal(h1.parts["live"], h0.parts["live"])
+            if h1.rbtm:
+                n_rbtm += 1
+                assert abs(h1.exec_flops - fl) <= 0.05 * fl, (site, h1.exec_flops, fl)
+            d.microiteration(1, site)
+    assert n_rbtm >= 4
