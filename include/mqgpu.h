@@ -337,6 +337,16 @@ int mq_plan_release(mq_plan *plan);
 int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
                 const double *A, int lda, const double *B, int ldb,
                 double *C, int ldc, int kchunk);
+/* mq_gemm_f64 without any allocation (no reference counterpart: for callers
+ * that cannot allocate device memory, e.g. code under HIP graph capture or
+ * tight serving loops).  The task lists and split-K partial tiles go to a
+ * fixed per-device scratch array of 32 MiB held by the library until the
+ * process ends; a product that needs more fails with MQ_ERR_ARG and a message
+ * naming the size.  Calls that overlap on one GPU share that array and race.
+ * Same arguments, kernels and results (bit for bit) as mq_gemm_f64. */
+int mq_gemm_f64_noalloc(mq_ctx *ctx, int transA, int transB, int M, int N, int K,
+                        const double *A, int lda, const double *B, int ldb,
+                        double *C, int ldc, int kchunk);
 /* The same for the FP32-operand kernel of the mixed forms, with the tile shape
  * tm x tn (tm 64 or 128, tn a multiple of 16 up to 128): FP32 products, or the
  * BF16x3 products of MQ_PREC_BF16X3 when bf16x3 != 0. */

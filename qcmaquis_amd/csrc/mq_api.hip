@@ -5,15 +5,43 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "mq_kernels.hpp"
 
 using namespace mq;
 
+// Scratch of the allocation-free entry point (mq_gemm_f64_noalloc): one fixed
+// array per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: set-up (mq_ctx_create, once per device, with the
+// device current) asks for its device address with hipGetSymbolAddress and
+// keeps it per device; kernels and copies get that address.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device -> address of `scratch` on that device
+
+// device address of the scratch array on `device` (which must be current)
+void *scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return ptr;
+}
+}  // namespace
+
 struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void *scratch = nullptr;  // this device's `scratch` array (device address)
     std::string err;
     FuseRule fuse;  // MQ_OPT_FUSE, MQ_OPT_FUSE_RATIO, MQ_OPT_FUSE_MIN
     int kchunk = 2048;  // MQ_OPT_KCHUNK
@@ -166,6 +194,7 @@ int mq_ctx_create(int device, mq_ctx **out) {
         c->device = device;
         check(hipSetDevice(device), "hipSetDevice");
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        c->scratch = scratch_setup(device);
         *out = c.release();
         return MQ_OK;
     } catch (std::exception &e) {
@@ -734,6 +763,70 @@ int mq_gemm_f64(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const 
                           ctx->stream),
               "gemm");
         check(hipStreamSynchronize(ctx->stream), "gemm sync");
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(ctx, e, MQ_ERR_HIP);
+    }
+}
+
+// mq_gemm_f64 without allocation: the task lists and the split-K partial tiles
+// go to this device's scratch array (kScratchBytes), laid out as
+//   [group | split group ids | tiles | partials], each part 256-byte aligned.
+// The partials need no clearing (the split tiles write every element of
+// theirs).  Calls that overlap on one GPU share the array and race.
+int mq_gemm_f64_noalloc(mq_ctx *ctx, int transA, int transB, int M, int N, int K, const double *A, int lda,
+                        const double *B, int ldb, double *C, int ldc, int kchunk) {
+    if (!ctx || M < 0 || N < 0 || K < 0 || kchunk < 16) return MQ_ERR_ARG;
+    try {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!ctx->scratch) throw std::runtime_error("context without scratch");
+        GemmGroup g{};
+        g.M = M;
+        g.N = N;
+        g.K = K;
+        g.lda = lda;
+        g.ldb = ldb;
+        g.ldc = ldc;
+        g.transA = transA ? 1 : 0;
+        g.transB = transB ? 1 : 0;
+        GemmList gl;
+        tile_gemm(gl, g, 0, 0, kchunk);
+        order_tiles(gl);
+        std::vector<int> sp;
+        for (size_t i = 0; i < gl.groups.size(); ++i)
+            if (gl.groups[i].split > 1) sp.push_back((int)i);
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t o_grp = 0;
+        const size_t o_sp = o_grp + up(sizeof(GemmGroup) * gl.groups.size());
+        const size_t o_til = o_sp + up(sizeof(int) * sp.size());
+        const size_t o_ws = o_til + up(sizeof(GemmTile) * gl.tiles.size());
+        const size_t need = o_ws + sizeof(double) * (size_t)gl.ws;
+        if (need > kScratchBytes) {
+            ctx->err = "mq_gemm_f64_noalloc: this product needs " + std::to_string(need) +
+                       " bytes of scratch (task lists and split-K partials); the maximum is " +
+                       std::to_string(kScratchBytes);
+            return MQ_ERR_ARG;
+        }
+        unsigned char *base = static_cast<unsigned char *>(ctx->scratch);
+        hipStream_t s = ctx->stream;
+        if (!gl.groups.empty())
+            check(hipMemcpyAsync(base + o_grp, gl.groups.data(), sizeof(GemmGroup) * gl.groups.size(),
+                                 hipMemcpyHostToDevice, s),
+                  "group upload");
+        if (!sp.empty())
+            check(hipMemcpyAsync(base + o_sp, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice, s),
+                  "split upload");
+        if (!gl.tiles.empty())
+            check(hipMemcpyAsync(base + o_til, gl.tiles.data(), sizeof(GemmTile) * gl.tiles.size(),
+                                 hipMemcpyHostToDevice, s),
+                  "tile upload");
+        check(launch_gemm(reinterpret_cast<const GemmGroup *>(base + o_grp),
+                          reinterpret_cast<const GemmTile *>(base + o_til), (int)gl.tiles.size(),
+                          reinterpret_cast<const int *>(base + o_sp), (int)sp.size(), A, B, C,
+                          reinterpret_cast<double *>(base + o_ws), nullptr, ST_MISC, s),
+              "gemm");
+        // the host task lists must outlive their copies
+        check(hipStreamSynchronize(s), "gemm sync");
         return MQ_OK;
     } catch (std::exception &e) {
         return fail(ctx, e, MQ_ERR_HIP);
