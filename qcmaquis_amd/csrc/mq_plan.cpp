@@ -160,7 +160,9 @@ int column_tile(int N) {
     return best;
 }
 
-void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
+// chunk_align: K chunks of split groups are multiples of it (16, or kWaChunk for the
+// k_gemm_wa groups, whose operand is built kWaChunk columns at a time)
+void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target, int chunk_align = 16) {
     if (g.rowmap < 0 || (g.rowmap > 0 && g.rowmap - 1 + g.M > (int64_t)gl.rowmap.size()))
         throw std::runtime_error("GEMM row map out of range");
     if ((tm != 0 && tm != 64 && tm != 128) || (tn != 0 && (tn < 16 || tn > 128 || tn % 16)))
@@ -175,7 +177,7 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     // split K when the output is small and K long: chunks of about kchunk_target
     if (g.K > 2 * kchunk_target && mt * nt < 64 && g.rowmap == 0) {
         int s = std::min(ceil_div(g.K, kchunk_target), 64);
-        int kc = ceil_div(ceil_div(g.K, s), 16) * 16;
+        int kc = ceil_div(ceil_div(g.K, s), chunk_align) * chunk_align;
         s = ceil_div(g.K, kc);
         if (s > 1) {
             g.split = s;
@@ -948,6 +950,7 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     P.n_t = t.t_total;
     std::vector<int> K(s.nr, 0);
     std::vector<Seg> segs;
+    std::vector<int> seg_first;  // first strip of each segment (one strip per row range, in row order)
     std::vector<int> strip_al;
     std::vector<Raw> raw;
     // reference schedule (site_hamil.hpp:35): when it is lbtm its FLOPs are counted in this pass
@@ -968,6 +971,7 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
                 sg.koff = K[sg.al];
                 K[sg.al] += s.rsize(sg.ar);
                 segs.push_back(sg);
+                seg_first.push_back((int)P.wapply.strips.size());
                 emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, P.wapply, strip_al);
                 ref_trim += 2.0 * s.X.blk[s.x_of_r[sg.al]].rows * s.rsize(sg.ar) * s.rsize(sg.al);
             }
@@ -987,18 +991,29 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     // writes, in every column of its output block (strip dst is still relative to
     // its al block here: koff * rows + row0)
     P.live.assign((size_t)s.X.total, 0);
-    for (size_t i = 0; i < P.wapply.strips.size(); ++i) {
-        const Strip &st = P.wapply.strips[i];
-        bool any = false;
-        for (int k = st.s0; k < st.s1 && !any; ++k) any = P.wapply.srcs[k].coef != 0.0;
-        if (!any) continue;
-        const VBlk &xb = s.X.blk[s.x_of_r[strip_al[i]]];
-        const int64_t row0 = st.dst % xb.rows;
-        for (int c = 0; c < xb.cols; ++c)
-            std::fill_n(P.live.begin() + xb.off + (int64_t)c * xb.rows + row0, st.rows, (uint8_t)1);
+    {
+        std::vector<std::vector<uint8_t>> row_live(s.nr);  // per al: rows any strip reaches
+        for (size_t i = 0; i < P.wapply.strips.size(); ++i) {
+            const Strip &st = P.wapply.strips[i];
+            bool any = false;
+            for (int k = st.s0; k < st.s1 && !any; ++k) any = P.wapply.srcs[k].coef != 0.0;
+            if (!any) continue;
+            const VBlk &xb = s.X.blk[s.x_of_r[strip_al[i]]];
+            auto &rl = row_live[strip_al[i]];
+            if (rl.empty()) rl.assign(xb.rows, 0);
+            std::fill_n(rl.begin() + st.dst % xb.rows, st.rows, (uint8_t)1);
+        }
+        for (auto &xb : s.X.blk) {
+            const auto &rl = row_live[s.ri.get(xb.c)];
+            if (rl.empty()) continue;
+            for (int c = 0; c < xb.cols; ++c)
+                std::copy(rl.begin(), rl.end(), P.live.begin() + xb.off + (int64_t)c * xb.rows);
+        }
     }
     patch_strips(P.wapply, strip_al, c_off);
-    P.n_c = ct;
+    // the C panels are not stored: the output GEMM builds them chunk by chunk (k_gemm_wa)
+    (void)ct;
+    P.n_c = 0;
     P.n_rb = rbt;
     for (auto &sg : segs) {
         int rar = s.rsize(sg.ar), ral = s.rsize(sg.al);
@@ -1006,20 +1021,73 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
         P.arrR.push_back(mk_copy(rb_off[sg.al] + sg.koff, sg.rblk->off, rar, ral, K[sg.al], sg.rtrans ? ral : rar,
                                  sg.rtrans, sg.rscale));
     }
+    P.wa = 1;
+    std::vector<std::vector<int>> segs_of(s.nr);  // per al: its segments in K order
+    for (size_t i = 0; i < segs.size(); ++i) segs_of[segs[i].al].push_back((int)i);
     for (auto &xb : s.X.blk) {
         int r = s.ri.get(xb.c);
         GemmGroup g{};
-        g.a = c_off[r];
+        g.a = 0;  // T arena base (per vector in batched lists)
         g.b = rb_off[r];
         g.c = xb.off;
         g.M = xb.rows;
         g.N = xb.cols;
         g.K = K[r];
-        g.lda = xb.rows;
+        g.lda = (int)P.wa_chunks.size();  // chunk table base
         g.ldb = std::max(K[r], 1);
         g.ldc = xb.rows;
-        add_gemm(P.g2, g, tm, tn, kc);
+        add_gemm(P.g2, g, tm, tn, kc, kWaChunk);
+        if (g.M == 0 || g.N == 0) continue;
+        const int gtm = P.g2.groups.back().tm;
+        // row ranges of al, in row order: the strips of each of its segments
+        std::vector<std::pair<int, int>> rs;  // (row0, rows)
+        for (int p = 0; p < s.np; ++p) {
+            int lb = s.rmp_l[(size_t)r * s.np + p];
+            if (lb < 0) continue;
+            int lsz = s.lsize(lb);
+            for (int ss = 0; ss < s.phys.s[p]; ++ss) rs.push_back({s.pbl[(size_t)p * s.nl + lb] + ss * lsz, lsz});
+        }
+        std::sort(rs.begin(), rs.end());
+        const std::vector<int> &sg_al = segs_of[r];
+        const int nch = ceil_div(std::max(g.K, 1), kWaChunk);
+        for (int mt = 0; mt * gtm < g.M; ++mt) {
+            const int m0 = mt * gtm, m1 = std::min(g.M, m0 + gtm);
+            size_t q0 = 0;
+            while (q0 < rs.size() && rs[q0].first + rs[q0].second <= m0) ++q0;
+            size_t gi = 0;
+            for (int c = 0; c < nch; ++c) {
+                const int k0 = c * kWaChunk, k1 = std::min(g.K, k0 + kWaChunk);
+                P.wa_chunks.push_back((int)P.wa_pieces.size());
+                int e0 = 0;
+                while (gi < sg_al.size() && segs[sg_al[gi]].koff + s.rsize(segs[sg_al[gi]].ar) <= k0) ++gi;
+                for (size_t gk = gi; gk < sg_al.size() && segs[sg_al[gk]].koff < k1; ++gk) {
+                    const Seg &sg = segs[sg_al[gk]];
+                    const int ka = std::max(k0, sg.koff), kb = std::min(k1, sg.koff + s.rsize(sg.ar));
+                    for (size_t q = q0; q < rs.size() && rs[q].first < m1; ++q) {
+                        const int ra = std::max(m0, rs[q].first), rb = std::min(m1, rs[q].first + rs[q].second);
+                        const Strip &st = P.wapply.strips[seg_first[sg_al[gk]] + q];
+                        if (st.rows != rs[q].second || st.cols != s.rsize(sg.ar))
+                            throw std::runtime_error("W-application piece outside its strip");
+                        WaPiece pc;
+                        pc.s0 = st.s0;
+                        pc.s1 = st.s1;
+                        pc.r0 = ra - rs[q].first;
+                        pc.c0 = ka - sg.koff;
+                        pc.nr = rb - ra;
+                        pc.nc = kb - ka;
+                        pc.dr = ra - m0;
+                        pc.dc = ka - k0;
+                        pc.e0 = e0;
+                        e0 += pc.nr * pc.nc;
+                        P.wa_pieces.push_back(pc);
+                    }
+                }
+                if (e0 != (m1 - m0) * (k1 - k0) && g.K > 0)
+                    throw std::runtime_error("W-application pieces do not cover an output GEMM chunk");
+            }
+        }
     }
+    P.wa_chunks.push_back((int)P.wa_pieces.size());
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
     if (!ref_count) {
@@ -1091,15 +1159,24 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     // writes, in every column of its output block (strip dst is still relative to
     // its al block here: koff * rows + row0)
     P.live.assign((size_t)s.X.total, 0);
-    for (size_t i = 0; i < P.wapply.strips.size(); ++i) {
-        const Strip &st = P.wapply.strips[i];
-        bool any = false;
-        for (int k = st.s0; k < st.s1 && !any; ++k) any = P.wapply.srcs[k].coef != 0.0;
-        if (!any) continue;
-        const VBlk &xb = s.X.blk[s.x_of_r[strip_al[i]]];
-        const int64_t row0 = st.dst % xb.rows;
-        for (int c = 0; c < xb.cols; ++c)
-            std::fill_n(P.live.begin() + xb.off + (int64_t)c * xb.rows + row0, st.rows, (uint8_t)1);
+    {
+        std::vector<std::vector<uint8_t>> row_live(s.nr);  // per al: rows any strip reaches
+        for (size_t i = 0; i < P.wapply.strips.size(); ++i) {
+            const Strip &st = P.wapply.strips[i];
+            bool any = false;
+            for (int k = st.s0; k < st.s1 && !any; ++k) any = P.wapply.srcs[k].coef != 0.0;
+            if (!any) continue;
+            const VBlk &xb = s.X.blk[s.x_of_r[strip_al[i]]];
+            auto &rl = row_live[strip_al[i]];
+            if (rl.empty()) rl.assign(xb.rows, 0);
+            std::fill_n(rl.begin() + st.dst % xb.rows, st.rows, (uint8_t)1);
+        }
+        for (auto &xb : s.X.blk) {
+            const auto &rl = row_live[s.ri.get(xb.c)];
+            if (rl.empty()) continue;
+            for (int c = 0; c < xb.cols; ++c)
+                std::copy(rl.begin(), rl.end(), P.live.begin() + xb.off + (int64_t)c * xb.rows);
+        }
     }
     patch_strips(P.wapply, strip_al, c_off);
     P.n_c = ct;

@@ -96,10 +96,12 @@ GEMMTILE = np.dtype([("g", "<i4"), ("mt", "<i4"), ("nt", "<i4"), ("ks", "<i4")],
 STRIP = np.dtype([("dst", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("ldd", "<i4"), ("s0", "<i4"),
                   ("s1", "<i4")], align=True)
 SRC = np.dtype([("off", "<i8"), ("ld", "<i4"), ("coef", "<f8")], align=True)
+WAPIECE = np.dtype([("s0", "<i4"), ("s1", "<i4"), ("r0", "<i4"), ("c0", "<i4"), ("nr", "<i4"), ("nc", "<i4"),
+                    ("dr", "<i4"), ("dc", "<i4"), ("e0", "<i4")], align=True)
 PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D, STRIP, SRC,
-               ROWMAP, ROWMAP, np.dtype(np.uint8)]
+               ROWMAP, ROWMAP, np.dtype(np.uint8), WAPIECE, np.dtype(np.int32)]
 PART_NAMES = ["arrL", "arrR", "reshape", "g1_groups", "g1_tiles", "strips", "srcs", "g2_groups", "g2_tiles",
-              "scatter", "comb_strips", "comb_srcs", "g1_rowmap", "g2_rowmap", "live"]
+              "scatter", "comb_strips", "comb_srcs", "g1_rowmap", "g2_rowmap", "live", "wa_pieces", "wa_chunks"]
 KIND = {"sigma": 0, "lstep": 1, "rstep": 2, "diag": 3}
 
 
@@ -325,11 +327,25 @@ class GpuEngine:
             sp.close()
 
     def jd(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=(), x0=None):
+        """Jacobi-Davidson on one site problem; last_timing records the host plan
+        build (with the boundary binding), the vector upload, the solve and the
+        download, and the solve's two phases (mq_plan_solve_info)."""
+        import time
+        t0 = time.perf_counter()
         sp = self.sigma_plan(symm, psi, L, R, W)
         try:
-            theta, u, its = sp.jd(self.to_dev(psi.to_flat() if x0 is None else x0), max_iter, tol, ortho)
+            x = self.to_dev(psi.to_flat() if x0 is None else x0)
+            self.sync()
+            t1 = time.perf_counter()
+            theta, u, its = sp.jd(x, max_iter, tol, ortho)
+            t2 = time.perf_counter()
+            uh = u.cpu().numpy()
+            t3 = time.perf_counter()
             self.flops += its * sp.work_flops
-            return theta, u.cpu().numpy(), its
+            info = sp.solve_info()
+            self.last_timing = dict(t_plan=t1 - t0, t_iter=t2 - t1, t_down=t3 - t2,
+                                    form_iterations=info["form_iterations"], fp64_iterations=info["fp64_iterations"])
+            return theta, uh, its
         finally:
             sp.close()
 

@@ -116,7 +116,17 @@ def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print, roots=1):
 NAMES = {MQ_PREC_FP64: "fp64", MQ_PREC_FP32: "mixed", MQ_PREC_BF16X3: "bf16x3"}
 
 
-def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd", roots=1):
+TIMERS = ("t_site", "t_plan", "t_iter", "t_down", "t_split", "t_step")
+
+
+def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd", roots=1, converge=0.0,
+        max_sweeps=0):
+    """Sweeps with the bond dimensions of `ramp`; with converge > 0, further sweeps
+    at the last bond dimension until the sweep energy changes by less than
+    `converge` (at most max_sweeps sweeps in all).  Each sweep's record sums the
+    microiterations' time per part: two-site tensor and MPO (t_site), plan build
+    with boundary binding (t_plan), the solve (t_iter), result download (t_down),
+    host SVD truncation (t_split) and boundary step (t_step)."""
     from .gpu import GpuEngine
     eng = GpuEngine(0, prec=prec)
     eng.set_ref_flops(False)  # the sweep needs no reference FLOP counts
@@ -124,15 +134,28 @@ def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd", 
         d = _driver(ham, eng, ramp[0], roots, jd_maxiter, jd_tol, seed, eigensolver)
         out = []
         t0 = time.perf_counter()
-        for s, m in enumerate(ramp):
+        ms = list(ramp)
+        s = 0
+        while s < len(ms):
+            m = ms[s]
             d.m = m
+            ts = time.perf_counter()
             E = d.sweep(s)
             recs = [r for r in d.records if r["sweep"] == s]
-            out.append(dict(sweep=s, m=m, energy=E, energies=_energies(d, E), max_bond=max(r["bond"] for r in recs),
-                            jd_seconds=sum(r.get("t_jd", r.get("t_solve", 0.0)) for r in recs),
-                            seconds=time.perf_counter() - t0))
-            log(f"  {NAMES[prec]} sweep {s} m {m} E {E:.12f} "
-                f"t {time.perf_counter() - t0:.1f}s")
+            rec = dict(sweep=s, m=m, energy=E, energies=_energies(d, E), max_bond=max(r["bond"] for r in recs),
+                       jd_seconds=sum(r.get("t_jd", r.get("t_solve", 0.0)) for r in recs),
+                       sweep_seconds=time.perf_counter() - ts, seconds=time.perf_counter() - t0,
+                       jd_iterations=sum(r.get("jd_iters", 0) for r in recs),
+                       fp64_iterations=sum(r.get("fp64_iterations", 0) for r in recs))
+            for k in TIMERS:
+                rec[k] = sum(r.get(k, 0.0) for r in recs)
+            out.append(rec)
+            log(f"  {NAMES[prec]} sweep {s} m {m} E {E:.12f} t {time.perf_counter() - t0:.1f}s ("
+                + ", ".join(f"{k[2:]} {rec[k]:.2f}" for k in TIMERS) + ")")
+            if converge > 0 and s == len(ms) - 1 and len(ms) < max_sweeps and (
+                    len(out) < 2 or abs(out[-1]["energy"] - out[-2]["energy"]) >= converge):
+                ms.append(ms[-1])
+            s += 1
         return out
     finally:
         eng.close()
@@ -151,6 +174,9 @@ def main(argv=None):
                     help="IETL_JCD (default) or IETL_DAVIDSON (SU2 only)")
     ap.add_argument("--roots", type=int, default=1,
                     help="states optimised together (state-averaged sweeps with the block Davidson)")
+    ap.add_argument("--converge", type=float, default=0.0,
+                    help="after the ramp, sweep at its last bond dimension until the energy changes by less")
+    ap.add_argument("--max-sweeps", type=int, default=30)
     ap.add_argument("--out")
     a = ap.parse_args(argv)
     if a.fcidump:
@@ -171,7 +197,8 @@ def main(argv=None):
              "bf16x3": [("bf16x3", MQ_PREC_BF16X3)],
              "both": [("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)],
              "all": [("fp64", MQ_PREC_FP64)] + list(LOW_FORMS)}[a.prec]
-    res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver, roots=a.roots)
+    res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver, roots=a.roots,
+                     converge=a.converge, max_sweeps=a.max_sweeps)
            for name, p in precs}
     rec = dict(L=ham.L, ramp=ramp, roots=a.roots, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol,
                eigensolver=a.eigensolver, results=res)
@@ -180,6 +207,9 @@ def main(argv=None):
             rec[f"delta_E_{name}"] = [m["energy"] - f["energy"] for f, m in zip(res["fp64"], res[name])]
             for s, dE in enumerate(rec[f"delta_E_{name}"]):
                 print(f"sweep {s}: E({name}) - E(fp64) = {dE:.3e} Eh")
+            rec[f"final_delta_E_{name}"] = res[name][-1]["energy"] - res["fp64"][-1]["energy"]
+            print(f"final: E({name}) - E(fp64) = {rec[f'final_delta_E_{name}']:.3e} Eh after "
+                  f"{len(res[name])} / {len(res['fp64'])} sweeps")
     if a.out:
         with open(a.out, "w") as f:
             json.dump(rec, f, indent=1)

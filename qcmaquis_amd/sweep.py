@@ -125,6 +125,7 @@ class TwoSiteDMRG:
         change = (i == L - 2) or (i == n_microiterations(L) - 1)
         t0 = time.perf_counter()
         psi, Lb, Rb, W = self.site_problem(site)
+        ts = time.perf_counter()
         theta, u, its = self.solve(ham.symm, psi, Lb, Rb, W, self.maxiter, self.tol)
         t1 = time.perf_counter()
         E = theta + ham.ecore
@@ -132,6 +133,7 @@ class TwoSiteDMRG:
         l2r = forward and not change
         A, B, info = split_twosite(ut, ham.phys, ham.phys, self.mps[site].left, self.mps[site + 1].right,
                                    self.m, self.cutoff, l2r=l2r, su2=ham.su2)
+        tsplit = time.perf_counter()
         self.mps[site], self.mps[site + 1] = A, B
         if l2r:
             self.left[site + 1] = self.engine.left_step(ham.symm, self.mps[site], self.left[site], ham.mpo[site])
@@ -149,9 +151,13 @@ class TwoSiteDMRG:
             else:
                 right_normalize_and_return(self.mps[site])
         t2 = time.perf_counter()
+        # t_jd: site problem and local solve; t_rest: truncation and boundary step, of
+        # which t_site (two-site tensor and MPO), t_split (the SVD truncation on the
+        # host) and t_step (boundary step and normalisation) are parts
         rec = dict(sweep=sweep, micro=i, site=site, energy=E, jd_iters=its, bond=info["bond_dimension"],
                    truncated_weight=info["truncated_weight"], t_jd=t1 - t0, t_rest=t2 - t1,
-                   dim=len(u))
+                   t_site=ts - t0, t_split=tsplit - t1, t_step=t2 - tsplit, dim=len(u))
+        rec.update(getattr(self.engine, "last_timing", None) or {})
         self.records.append(rec)
         self.energies.append(E)
         if self.log:
