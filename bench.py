@@ -46,7 +46,7 @@ PEAK_FP32_TFLOPS = 157.3
 # BF16x3: three bf16 MFMA products (dense bf16 peak 2.5 PFLOP/s) per FP32-equivalent product
 PEAK_BF16X3_TFLOPS = 2500.0 / 3
 PEAK_HBM_GBS = 8000.0
-STAGES = ["arrange", "reshape", "gemm_T", "w_apply", "gemm_out"]  # w_apply: 0 when fused into gemm_out
+STAGES = ["arrange", "reshape", "gemm_T", "w_apply", "gemm_out"]
 
 
 def log(*a):
@@ -78,12 +78,12 @@ def cpu_baseline(ham, psi, Lb, Rb, W, y_gpu, seconds):
 
 
 def sigma_kernels(form):
-    """Kernel names (as rocprofv3 reports them) of the stages of one sigma in a form
-    (lbtm plans: the output GEMM applies W in its operand load, k_gemm_wa)."""
+    """Kernel names (as rocprofv3 reports them) of the stages of one sigma in a form."""
     t = "double" if form == "fp64" else "float"
     x3 = "true" if form == "bf16x3" else "false"
     return {"reshape": f"k_copy2d<0, double, {t}>", "gemm_T": f"k_gemm<0, {t}, {t}, {x3}>",
-            "gemm_out": f"k_gemm_wa<{t}, double, {x3}>", "split_reduce": f"k_split_reduce<1, {t}, double>"}
+            "w_apply": f"k_wapply<0, {t}, false>", "gemm_out": f"k_gemm<1, {t}, double, {x3}>",
+            "split_reduce": f"k_split_reduce<1, {t}, double>"}
 
 
 def plan_key(form, plan):
@@ -363,8 +363,8 @@ def main():
                 traffic_profile=None if prof is None else prof["file"])
     # the slowest stage kernel on its own: executed flops and the implementation's
     # own bytes (its operands and intermediates) per launch
-    dom = max((2, 4), key=lambda i: avg[i])
-    kname = {2: kern["gemm_T"], 4: kern["gemm_out"]}[dom]
+    dom = max(range(2, 5), key=lambda i: avg[i])
+    kname = {2: kern["gemm_T"], 3: kern["w_apply"], 4: kern["gemm_out"]}[dom]
     t_s = avg[dom] * 1e-3
     ex_f = {2: work[0], 3: work[1], 4: work[2]}[dom]
     ex_b = {2: work[3], 3: work[4], 4: work[5]}[dom]

@@ -230,8 +230,7 @@ int mq_plan_bind(mq_plan *plan, const double *L, const double *R);
 int mq_plan_live_mask(const mq_plan *plan, uint8_t *mask);
 /* Instrumented mq_sigma (no reference counterpart): HIP events around each
  * stage on the plan's stream.  ms[0] boundary arrangement, ms[1] reshape,
- * ms[2] T GEMM, ms[3] W application, ms[4] output GEMM (lbtm plans apply W in
- * the output GEMM's operand load, so their ms[3] is 0).  work[0..2] are the
+ * ms[2] T GEMM, ms[3] W application, ms[4] output GEMM.  work[0..2] are the
  * stage FLOPs (T GEMM, W application axpys, output GEMM), work[3..5] the
  * minimum HBM bytes of the same three stages. */
 int mq_sigma_timed(mq_plan *plan, const double *L, const double *R,
@@ -372,8 +371,7 @@ enum mq_plan_part {
     MQ_PART_G1_TILES = 4, MQ_PART_STRIPS = 5, MQ_PART_SRCS = 6, MQ_PART_G2_GROUPS = 7,
     MQ_PART_G2_TILES = 8, MQ_PART_SCATTER = 9, MQ_PART_COMB_STRIPS = 10, MQ_PART_COMB_SRCS = 11,
     MQ_PART_G1_ROWMAP = 12, MQ_PART_G2_ROWMAP = 13,
-    MQ_PART_LIVE = 14, /* sigma plans: the mq_plan_live_mask bytes */
-    MQ_PART_WA_PIECES = 15, MQ_PART_WA_CHUNKS = 16 /* lbtm sigma plans: the output GEMM's operand program */
+    MQ_PART_LIVE = 14 /* sigma plans: the mq_plan_live_mask bytes */
 };
 int mq_host_plan_create(const mq_site_desc *desc, int kind, mq_host_plan **out);
 int mq_host_plan_part(const mq_host_plan *hp, int part, const void **ptr, int64_t *count, int64_t *elem_bytes);

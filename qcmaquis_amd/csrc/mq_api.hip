@@ -134,8 +134,6 @@ struct mq_plan {
     DevArray<Strip> strips, cstrips;
     DevArray<Src> srcs, csrcs;
     DevArray<int> packs, cpacks;  // strip packs of wapply / combL
-    DevArray<WaPiece> wa_pieces;  // output GEMM with the W application (plans with hp.wa)
-    DevArray<int> wa_chunks;
     DevGemm g1, g2;
     DevArray<double> la, rb, prp, t, c, tmp, ws;
     double w_src_el = 0, w_dst_el = 0;  // W-application source / destination elements
@@ -290,14 +288,6 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         check_packs(h.combL);
         p->packs.upload(h.wapply.packs, s);
         p->cpacks.upload(h.combL.packs, s);
-        if (h.wa) {
-            for (auto &pc : h.wa_pieces)
-                if (pc.s0 < 0 || pc.s1 < pc.s0 || pc.s1 > (int)h.wapply.srcs.size() || pc.nr < 1 || pc.nc < 1 ||
-                    pc.dr < 0 || pc.dc < 0 || pc.dc + pc.nc > kWaChunk || pc.dr + pc.nr > 128)
-                    throw std::runtime_error("output GEMM piece out of range");
-            p->wa_pieces.upload(h.wa_pieces, s);
-            p->wa_chunks.upload(h.wa_chunks, s);
-        }
         p->g1.upload(h.g1, s);
         p->g2.upload(h.g2, s);
         p->la.alloc(h.n_la);
@@ -436,8 +426,6 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
         mark(5);
         return;
     }
-    // lbtm: reshape, T GEMM, then the output GEMM that applies W in its operand load
-    if (!h.wa) throw std::runtime_error("sigma plan without its output GEMM program");
     if (prec == MQ_PREC_FP64) {
         check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp.p, s, true), "reshape");
         mark(2);
@@ -445,10 +433,12 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
                           p->prp.p, p->t.p, p->ws.p, p->g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
         mark(3);
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, p->t.p, p->c.p, ST_SIGMA, s),
+              "W apply");
         mark(4);
-        check(launch_gemm_wa(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit,
-                             p->wa_chunks.p, p->wa_pieces.p, p->srcs.p, p->t.p, p->rb.p, y, p->ws.p, s),
-              "gemm W R");
+        check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c.p,
+                          p->rb.p, y, p->ws.p, p->g2.rowmap.p, ST_SIGMA_OUT, s),
+              "gemm R");
     } else {
         check(launch_copy2d(p->reshape.p, (int)h.reshape.size(), x, p->prp32.p, s, true), "reshape");
         mark(2);
@@ -456,10 +446,12 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
                           p->prp32.p, p->t32.p, p->ws32.p, p->g1.rowmap.p, ST_SIGMA, s, x3),
               "gemm T");
         mark(3);
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, p->t32.p, p->c32.p, ST_SIGMA, s),
+              "W apply");
         mark(4);
-        check(launch_gemm_wa(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit,
-                             p->wa_chunks.p, p->wa_pieces.p, p->srcs.p, p->t32.p, p->rb32.p, y, p->ws32.p, s, x3),
-              "gemm W R");
+        check(launch_gemm(p->g2.groups.p, p->g2.tiles.p, p->g2.ntiles, p->g2.split.p, p->g2.nsplit, p->c32.p,
+                          p->rb32.p, y, p->ws32.p, p->g2.rowmap.p, ST_SIGMA_OUT, s, x3),
+              "gemm R");
     }
     mark(5);
 }
@@ -483,8 +475,7 @@ static mq_plan::BatchLists &batch_setup(mq_plan *p, int nvec, bool f32) {
         B->ncopies = (int64_t)h.reshape.size() * nvec;
         // per-vector operands advance by their arena size; shared ones (panels) by 0
         GemmList g1 = h.rbtm ? batch_gemm(h.g1, nvec, h.n_prp, 0, h.n_t) : batch_gemm(h.g1, nvec, 0, h.n_prp, h.n_t);
-        // (the lbtm output GEMM reads the T arena of its vector at the group's a)
-        GemmList g2 = h.rbtm ? batch_gemm(h.g2, nvec, 0, h.n_c, h.n_tmp) : batch_gemm(h.g2, nvec, h.n_t, 0, h.vec_len);
+        GemmList g2 = h.rbtm ? batch_gemm(h.g2, nvec, 0, h.n_c, h.n_tmp) : batch_gemm(h.g2, nvec, h.n_c, 0, h.vec_len);
         if (h.rbtm) {
             B->scatter.upload(batch_copies(h.scatter, nvec, h.n_tmp, h.vec_len), s);
             B->nscatter = (int64_t)h.scatter.size() * nvec;
@@ -568,17 +559,23 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la.p, B.prp.p,
                           B.t.p, B.ws.p, B.g1.rowmap.p, ST_SIGMA, s),
               "gemm T");
-        check(launch_gemm_wa(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, p->wa_chunks.p,
-                             p->wa_pieces.p, p->srcs.p, B.t.p, p->rb.p, Y, B.ws.p, s),
-              "gemm W R");
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, B.t.p, B.c.p, ST_SIGMA, s, nvec,
+                            h.n_t, h.n_c),
+              "W apply");
+        check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c.p, p->rb.p, Y,
+                          B.ws.p, B.g2.rowmap.p, ST_SIGMA_OUT, s),
+              "gemm R");
     } else {
         check(launch_copy2d(B.reshape.p, (int)B.ncopies, X, B.prp32.p, s, true), "reshape");
         check(launch_gemm(B.g1.groups.p, B.g1.tiles.p, B.g1.ntiles, B.g1.split.p, B.g1.nsplit, p->la32.p,
                           B.prp32.p, B.t32.p, B.ws32.p, B.g1.rowmap.p, ST_SIGMA, s, prec == MQ_PREC_BF16X3),
               "gemm T");
-        check(launch_gemm_wa(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, p->wa_chunks.p,
-                             p->wa_pieces.p, p->srcs.p, B.t32.p, p->rb32.p, Y, B.ws32.p, s, prec == MQ_PREC_BF16X3),
-              "gemm W R");
+        check(launch_wapply(p->strips.p, p->packs.p, npacks(h.wapply), p->srcs.p, B.t32.p, B.c32.p, ST_SIGMA, s,
+                            nvec, h.n_t, h.n_c),
+              "W apply");
+        check(launch_gemm(B.g2.groups.p, B.g2.tiles.p, B.g2.ntiles, B.g2.split.p, B.g2.nsplit, B.c32.p,
+                          p->rb32.p, Y, B.ws32.p, B.g2.rowmap.p, ST_SIGMA_OUT, s, prec == MQ_PREC_BF16X3),
+              "gemm R");
     }
 }
 
@@ -688,7 +685,7 @@ int mq_sigma_graph(mq_plan *plan, const double *L, const double *R, const double
 
 // Stage-timed sigma: HIP events on the plan's stream around each stage.
 // ms[0] arrange L+R (0 if already bound), ms[1] reshape, ms[2] T GEMM,
-// ms[3] W application (0 for lbtm plans), ms[4] output GEMM.  work[] returns the algorithmic
+// ms[3] W application, ms[4] output GEMM.  work[] returns the algorithmic
 // FLOPs of stages 2..4 and the bytes each stage must move at minimum.
 int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double *x, double *y, int prec,
                    float ms[5], double work[6]) {
@@ -729,10 +726,9 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
             if (h.rbtm) {  // t GEMM reads psi + RB; L GEMM reads LA + BUF and writes S (FP64), then S -> y
                 work[3] = es * (h.n_prp + h.n_rb + h.n_t);
                 work[5] = es * (h.n_c + h.n_la) + 8.0 * (2.0 * h.n_tmp + h.vec_len);
-            } else {  // the output GEMM applies W in its operand load: it reads the T sources and RB
+            } else {
                 work[3] = es * (h.n_la + h.n_prp + h.n_t);
-                work[4] = 0.0;
-                work[5] = es * (srcel + h.n_rb) + 8.0 * h.vec_len;
+                work[5] = es * (h.n_c + h.n_rb) + 8.0 * h.vec_len;
             }
         }
         return MQ_OK;
@@ -1861,8 +1857,6 @@ int mq_host_plan_part(const mq_host_plan *h, int part, const void **ptr, int64_t
         case MQ_PART_G1_ROWMAP: *ptr = p.g1.rowmap.data(); *count = p.g1.rowmap.size(); *elem = sizeof(RowMap); break;
         case MQ_PART_G2_ROWMAP: *ptr = p.g2.rowmap.data(); *count = p.g2.rowmap.size(); *elem = sizeof(RowMap); break;
         case MQ_PART_LIVE: *ptr = p.live.data(); *count = p.live.size(); *elem = 1; break;
-        case MQ_PART_WA_PIECES: *ptr = p.wa_pieces.data(); *count = p.wa_pieces.size(); *elem = sizeof(WaPiece); break;
-        case MQ_PART_WA_CHUNKS: *ptr = p.wa_chunks.data(); *count = p.wa_chunks.size(); *elem = sizeof(int); break;
         default: return MQ_ERR_ARG;
     }
     return MQ_OK;

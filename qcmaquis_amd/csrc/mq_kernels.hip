@@ -525,257 +525,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(X3 ? 3
         gemm_tile_n<64, T, TO, X3>(g, t, A, B, C, W, rowmap, lds);
 }
 
-// ------------------------------------------------------------------ output GEMM with the W application
-// C(M x N) = A(M x K) * B(K x N) where A (the sigma's C panel) is never stored:
-// chunk by chunk of kWaChunk columns the workgroup sums A's elements from the
-// W-application sources into LDS (the chunk's pieces, see WaPiece), then
-// multiplies the chunk with the matching rows of B.  One workgroup per output
-// tile and K split; 256 threads = 4 waves stacked along M, each wave TM/4 rows
-// x TN columns, with the MFMA operand feeding of k_gemm.  The sources of a
-// pass's WAU elements per thread are loaded in lock step, so their loads are
-// issued together (as in k_wapply's packs of small strips).
-constexpr int kWaPieceCap = 128;  // pieces staged in LDS per pass
-// elements per thread per pass: 8 (FP32), 4 (FP64, whose accumulators take twice the registers)
-template <class T>
-constexpr int wa_units() {
-    return sizeof(T) == 8 ? 4 : 8;
-}
-constexpr int WA_QUADS = kWaChunk / 4 + 1;
-
-template <int TM, int TN, class T, bool X3>
-struct WaLds {
-    static constexpr int a_bytes = X3 ? TM * WA_QUADS * (int)sizeof(Bf16Quad) : kWaChunk * (TM + 2) * (int)sizeof(T);
-    static constexpr int b_bytes = X3 ? TN * WA_QUADS * (int)sizeof(Bf16Quad) : kWaChunk * (TN + 2) * (int)sizeof(T);
-    static constexpr int bytes = a_bytes + b_bytes + kWaPieceCap * (int)sizeof(WaPiece);
-};
-template <class T, bool X3>
-constexpr int gemm_wa_lds_bytes() {
-    return WaLds<TMAX, 128, T, X3>::bytes;
-}
-
-template <int TM, int TN, class T, class TO, bool X3>
-__device__ __forceinline__ void gemm_wa_tile(const GemmGroup &g, const GemmTile &t, const int *__restrict__ chunks,
-                                             const WaPiece *__restrict__ pieces, const Src *__restrict__ srcs,
-                                             const T *__restrict__ Tarr, const T *__restrict__ B, TO *__restrict__ C,
-                                             T *__restrict__ W, double *lds) {
-    static_assert(!X3 || sizeof(T) == 4, "BF16x3 products take FP32 operands");
-    typedef Mfma<T> MF;
-    constexpr int KC = kWaChunk;
-    constexpr int MI = TM / 64, NJ = TN / 16;
-    typedef WaLds<TM, TN, T, X3> LD;
-    constexpr int WAU = wa_units<T>();
-    char *base = reinterpret_cast<char *>(lds);
-    T(*As)[TM + 2] = reinterpret_cast<T(*)[TM + 2]>(base);
-    T(*Bs)[TN + 2] = reinterpret_cast<T(*)[TN + 2]>(base + LD::a_bytes);
-    Bf16Quad(*Ax)[WA_QUADS] = reinterpret_cast<Bf16Quad(*)[WA_QUADS]>(base);
-    Bf16Quad(*Bx)[WA_QUADS] = reinterpret_cast<Bf16Quad(*)[WA_QUADS]>(base + LD::a_bytes);
-    WaPiece *sp = reinterpret_cast<WaPiece *>(base + LD::a_bytes + LD::b_bytes);
-    const int m0 = t.mt * TM, n0 = t.nt * TN;
-    const int kbeg = t.ks * g.kchunk;
-    const int kend = min(g.K, kbeg + g.kchunk);
-    const int nch = (max(g.K, 1) + KC - 1) / KC;
-    const int cbase = g.lda + t.mt * nch;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave * (TM / 4);
-    const int lr = lane & 15, lq = lane >> 4;
-    typename MF::acc acc[MI][NJ];
-#pragma unroll
-    for (int i = 0; i < MI; ++i)
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) acc[i][j] = typename MF::acc{0, 0, 0, 0};
-    for (int k0 = kbeg; k0 < kend; k0 += KC) {
-        const int kw = min(KC, kend - k0);
-        // A: zero where no piece reaches (rows past M, columns past the chunk's end)
-        if (m0 + TM > g.M || kw < KC) {
-            if constexpr (X3) {
-                for (int e = threadIdx.x; e < TM * WA_QUADS; e += 256) Ax[e / WA_QUADS][e % WA_QUADS] = Bf16Quad{};
-            } else {
-                for (int e = threadIdx.x; e < KC * TM; e += 256) As[e / TM][e % TM] = (T)0;
-            }
-        }
-        // B rows k0 .. k0 + KC, columns n0 .. n0 + TN (zero outside)
-#pragma unroll
-        for (int r = 0; r < KC * TN / 256; ++r) {
-            const int e = threadIdx.x + 256 * r;
-            const int k = e % KC, j = e / KC;
-            T v = 0;
-            if (k < kw && n0 + j < g.N) v = B[g.b + (k0 + k) + (int64_t)(n0 + j) * g.ldb];
-            if constexpr (X3) {
-                short h, l;
-                bf16_split(v, h, l);
-                Bx[j][k >> 2].hi[k & 3] = h;
-                Bx[j][k >> 2].lo[k & 3] = l;
-            } else {
-                Bs[k][j] = v;
-            }
-        }
-        __syncthreads();
-        // A: the chunk's pieces, up to kWaPieceCap at a time
-        const int p0 = chunks[cbase + k0 / KC], p1 = chunks[cbase + k0 / KC + 1];
-        for (int pb = p0; pb < p1; pb += kWaPieceCap) {
-            const int np = min(kWaPieceCap, p1 - pb);
-            if ((int)threadIdx.x < np) sp[threadIdx.x] = pieces[pb + threadIdx.x];
-            __syncthreads();
-            const int ebase = sp[0].e0;
-            const int total = sp[np - 1].e0 + sp[np - 1].nr * sp[np - 1].nc - ebase;
-            for (int eb = 0; eb < total; eb += 256 * WAU) {
-                int s0[WAU], s1[WAU], row[WAU], col[WAU], dst[WAU];  // dst: row | column << 16 in the chunk
-                int nsrc = 0;
-#pragma unroll
-                for (int u = 0; u < WAU; ++u) {
-                    const int e = eb + threadIdx.x + 256 * u;
-                    s0[u] = s1[u] = 0;
-                    row[u] = col[u] = dst[u] = -1;
-                    if (e < total) {
-                        int lo = 0, hi = np - 1;  // last piece with e0 - ebase <= e
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (sp[mid].e0 - ebase <= e) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        const WaPiece &pc = sp[lo];
-                        const int loc = e - (pc.e0 - ebase);
-                        const int jj = loc / pc.nr, ii = loc - jj * pc.nr;
-                        s0[u] = pc.s0;
-                        s1[u] = pc.s1;
-                        row[u] = pc.r0 + ii;
-                        col[u] = pc.c0 + jj;
-                        dst[u] = (pc.dr + ii) | ((pc.dc + jj) << 16);
-                        nsrc = max(nsrc, pc.s1 - pc.s0);
-                    }
-                }
-                T accw[WAU];
-#pragma unroll
-                for (int u = 0; u < WAU; ++u) accw[u] = 0;
-                for (int q = 0; q < nsrc; ++q) {
-                    T v[WAU], c[WAU];
-#pragma unroll
-                    for (int u = 0; u < WAU; ++u) {
-                        v[u] = 0;
-                        c[u] = 0;
-                        if (s0[u] + q < s1[u]) {
-                            const Src sr = srcs[s0[u] + q];
-                            c[u] = (T)sr.coef;
-                            v[u] = Tarr[g.a + sr.off + row[u] + (int64_t)col[u] * sr.ld];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < WAU; ++u) accw[u] += c[u] * v[u];
-                }
-#pragma unroll
-                for (int u = 0; u < WAU; ++u) {
-                    if (dst[u] < 0) continue;
-                    const int dr = dst[u] & 0xffff, dc = dst[u] >> 16;
-                    if constexpr (X3) {
-                        short h, l;
-                        bf16_split(accw[u], h, l);
-                        Ax[dr][dc >> 2].hi[dc & 3] = h;
-                        Ax[dr][dc >> 2].lo[dc & 3] = l;
-                    } else {
-                        As[dc][dr] = accw[u];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // the chunk's products
-        if constexpr (X3) {
-#pragma unroll
-            for (int h = 0; h < KC / 16; ++h) {
-                Bf16Quad av[MI];
-#pragma unroll
-                for (int i = 0; i < MI; ++i) av[i] = Ax[wm + 16 * i + lr][4 * h + lq];
-#pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    const Bf16Quad bj = Bx[16 * j + lr][4 * h + lq];
-#pragma unroll
-                    for (int i = 0; i < MI; ++i) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bj.hi, av[i].lo, acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bj.lo, av[i].hi, acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bj.hi, av[i].hi, acc[i][j], 0, 0, 0);
-                    }
-                }
-            }
-        } else {
-#pragma unroll
-            for (int kk = 0; kk < KC; kk += 4) {
-                T av[MI];
-#pragma unroll
-                for (int i = 0; i < MI; ++i) av[i] = As[kk + lq][wm + 16 * i + lr];
-#pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    T bj = Bs[kk + lq][16 * j + lr];
-#pragma unroll
-                    for (int i = 0; i < MI; ++i) acc[i][j] = MF::run(bj, av[i], acc[i][j]);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (g.split == 1) {
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-            const int gm = m0 + wm + 16 * i + lr;
-            if (gm >= g.M) continue;
-            TO *out = C + g.c + gm;
-#pragma unroll
-            for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gn = n0 + 16 * j + MF::dcol(r, lq);
-                    if (gn < g.N) out[(int64_t)gn * g.ldc] = (TO)acc[i][j][r];
-                }
-        }
-    } else {
-        T *out = W + g.part + (int64_t)t.ks * g.M * g.N;
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-            const int gm = m0 + wm + 16 * i + lr;
-            if (gm >= g.M) continue;
-#pragma unroll
-            for (int j = 0; j < NJ; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gn = n0 + 16 * j + MF::dcol(r, lq);
-                    if (gn < g.N) out[gm + (int64_t)gn * g.M] = acc[i][j][r];
-                }
-        }
-    }
-}
-
-template <int TM, class T, class TO, bool X3>
-__device__ __forceinline__ void gemm_wa_tile_n(const GemmGroup &g, const GemmTile &t, const int *chunks,
-                                               const WaPiece *pieces, const Src *srcs, const T *Tarr, const T *B,
-                                               TO *C, T *W, double *lds) {
-    switch (g.tn) {
-        case 16: gemm_wa_tile<TM, 16, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-        case 32: gemm_wa_tile<TM, 32, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-        case 48: gemm_wa_tile<TM, 48, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-        case 64: gemm_wa_tile<TM, 64, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-        case 80: gemm_wa_tile<TM, 80, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-        case 96: gemm_wa_tile<TM, 96, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-        case 112: gemm_wa_tile<TM, 112, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-        default: gemm_wa_tile<TM, 128, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds); break;
-    }
-}
-
-// Two waves per SIMD: left to itself the compiler gives the FP64 body 140 VGPRs +
-// 128 AGPRs, one wave.
-template <class T, class TO, bool X3>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_gemm_wa(const GemmGroup *__restrict__ groups, const GemmTile *__restrict__ tiles,
-                                                 const int *__restrict__ chunks, const WaPiece *__restrict__ pieces,
-                                                 const Src *__restrict__ srcs, const T *__restrict__ Tarr,
-                                                 const T *__restrict__ B, TO *__restrict__ C, T *__restrict__ W) {
-    __shared__ double lds[(gemm_wa_lds_bytes<T, X3>() + 7) / 8];
-    const GemmTile t = tiles[blockIdx.x];
-    if (t.g < 0) return;  // padding tile of the XCD-ordered list
-    const GemmGroup g = groups[t.g];
-    if (g.tm == 128)
-        gemm_wa_tile_n<128, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds);
-    else
-        gemm_wa_tile_n<64, T, TO, X3>(g, t, chunks, pieces, srcs, Tarr, B, C, W, lds);
-}
-
 // C = sum over splits of the partial tiles, in split order (deterministic),
 // accumulated in FP64.
 template <int STAGE, class T, class TO>
@@ -937,34 +686,6 @@ hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntile
                                                 s);
     return gemm_launch<float, double, false>(groups, tiles, ntiles, split_gids, nsplit, A, B, C, W, rowmap, stage,
                                              s);
-}
-
-template <class T, class TO, bool X3>
-static hipError_t gemm_wa_launch(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
-                                 int nsplit, const int *chunks, const WaPiece *pieces, const Src *srcs, const T *Tarr,
-                                 const T *B, TO *C, T *W, hipStream_t s) {
-    if (ntiles > 0)
-        hipLaunchKernelGGL((k_gemm_wa<T, TO, X3>), dim3(ntiles), dim3(256), 0, s, groups, tiles, chunks, pieces, srcs,
-                           Tarr, B, C, W);
-    if (nsplit > 0)
-        hipLaunchKernelGGL((k_split_reduce<ST_SIGMA_OUT, T, TO>), dim3(64, nsplit), dim3(256), 0, s, groups,
-                           split_gids, W, C);
-    return hipGetLastError();
-}
-hipError_t launch_gemm_wa(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
-                          int nsplit, const int *chunks, const WaPiece *pieces, const Src *srcs, const double *Tarr,
-                          const double *B, double *C, double *W, hipStream_t s) {
-    return gemm_wa_launch<double, double, false>(groups, tiles, ntiles, split_gids, nsplit, chunks, pieces, srcs, Tarr,
-                                                 B, C, W, s);
-}
-hipError_t launch_gemm_wa(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
-                          int nsplit, const int *chunks, const WaPiece *pieces, const Src *srcs, const float *Tarr,
-                          const float *B, double *C, float *W, hipStream_t s, bool bf16x3) {
-    if (bf16x3)
-        return gemm_wa_launch<float, double, true>(groups, tiles, ntiles, split_gids, nsplit, chunks, pieces, srcs,
-                                                   Tarr, B, C, W, s);
-    return gemm_wa_launch<float, double, false>(groups, tiles, ntiles, split_gids, nsplit, chunks, pieces, srcs, Tarr,
-                                                B, C, W, s);
 }
 
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,

@@ -39,15 +39,6 @@ hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntile
 hipError_t launch_gemm(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids, int nsplit,
                        const float *A, const float *B, double *C, float *W, const RowMap *rowmap, int stage,
                        hipStream_t s, bool bf16x3 = false);
-// the sigma's output GEMM with the W application in its operand load (plans with
-// wa set): C = A B with A built from the W-application sources (srcs, reading the
-// T arena at each group's a) per chunk (chunks, pieces), then split-K reduced
-hipError_t launch_gemm_wa(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
-                          int nsplit, const int *chunks, const WaPiece *pieces, const Src *srcs, const double *Tarr,
-                          const double *B, double *C, double *W, hipStream_t s);
-hipError_t launch_gemm_wa(const GemmGroup *groups, const GemmTile *tiles, int ntiles, const int *split_gids,
-                          int nsplit, const int *chunks, const WaPiece *pieces, const Src *srcs, const float *Tarr,
-                          const float *B, double *C, float *W, hipStream_t s, bool bf16x3);
 hipError_t launch_multidot(const double *const *xs, const double *const *ys, int npairs, int64_t n, int nblocks,
                            double *partial, hipStream_t s);
 // out = (x - f u) / (theta - D), elementwise, leaving the elements where theta == D

@@ -160,9 +160,7 @@ int column_tile(int N) {
     return best;
 }
 
-// chunk_align: K chunks of split groups are multiples of it (16, or kWaChunk for the
-// k_gemm_wa groups, whose operand is built kWaChunk columns at a time)
-void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target, int chunk_align = 16) {
+void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target) {
     if (g.rowmap < 0 || (g.rowmap > 0 && g.rowmap - 1 + g.M > (int64_t)gl.rowmap.size()))
         throw std::runtime_error("GEMM row map out of range");
     if ((tm != 0 && tm != 64 && tm != 128) || (tn != 0 && (tn < 16 || tn > 128 || tn % 16)))
@@ -177,7 +175,7 @@ void add_gemm(GemmList &gl, GemmGroup g, int tm, int tn, int kchunk_target, int 
     // split K when the output is small and K long: chunks of about kchunk_target
     if (g.K > 2 * kchunk_target && mt * nt < 64 && g.rowmap == 0) {
         int s = std::min(ceil_div(g.K, kchunk_target), 64);
-        int kc = ceil_div(ceil_div(g.K, s), chunk_align) * chunk_align;
+        int kc = ceil_div(ceil_div(g.K, s), 16) * 16;
         s = ceil_div(g.K, kc);
         if (s > 1) {
             g.split = s;
@@ -667,7 +665,21 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
         for (size_t i = 0; i < ord.size(); ++i) sorted[i] = comb->strips[ord[i]];
         comb->strips.swap(sorted);
     }
-    for (auto &pn : panels) {
+    // row maps of the fused panels: each panel's rows at a known offset, filled in parallel
+    std::vector<int64_t> map0(panels.size() + 1, g1 ? (int64_t)g1->rowmap.size() : 0);
+    for (size_t k = 0; k < panels.size(); ++k) map0[k + 1] = map0[k] + panels[k].rows;
+    if (g1) {
+        g1->rowmap.resize((size_t)map0.back());
+        parallel_for((int)panels.size(), [&](int k) {
+            const Panel &pn = panels[k];
+            const int rsz = s.rsize(pn.ar);
+            RowMap *rm = g1->rowmap.data() + map0[k];
+            for (auto &rb : pn.blocks)
+                for (int r = 0; r < rb.second; ++r) *rm++ = {(int64_t)rb.first * rsz + r, rb.second, 0};
+        });
+    }
+    for (size_t k = 0; k < panels.size(); ++k) {
+        const Panel &pn = panels[k];
         int rsz = s.rsize(pn.ar), ksz = s.lsize(pn.lk);
         t.gemm_flops += 2.0 * pn.rows * rsz * ksz;
         if (!g1) continue;
@@ -681,9 +693,7 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
         g.lda = pn.rows;
         g.ldb = ksz;
         g.ldc = pn.rows;
-        g.rowmap = (int64_t)g1->rowmap.size() + 1;
-        for (auto &rb : pn.blocks)
-            for (int r = 0; r < rb.second; ++r) g1->rowmap.push_back({(int64_t)rb.first * rsz + r, rb.second, 0});
+        g.rowmap = map0[k] + 1;
         add_gemm(*g1, g, tm, tn, kc);
     }
     return t;
@@ -950,7 +960,6 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     P.n_t = t.t_total;
     std::vector<int> K(s.nr, 0);
     std::vector<Seg> segs;
-    std::vector<int> seg_first;  // first strip of each segment (one strip per row range, in row order)
     std::vector<int> strip_al;
     std::vector<Raw> raw;
     // reference schedule (site_hamil.hpp:35): when it is lbtm its FLOPs are counted in this pass
@@ -971,7 +980,6 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
                 sg.koff = K[sg.al];
                 K[sg.al] += s.rsize(sg.ar);
                 segs.push_back(sg);
-                seg_first.push_back((int)P.wapply.strips.size());
                 emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, P.wapply, strip_al);
                 ref_trim += 2.0 * s.X.blk[s.x_of_r[sg.al]].rows * s.rsize(sg.ar) * s.rsize(sg.al);
             }
@@ -1011,9 +1019,7 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
         }
     }
     patch_strips(P.wapply, strip_al, c_off);
-    // the C panels are not stored: the output GEMM builds them chunk by chunk (k_gemm_wa)
-    (void)ct;
-    P.n_c = 0;
+    P.n_c = ct;
     P.n_rb = rbt;
     for (auto &sg : segs) {
         int rar = s.rsize(sg.ar), ral = s.rsize(sg.al);
@@ -1021,73 +1027,20 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
         P.arrR.push_back(mk_copy(rb_off[sg.al] + sg.koff, sg.rblk->off, rar, ral, K[sg.al], sg.rtrans ? ral : rar,
                                  sg.rtrans, sg.rscale));
     }
-    P.wa = 1;
-    std::vector<std::vector<int>> segs_of(s.nr);  // per al: its segments in K order
-    for (size_t i = 0; i < segs.size(); ++i) segs_of[segs[i].al].push_back((int)i);
     for (auto &xb : s.X.blk) {
         int r = s.ri.get(xb.c);
         GemmGroup g{};
-        g.a = 0;  // T arena base (per vector in batched lists)
+        g.a = c_off[r];
         g.b = rb_off[r];
         g.c = xb.off;
         g.M = xb.rows;
         g.N = xb.cols;
         g.K = K[r];
-        g.lda = (int)P.wa_chunks.size();  // chunk table base
+        g.lda = xb.rows;
         g.ldb = std::max(K[r], 1);
         g.ldc = xb.rows;
-        add_gemm(P.g2, g, tm, tn, kc, kWaChunk);
-        if (g.M == 0 || g.N == 0) continue;
-        const int gtm = P.g2.groups.back().tm;
-        // row ranges of al, in row order: the strips of each of its segments
-        std::vector<std::pair<int, int>> rs;  // (row0, rows)
-        for (int p = 0; p < s.np; ++p) {
-            int lb = s.rmp_l[(size_t)r * s.np + p];
-            if (lb < 0) continue;
-            int lsz = s.lsize(lb);
-            for (int ss = 0; ss < s.phys.s[p]; ++ss) rs.push_back({s.pbl[(size_t)p * s.nl + lb] + ss * lsz, lsz});
-        }
-        std::sort(rs.begin(), rs.end());
-        const std::vector<int> &sg_al = segs_of[r];
-        const int nch = ceil_div(std::max(g.K, 1), kWaChunk);
-        for (int mt = 0; mt * gtm < g.M; ++mt) {
-            const int m0 = mt * gtm, m1 = std::min(g.M, m0 + gtm);
-            size_t q0 = 0;
-            while (q0 < rs.size() && rs[q0].first + rs[q0].second <= m0) ++q0;
-            size_t gi = 0;
-            for (int c = 0; c < nch; ++c) {
-                const int k0 = c * kWaChunk, k1 = std::min(g.K, k0 + kWaChunk);
-                P.wa_chunks.push_back((int)P.wa_pieces.size());
-                int e0 = 0;
-                while (gi < sg_al.size() && segs[sg_al[gi]].koff + s.rsize(segs[sg_al[gi]].ar) <= k0) ++gi;
-                for (size_t gk = gi; gk < sg_al.size() && segs[sg_al[gk]].koff < k1; ++gk) {
-                    const Seg &sg = segs[sg_al[gk]];
-                    const int ka = std::max(k0, sg.koff), kb = std::min(k1, sg.koff + s.rsize(sg.ar));
-                    for (size_t q = q0; q < rs.size() && rs[q].first < m1; ++q) {
-                        const int ra = std::max(m0, rs[q].first), rb = std::min(m1, rs[q].first + rs[q].second);
-                        const Strip &st = P.wapply.strips[seg_first[sg_al[gk]] + q];
-                        if (st.rows != rs[q].second || st.cols != s.rsize(sg.ar))
-                            throw std::runtime_error("W-application piece outside its strip");
-                        WaPiece pc;
-                        pc.s0 = st.s0;
-                        pc.s1 = st.s1;
-                        pc.r0 = ra - rs[q].first;
-                        pc.c0 = ka - sg.koff;
-                        pc.nr = rb - ra;
-                        pc.nc = kb - ka;
-                        pc.dr = ra - m0;
-                        pc.dc = ka - k0;
-                        pc.e0 = e0;
-                        e0 += pc.nr * pc.nc;
-                        P.wa_pieces.push_back(pc);
-                    }
-                }
-                if (e0 != (m1 - m0) * (k1 - k0) && g.K > 0)
-                    throw std::runtime_error("W-application pieces do not cover an output GEMM chunk");
-            }
-        }
+        add_gemm(P.g2, g, tm, tn, kc);
     }
-    P.wa_chunks.push_back((int)P.wa_pieces.size());
     P.exec_flops = P.g1.flops() + P.g2.flops();
     for (auto &st : P.wapply.strips) P.exec_flops += 2.0 * st.rows * st.cols * (st.s1 - st.s0);
     if (!ref_count) {
@@ -1154,29 +1107,6 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
         ct += (int64_t)xb.rows * K[r];
         tmp_off[r] = tt;
         tt += (int64_t)K[r] * xb.cols;
-    }
-    // the coordinates H_eff couples: every row that a strip with a nonzero source
-    // writes, in every column of its output block (strip dst is still relative to
-    // its al block here: koff * rows + row0)
-    P.live.assign((size_t)s.X.total, 0);
-    {
-        std::vector<std::vector<uint8_t>> row_live(s.nr);  // per al: rows any strip reaches
-        for (size_t i = 0; i < P.wapply.strips.size(); ++i) {
-            const Strip &st = P.wapply.strips[i];
-            bool any = false;
-            for (int k = st.s0; k < st.s1 && !any; ++k) any = P.wapply.srcs[k].coef != 0.0;
-            if (!any) continue;
-            const VBlk &xb = s.X.blk[s.x_of_r[strip_al[i]]];
-            auto &rl = row_live[strip_al[i]];
-            if (rl.empty()) rl.assign(xb.rows, 0);
-            std::fill_n(rl.begin() + st.dst % xb.rows, st.rows, (uint8_t)1);
-        }
-        for (auto &xb : s.X.blk) {
-            const auto &rl = row_live[s.ri.get(xb.c)];
-            if (rl.empty()) continue;
-            for (int c = 0; c < xb.cols; ++c)
-                std::copy(rl.begin(), rl.end(), P.live.begin() + xb.off + (int64_t)c * xb.rows);
-        }
     }
     patch_strips(P.wapply, strip_al, c_off);
     P.n_c = ct;

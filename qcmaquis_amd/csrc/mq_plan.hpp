@@ -87,34 +87,12 @@ struct StripList {
 // (re)compute the packs of a strip list, keeping the strip order
 void make_packs(StripList &sl);
 
-// The sigma's output GEMM with the W application in its operand load
-// (k_gemm_wa): the A operand C_al of an output group is never stored.  Each
-// (row tile, K chunk of kWaChunk columns) of the group lists its pieces, the
-// parts of the W-application strips it covers: rows [dr, dr + nr) x columns
-// [dc, dc + nc) of the chunk hold rows [r0, r0 + nr) x columns [c0, c0 + nc) of
-// a strip whose sources are srcs[s0 .. s1).  e0 is the piece's first element in
-// the chunk's element order (pieces in order, each column-major).  A group's
-// chunk table starts at lda (the group's own lda is not used): chunk (mt, c)
-// has the pieces wa_chunks[lda + mt * nchunks + c] .. [+ 1].  The group's a is
-// the T arena base of its vector (0 for one vector).
-constexpr int kWaChunk = 32;
-struct WaPiece {
-    int s0, s1;
-    int r0, c0;
-    int nr, nc;
-    int dr, dc;
-    int e0;
-};
-
 enum PlanKind { PLAN_SIGMA = 0, PLAN_LSTEP = 1, PLAN_RSTEP = 2, PLAN_DIAG = 3 };
 
 // Host-side plan; the API layer uploads the task arrays.
 struct HostPlan {
     int kind = PLAN_SIGMA;
     int rbtm = 0;  // sigma plans: 1 for the rbtm schedule (build_sigma_plan_rbtm)
-    int wa = 0;    // sigma plans: 1 when g2 is the k_gemm_wa output GEMM (lbtm)
-    std::vector<WaPiece> wa_pieces;
-    std::vector<int> wa_chunks;
     int symm = MQ_SYMM_SU2U1;
     int64_t vec_len = 0;
     // arena sizes (elements)

@@ -96,12 +96,10 @@ GEMMTILE = np.dtype([("g", "<i4"), ("mt", "<i4"), ("nt", "<i4"), ("ks", "<i4")],
 STRIP = np.dtype([("dst", "<i8"), ("rows", "<i4"), ("cols", "<i4"), ("ldd", "<i4"), ("s0", "<i4"),
                   ("s1", "<i4")], align=True)
 SRC = np.dtype([("off", "<i8"), ("ld", "<i4"), ("coef", "<f8")], align=True)
-WAPIECE = np.dtype([("s0", "<i4"), ("s1", "<i4"), ("r0", "<i4"), ("c0", "<i4"), ("nr", "<i4"), ("nc", "<i4"),
-                    ("dr", "<i4"), ("dc", "<i4"), ("e0", "<i4")], align=True)
 PART_DTYPES = [COPY2D, COPY2D, COPY2D, GEMMGROUP, GEMMTILE, STRIP, SRC, GEMMGROUP, GEMMTILE, COPY2D, STRIP, SRC,
-               ROWMAP, ROWMAP, np.dtype(np.uint8), WAPIECE, np.dtype(np.int32)]
+               ROWMAP, ROWMAP, np.dtype(np.uint8)]
 PART_NAMES = ["arrL", "arrR", "reshape", "g1_groups", "g1_tiles", "strips", "srcs", "g2_groups", "g2_tiles",
-              "scatter", "comb_strips", "comb_srcs", "g1_rowmap", "g2_rowmap", "live", "wa_pieces", "wa_chunks"]
+              "scatter", "comb_strips", "comb_srcs", "g1_rowmap", "g2_rowmap", "live"]
 KIND = {"sigma": 0, "lstep": 1, "rstep": 2, "diag": 3}
 
 

@@ -106,65 +106,9 @@ def run_sigma(hp, Ldata, Rdata, x):
     copy2d(P["arrR"], Rdata, rb)
     copy2d(P["reshape"], x, prp)
     gemm(P["g1_groups"], P["g1_tiles"], la, prp, T, P["g1_rowmap"])
-    if len(P["wa_chunks"]):
-        gemm_wa(P["g2_groups"], P["g2_tiles"], P["wa_chunks"], P["wa_pieces"], P["srcs"], T, rb, y)
-    else:
-        wapply(P["strips"], P["srcs"], T, Cb)
-        gemm(P["g2_groups"], P["g2_tiles"], Cb, rb, y)
+    wapply(P["strips"], P["srcs"], T, Cb)
+    gemm(P["g2_groups"], P["g2_tiles"], Cb, rb, y)
     return y
-
-
-WA_CHUNK = 32  # kWaChunk
-
-
-def gemm_wa(groups, tiles, chunks, pieces, srcs, T, B, C):
-    """The output GEMM that applies W in its operand load (k_gemm_wa): each
-    group's A is assembled from its chunk pieces (every element of rows < M and
-    columns < K exactly once, with the sources summed as the kernel sums them),
-    then C = A B; tile coverage as gemm()."""
-    A = np.zeros(1)
-    ga = []
-    off = 0
-    for g in groups:
-        ga.append(off)
-        off += int(g["M"]) * int(g["K"])
-    A = np.zeros(max(off, 1))
-    for gi, g in enumerate(groups):
-        M, K, tm = int(g["M"]), int(g["K"]), int(g["tm"])
-        if M == 0 or int(g["N"]) == 0:
-            continue
-        nch = -(-max(K, 1) // WA_CHUNK)
-        seen = np.zeros((M, K), dtype=np.int32)
-        Ag = _mat(A, ga[gi], M, K, M) if K else None
-        for mt in range(-(-M // tm)):
-            m0 = mt * tm
-            for c in range(nch):
-                ci = int(g["lda"]) + mt * nch + c
-                k0 = c * WA_CHUNK
-                e0 = 0
-                for p in pieces[int(chunks[ci]):int(chunks[ci + 1])]:
-                    nr, nc = int(p["nr"]), int(p["nc"])
-                    assert int(p["e0"]) == e0
-                    e0 += nr * nc
-                    acc = np.zeros((nr, nc))
-                    for k in range(int(p["s0"]), int(p["s1"])):
-                        sr = srcs[k]
-                        acc += sr["coef"] * _mat(T, int(g["a"]) + int(sr["off"]) + int(p["r0"]) + int(p["c0"]) * int(sr["ld"]),
-                                                 nr, nc, int(sr["ld"]))
-                    r, cc = m0 + int(p["dr"]), k0 + int(p["dc"])
-                    assert int(p["dr"]) + nr <= tm and int(p["dc"]) + nc <= WA_CHUNK
-                    Ag[r:r + nr, cc:cc + nc] = acc
-                    seen[r:r + nr, cc:cc + nc] += 1
-        assert (seen == 1).all(), "output GEMM operand element not covered exactly once"
-    gemm(groups_as_plain(groups, ga), tiles, A, B, C)
-
-
-def groups_as_plain(groups, ga):
-    out = groups.copy()
-    for gi in range(len(out)):
-        out[gi]["a"] = ga[gi]
-        out[gi]["lda"] = max(int(out[gi]["M"]), 1)
-    return out
 
 
 def run_step(hp, kind, Bdata, A):
