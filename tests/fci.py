@@ -74,3 +74,42 @@ def fci_energies(h, g, ecore, L, na, nb, k=4):
     else:
         w = np.sort(sla.eigsh(H, k=k, which="SA", tol=1e-12)[0])
     return w + ecore
+
+
+def fci_ground_rdms(h, g, ecore, L, na, nb):
+    """Ground state of the (na, nb) sector: energy, spin-summed one-particle RDM
+    gamma_pq = <E_pq> and two-particle RDM Gamma_pqrs = <E_pq E_rs> - delta_qr <E_ps>."""
+    sa, sb = _strings(L, na), _strings(L, nb)
+    Ea, Eb = _exc_tables(L, sa), _exc_tables(L, sb)
+    Ia, Ib = sp.identity(len(sa), format="csr"), sp.identity(len(sb), format="csr")
+    E = {}
+    for p in range(L):
+        for q in range(L):
+            E[(p, q)] = (sp.kron(Ea[(p, q)], Ib) + sp.kron(Ia, Eb[(p, q)])).tocsr()
+    n = len(sa) * len(sb)
+    H = sp.csr_matrix((n, n))
+    for p in range(L):
+        for q in range(L):
+            coef = h[p, q] - 0.5 * sum(g[p, r, r, q] for r in range(L))
+            if coef != 0:
+                H = H + coef * E[(p, q)]
+    for p in range(L):
+        for q in range(L):
+            Mpq = sp.csr_matrix((n, n))
+            for r in range(L):
+                for s in range(L):
+                    if g[p, q, r, s] != 0:
+                        Mpq = Mpq + g[p, q, r, s] * E[(r, s)]
+            H = H + 0.5 * (E[(p, q)] @ Mpq)
+    H = 0.5 * (H + H.T)
+    w, v = np.linalg.eigh(H.toarray())
+    psi = v[:, 0]
+    Epsi = {k: m @ psi for k, m in E.items()}
+    gamma = np.array([[psi @ Epsi[(p, q)] for q in range(L)] for p in range(L)])
+    Gamma = np.zeros((L, L, L, L))
+    for p in range(L):
+        for q in range(L):
+            for r in range(L):
+                for s in range(L):
+                    Gamma[p, q, r, s] = Epsi[(q, p)] @ Epsi[(r, s)] - (gamma[p, s] if q == r else 0.0)
+    return w[0] + ecore, gamma, Gamma
