@@ -250,7 +250,7 @@ int mq_boundary_step(mq_plan *plan, const double *in, const double *A, double *o
 /* Jacobi-Davidson with the reference semantics: modified Gram-Schmidt with
  * one refinement, Rayleigh matrix, lowest Ritz pair, residual test
  * |r| <= tol*|theta| or |r| < tol, correction t = -r + (r.u)/(u.u) u,
- * at most max_iter sigma vectors, no restart.  x0 (device, read) is the
+ * at most max_iter sigma vectors (1..1000), no restart.  x0 (device, read) is the
  * initial guess; u (device, written) the Ritz vector; *theta the eigenvalue.
  * n_ortho vectors (device, each vec_len) are projected out as
  * SingleSiteVS::project does (ietl_lanczos_solver.h:90-95). */
@@ -329,6 +329,53 @@ int mq_plan_solve_info(const mq_plan *plan, double info[4]);
  * many plans at once and need the device memory back. */
 int mq_plan_release(mq_plan *plan);
 
+/* -- two-site split on the device ---------------------------------------- */
+/* TwoSiteTensor::split_mps_l2r / split_mps_r2l (dmrg/framework/dmrg/
+ * mp_tensors/twositetensor.hpp:141-181) with the block SVD of svd_truncate
+ * (block_matrix/block_matrix_algorithms.h:211-333).  The two-site tensor psi
+ * is a device vector in the canonical left-paired layout of
+ * (phys1*phys2, left, right) -- the layout of the sigma vectors, for
+ * MQ_SYMM_SU2U1 in the two-site reduced coupling (ts_reduction.h) -- and is
+ * split along the bond between the two sites:
+ *   1. mq_split_create builds (host) the map from psi to the blocks of its
+ *      both-paired matrix [(phys1, left), (-phys2, right)] (ts_reduction.h
+ *      unreduce_left, then ts_reshape.h:88-145), one block per charge K;
+ *   2. mq_split_decompose runs the map and a one-sided Jacobi SVD of every
+ *      block on the device (FP64) and returns the singular values;
+ *   3. the caller picks the number of singular triplets kept per block -- the
+ *      reference's global rule: the largest Mmax overall, at least
+ *      cutoff * the largest -- and mq_split_assemble writes the two site
+ *      tensors to device arrays. */
+typedef struct mq_split mq_split;
+typedef struct mq_split_desc {
+    int32_t symm;            /* enum mq_symm */
+    mq_index phys1, phys2;   /* physical indices of the first and second site */
+    mq_index left, right;    /* left index of the first site, right index of the second */
+} mq_split_desc;
+int mq_split_create(mq_ctx *ctx, const mq_split_desc *desc, mq_split **out);
+int mq_split_destroy(mq_split *sp);
+/* The blocks, in descending charge order: block b has charge charge[2b..2b+1],
+ * rows[b] = size of (phys1, left) at that charge, cols[b] = size of
+ * (-phys2, right), and min(rows[b], cols[b]) singular values; *vec_len is the
+ * length of psi.  Arrays stay owned by the handle. */
+int mq_split_blocks(const mq_split *sp, int32_t *nblocks, const int32_t **charge, const int32_t **rows,
+                    const int32_t **cols, int64_t *vec_len);
+/* Block SVD of psi (device, read).  s (host, written): block after block, the
+ * min(rows, cols) singular values of each in descending order.  Fails with
+ * MQ_ERR_STATE when a block's Jacobi sweeps do not converge.  Synchronous. */
+int mq_split_decompose(mq_split *sp, const double *psi, double *s);
+/* Jacobi sweeps each block took in the last decomposition (diagnostic). */
+int mq_split_sweeps(const mq_split *sp, int32_t *sweeps);
+/* Site tensors from the last decomposition, keeping keep[b] (0..min(rows,
+ * cols)) leading singular triplets of block b.  l2r != 0 (split_mps_l2r):
+ * A = U, B = S V^T; l2r == 0 (split_mps_r2l): A = U S, B = V^T.  A (device,
+ * written): block b column-major rows[b] x keep[b] at a_off[b] -- the
+ * canonical left-paired layout of (phys1, left, kept bond) when a_off follows
+ * it; B (device, written): block b row-major keep[b] x cols[b] at b_off[b]
+ * (the right-paired block of (phys2, kept bond, right)).  Synchronous. */
+int mq_split_assemble(mq_split *sp, const int32_t *keep, const int64_t *a_off, const int64_t *b_off, int l2r,
+                      double *A, double *B);
+
 /* -- diagnostics (no reference counterpart) ------------------------------ */
 /* C(M x N, ldc) = op(A) op(B) with the library's grouped FP64 GEMM kernel on
  * device arrays (column-major; transA: A stored K x M, transB: B stored
@@ -382,6 +429,13 @@ int mq_host_plan_out_desc(const mq_host_plan *hp, mq_boundary_desc *out);
 /* 0 lbtm, 1 rbtm (sigma plans; see MQ_OPT_SCHEDULE). */
 int mq_host_plan_schedule(const mq_host_plan *hp);
 int mq_host_plan_destroy(mq_host_plan *hp);
+/* Host-side check of mq_split's gather map (no GPU): the both-paired blocks
+ * of psi (host), block after block in mq_split_blocks order, each row-major
+ * rows x cols, computed on the host from the same task list the device runs.
+ * *nblocks and *both_len (the sum of rows[b] * cols[b]) are set; with
+ * both == NULL nothing else is written. */
+int mq_host_split_both(const mq_split_desc *desc, const double *psi, int32_t *nblocks, int64_t *both_len,
+                       double *both);
 const char *mq_host_last_error(void);
 
 #ifdef __cplusplus

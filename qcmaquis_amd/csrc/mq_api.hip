@@ -1174,8 +1174,11 @@ struct Stall {
 extern "C" int mq_jacobi_davidson(mq_plan *plan, const double *L, const double *R, const double *x0, double *u_out,
                                   double *theta_out, int max_iter, double tol, int prec, const double *const *ortho,
                                   int n_ortho, int *iterations) {
-    if (!plan || !x0 || !u_out || !theta_out || max_iter < 1 || max_iter > 60 || n_ortho < 0 || n_ortho > 60)
+    if (!plan || !x0 || !u_out || !theta_out) return MQ_ERR_ARG;
+    if (max_iter < 1 || max_iter > 1000 || n_ortho < 0 || n_ortho > 60) {
+        plan->ctx->err = "mq_jacobi_davidson: max_iter must be in 1..1000 and n_ortho in 0..60";
         return MQ_ERR_ARG;
+    }
     if (plan->hp.kind != PLAN_SIGMA) {
         plan->ctx->err = "mq_jacobi_davidson: needs a sigma plan";
         return MQ_ERR_ARG;
@@ -1803,6 +1806,152 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
     }
 }
 
+// ------------------------------------------------------------------ two-site split
+// Sweeps of the Jacobi SVD before a block is reported as not converged (the
+// sweeps converge quadratically; blocks of a few hundred columns take 6-12).
+static constexpr int kSvdMaxSweeps = 60;
+// singular values of one block kept in LDS by the Jacobi kernel
+static constexpr int kSvdMaxCols = 8192;
+
+struct mq_split {
+    mq_ctx *ctx = nullptr;
+    SplitMap map;
+    DevArray<SplitSlab> slabs;
+    DevArray<SvdBlk> blk;
+    DevArray<int32_t> order, perm, sweeps, keep;
+    DevArray<int64_t> aoff, boff;
+    DevArray<double> X, V, S;
+    std::vector<int32_t> last_sweeps;  // Jacobi sweeps per block of the last decomposition
+    bool decomposed = false;
+};
+
+extern "C" {
+
+int mq_split_create(mq_ctx *ctx, const mq_split_desc *desc, mq_split **out) {
+    if (!ctx || !desc || !out) return MQ_ERR_ARG;
+    try {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        std::unique_ptr<mq_split> sp(new mq_split());
+        sp->ctx = ctx;
+        sp->map = build_split_map(*desc);
+        const SplitMap &m = sp->map;
+        if (m.max_c > kSvdMaxCols) {
+            ctx->err = "mq_split_create: a block has more than 8192 singular values";
+            return MQ_ERR_ARG;
+        }
+        hipStream_t s = ctx->stream;
+        sp->slabs.upload(m.slabs, s);
+        sp->blk.upload(m.blk, s);
+        sp->order.upload(m.order, s);
+        sp->X.alloc((size_t)m.x_total);
+        sp->V.alloc((size_t)m.v_total);
+        sp->S.alloc((size_t)m.s_total);
+        sp->perm.alloc((size_t)m.s_total);
+        sp->sweeps.alloc(m.blk.size());
+        check(hipStreamSynchronize(s), "split upload");
+        *out = sp.release();
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(ctx, e, MQ_ERR_PLAN);
+    }
+}
+
+int mq_split_destroy(mq_split *sp) {
+    if (!sp) return MQ_ERR_ARG;
+    (void)hipStreamSynchronize(sp->ctx->stream);
+    delete sp;
+    return MQ_OK;
+}
+
+int mq_split_blocks(const mq_split *sp, int32_t *nblocks, const int32_t **charge, const int32_t **rows,
+                    const int32_t **cols, int64_t *vec_len) {
+    if (!sp) return MQ_ERR_ARG;
+    if (nblocks) *nblocks = (int32_t)sp->map.blk.size();
+    if (charge) *charge = sp->map.charge.data();
+    if (rows) *rows = sp->map.rows.data();
+    if (cols) *cols = sp->map.cols.data();
+    if (vec_len) *vec_len = sp->map.psi_len;
+    return MQ_OK;
+}
+
+int mq_split_decompose(mq_split *sp, const double *psi, double *s_out) {
+    if (!sp || !psi || !s_out) return MQ_ERR_ARG;
+    const SplitMap &m = sp->map;
+    hipStream_t s = sp->ctx->stream;
+    try {
+        if (m.x_total) check(hipMemsetAsync(sp->X.p, 0, sizeof(double) * m.x_total, s), "split clear");
+        check(launch_split_gather(sp->slabs.p, (int)m.slabs.size(), psi, sp->X.p, s), "split gather");
+        check(launch_jacobi_svd(sp->blk.p, sp->order.p, (int)m.blk.size(), m.max_c, sp->X.p, sp->V.p, sp->S.p,
+                                sp->perm.p, sp->sweeps.p, kSvdMaxSweeps, s),
+              "jacobi svd");
+        std::vector<int32_t> sw(m.blk.size());
+        if (!sw.empty())
+            check(hipMemcpyAsync(sw.data(), sp->sweeps.p, sizeof(int32_t) * sw.size(), hipMemcpyDeviceToHost, s),
+                  "split sweeps");
+        if (m.s_total)
+            check(hipMemcpyAsync(s_out, sp->S.p, sizeof(double) * m.s_total, hipMemcpyDeviceToHost, s), "split S");
+        check(hipStreamSynchronize(s), "split decompose");
+        sp->last_sweeps = sw;
+        for (size_t b = 0; b < sw.size(); ++b)
+            if (sw[b] < 0) {
+                sp->ctx->err = "mq_split_decompose: the Jacobi sweeps of block " + std::to_string(b) +
+                               " did not converge";
+                sp->decomposed = false;
+                return MQ_ERR_STATE;
+            }
+        sp->decomposed = true;
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(sp->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+int mq_split_sweeps(const mq_split *sp, int32_t *sweeps) {
+    if (!sp || !sweeps) return MQ_ERR_ARG;
+    if (!sp->decomposed) return MQ_ERR_STATE;
+    std::copy(sp->last_sweeps.begin(), sp->last_sweeps.end(), sweeps);
+    return MQ_OK;
+}
+
+int mq_split_assemble(mq_split *sp, const int32_t *keep, const int64_t *a_off, const int64_t *b_off, int l2r,
+                      double *A, double *B) {
+    if (!sp || !keep || !a_off || !b_off) return MQ_ERR_ARG;
+    if (!sp->decomposed) {
+        sp->ctx->err = "mq_split_assemble: no decomposition to assemble from";
+        return MQ_ERR_STATE;
+    }
+    const SplitMap &m = sp->map;
+    const size_t nb = m.blk.size();
+    int max_keep = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        if (keep[b] < 0 || keep[b] > m.blk[b].C) {
+            sp->ctx->err = "mq_split_assemble: keep[" + std::to_string(b) + "] outside 0..min(rows, cols)";
+            return MQ_ERR_ARG;
+        }
+        if (keep[b] > 0 && (a_off[b] < 0 || b_off[b] < 0)) {
+            sp->ctx->err = "mq_split_assemble: negative offset";
+            return MQ_ERR_ARG;
+        }
+        max_keep = std::max(max_keep, (int)keep[b]);
+    }
+    if (max_keep > 0 && (!A || !B)) return MQ_ERR_ARG;
+    hipStream_t s = sp->ctx->stream;
+    try {
+        sp->keep.upload(std::vector<int32_t>(keep, keep + nb), s);
+        sp->aoff.upload(std::vector<int64_t>(a_off, a_off + nb), s);
+        sp->boff.upload(std::vector<int64_t>(b_off, b_off + nb), s);
+        check(launch_split_assemble(sp->blk.p, (int)nb, max_keep, sp->keep.p, sp->aoff.p, sp->boff.p, l2r ? 1 : 0,
+                                    sp->X.p, sp->V.p, sp->S.p, sp->perm.p, A, B, s),
+              "split assemble");
+        check(hipStreamSynchronize(s), "split assemble");
+        return MQ_OK;
+    } catch (std::exception &e) {
+        return fail(sp->ctx, e, MQ_ERR_HIP);
+    }
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------------ host-side introspection
 struct mq_host_plan {
     HostPlan hp;
@@ -1898,6 +2047,26 @@ int mq_host_plan_schedule(const mq_host_plan *h) { return h ? h->hp.rbtm : MQ_ER
 int mq_host_plan_destroy(mq_host_plan *h) {
     delete h;
     return MQ_OK;
+}
+
+int mq_host_split_both(const mq_split_desc *desc, const double *psi, int32_t *nblocks, int64_t *both_len,
+                       double *both) {
+    if (!desc || !nblocks || !both_len) return MQ_ERR_ARG;
+    try {
+        const SplitMap m = build_split_map(*desc);
+        *nblocks = (int32_t)m.blk.size();
+        int64_t n = 0;
+        for (auto &b : m.blk) n += (int64_t)b.rows * b.cols;
+        *both_len = n;
+        if (both) {
+            if (!psi) return MQ_ERR_ARG;
+            apply_split_map_host(m, psi, both);
+        }
+        return MQ_OK;
+    } catch (std::exception &e) {
+        g_host_err = e.what();
+        return MQ_ERR_PLAN;
+    }
 }
 
 }  // extern "C"

@@ -720,3 +720,191 @@ hipError_t launch_lincomb(const double *const *vs, const double *coef, int nv, i
 }
 
 }  // namespace mq
+
+// ------------------------------------------------------------------ two-site split
+namespace mq {
+
+// psi -> SVD working matrices, one workgroup per slab (SplitSlab)
+__global__ void __launch_bounds__(256) k_split_gather(const SplitSlab *__restrict__ slabs,
+                                                      const double *__restrict__ psi, double *__restrict__ X) {
+    const SplitSlab t = slabs[blockIdx.x];
+    const int n = t.rows * t.cols;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int i = e % t.rows, j = e / t.rows;
+        const int64_t so = i + (int64_t)j * t.src_cs;
+        double v = t.c0 * psi[t.src0 + so];
+        if (t.src1 >= 0) v += t.c1 * psi[t.src1 + so];
+        X[t.dst + (int64_t)i * t.dst_rs + (int64_t)j * t.dst_cs] = v;
+    }
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One-sided (Hestenes) Jacobi SVD of one working matrix X (R x C, C <= R,
+// columns contiguous) per workgroup: sweeps of plane rotations that make the
+// columns of X pairwise orthogonal while V (C x C, from the identity) collects
+// them, so that X V_final = U diag(sigma) with sigma_j the final column norms.
+// Each sweep visits every column pair once in round-robin order: round t pairs
+// column Ce-1 (a padding column when C is odd) with column t, and column
+// (t + k) mod (Ce-1) with (t - k) mod (Ce-1) for k = 1 .. Ce/2-1; the pairs of
+// a round are disjoint, one per wave.  A pair is rotated when
+// |x_p . x_q| > eps sqrt(R) |x_p| |x_q|; the sweeps stop when a whole sweep
+// rotates nothing.  Then sigma is sorted in descending order (ties by column)
+// into S and the column permutation into perm; sweeps[block] is the number of
+// sweeps, or -1 when max_sweeps did not converge.
+__global__ void __launch_bounds__(1024) k_jacobi_svd(const SvdBlk *__restrict__ blks, const int32_t *__restrict__ order,
+                                                     double *__restrict__ X, double *__restrict__ V,
+                                                     double *__restrict__ S, int32_t *__restrict__ perm,
+                                                     int32_t *__restrict__ sweeps, int max_sweeps) {
+    extern __shared__ double sig[];
+    __shared__ int rotated;
+    const int bi = order[blockIdx.x];
+    const SvdBlk b = blks[bi];
+    const int R = b.R, C = b.C;
+    double *x = X + b.x_off, *v = V + b.v_off;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int e = tid; e < C * C; e += blockDim.x) v[e] = (e % (C + 1) == 0) ? 1.0 : 0.0;
+    const int Ce = C + (C & 1);
+    const double tol = 2.220446049250313e-16 * sqrt((double)R);
+    int sweep = 0;
+    bool done = C < 2;
+    __syncthreads();
+    while (!done && sweep < max_sweeps) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int t = 0; t < Ce - 1; ++t) {
+            for (int k = wave; k < Ce / 2; k += nw) {
+                int p, q;
+                if (k == 0) {
+                    p = t;
+                    q = Ce - 1;
+                } else {
+                    p = (t + k) % (Ce - 1);
+                    q = (t - k + Ce - 1) % (Ce - 1);
+                }
+                if (p > q) {
+                    const int z = p;
+                    p = q;
+                    q = z;
+                }
+                if (q >= C) continue;
+                double *xp = x + (int64_t)p * R, *xq = x + (int64_t)q * R;
+                double a = 0.0, bb = 0.0, g = 0.0;
+                for (int i = lane; i < R; i += 64) {
+                    const double u = xp[i], w = xq[i];
+                    a += u * u;
+                    bb += w * w;
+                    g += u * w;
+                }
+                a = wave_sum(a);
+                bb = wave_sum(bb);
+                g = wave_sum(g);
+                if (fabs(g) > tol * sqrt(a * bb)) {
+                    const double zeta = (bb - a) / (2.0 * g);
+                    const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    if (tt != 0.0) {
+                        const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
+                        for (int i = lane; i < R; i += 64) {
+                            const double u = xp[i], w = xq[i];
+                            xp[i] = cs * u - sn * w;
+                            xq[i] = sn * u + cs * w;
+                        }
+                        double *vp = v + (int64_t)p * C, *vq = v + (int64_t)q * C;
+                        for (int i = lane; i < C; i += 64) {
+                            const double u = vp[i], w = vq[i];
+                            vp[i] = cs * u - sn * w;
+                            vq[i] = sn * u + cs * w;
+                        }
+                        if (lane == 0) rotated = 1;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        ++sweep;
+        done = rotated == 0;
+        __syncthreads();
+    }
+    for (int j = wave; j < C; j += nw) {
+        const double *xj = x + (int64_t)j * R;
+        double a = 0.0;
+        for (int i = lane; i < R; i += 64) a += xj[i] * xj[i];
+        a = wave_sum(a);
+        if (lane == 0) sig[j] = sqrt(a);
+    }
+    __syncthreads();
+    for (int j = tid; j < C; j += blockDim.x) {
+        const double sj = sig[j];
+        int r = 0;
+        for (int k = 0; k < C; ++k) {
+            const double sk = sig[k];
+            r += (sk > sj) || (sk == sj && k < j);
+        }
+        S[b.s_off + r] = sj;
+        perm[b.s_off + r] = j;
+    }
+    if (tid == 0) sweeps[bi] = done ? sweep : -1;
+}
+
+// The two site tensors from the decomposition: for kept triplet k of block b
+// (column j = perm[k] of the working matrices) the left vector u (X_j / sigma
+// or V_j) goes to column k of A and the right vector w (V_j or X_j / sigma) to
+// row k of B, with sigma on B (l2r) or on A (r2l).
+__global__ void __launch_bounds__(256) k_split_assemble(const SvdBlk *__restrict__ blks,
+                                                        const int32_t *__restrict__ keep,
+                                                        const int64_t *__restrict__ a_off,
+                                                        const int64_t *__restrict__ b_off, int l2r,
+                                                        const double *__restrict__ X, const double *__restrict__ V,
+                                                        const double *__restrict__ S, const int32_t *__restrict__ perm,
+                                                        double *__restrict__ A, double *__restrict__ B) {
+    const int bi = blockIdx.y;
+    const SvdBlk b = blks[bi];
+    const int K = keep[bi];
+    for (int k = blockIdx.x; k < K; k += gridDim.x) {
+        const int j = perm[b.s_off + k];
+        const double s = S[b.s_off + k];
+        const double inv = s > 0.0 ? 1.0 / s : 0.0;
+        const double *xj = X + b.x_off + (int64_t)j * b.R;
+        const double *vj = V + b.v_off + (int64_t)j * b.C;
+        const double *lu = b.trans ? vj : xj;
+        const double *rw = b.trans ? xj : vj;
+        double ls = b.trans ? 1.0 : inv, rs = b.trans ? inv : 1.0;
+        if (l2r)
+            rs *= s;
+        else
+            ls *= s;
+        double *a = A + a_off[bi] + (int64_t)k * b.rows;
+        for (int i = threadIdx.x; i < b.rows; i += blockDim.x) a[i] = ls * lu[i];
+        double *w = B + b_off[bi] + (int64_t)k * b.cols;
+        for (int i = threadIdx.x; i < b.cols; i += blockDim.x) w[i] = rs * rw[i];
+    }
+}
+
+hipError_t launch_split_gather(const SplitSlab *slabs, int n, const double *psi, double *X, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_split_gather, dim3(n), dim3(256), 0, s, slabs, psi, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_svd(const SvdBlk *blks, const int32_t *order, int nblk, int max_c, double *X, double *V,
+                             double *S, int32_t *perm, int32_t *sweeps, int max_sweeps, hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_svd, dim3(nblk), dim3(1024), sizeof(double) * std::max(1, max_c), s, blks, order, X,
+                       V, S, perm, sweeps, max_sweeps);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_assemble(const SvdBlk *blks, int nblk, int max_keep, const int32_t *keep,
+                                 const int64_t *a_off, const int64_t *b_off, int l2r, const double *X,
+                                 const double *V, const double *S, const int32_t *perm, double *A, double *B,
+                                 hipStream_t s) {
+    if (nblk <= 0 || max_keep <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_split_assemble, dim3(std::min(max_keep, 256), nblk), dim3(256), 0, s, blks, keep, a_off,
+                       b_off, l2r, X, V, S, perm, A, B);
+    return hipGetLastError();
+}
+
+}  // namespace mq

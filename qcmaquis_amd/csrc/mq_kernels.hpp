@@ -47,5 +47,15 @@ hipError_t launch_diag_scale(const double *x, const double *u, double f, const d
                              int64_t n, hipStream_t s);
 hipError_t launch_lincomb(const double *const *vs, const double *coef, int nv, int64_t n, double beta, double *out,
                           hipStream_t s);
+// two-site split: gather psi into the SVD working matrices (X must be zeroed
+// first), one-sided Jacobi SVD per block (order: launch order of the blocks,
+// max_c: the largest C), and the assembly of the two site tensors
+hipError_t launch_split_gather(const SplitSlab *slabs, int n, const double *psi, double *X, hipStream_t s);
+hipError_t launch_jacobi_svd(const SvdBlk *blks, const int32_t *order, int nblk, int max_c, double *X, double *V,
+                             double *S, int32_t *perm, int32_t *sweeps, int max_sweeps, hipStream_t s);
+hipError_t launch_split_assemble(const SvdBlk *blks, int nblk, int max_keep, const int32_t *keep,
+                                 const int64_t *a_off, const int64_t *b_off, int l2r, const double *X,
+                                 const double *V, const double *S, const int32_t *perm, double *A, double *B,
+                                 hipStream_t s);
 
 }  // namespace mq

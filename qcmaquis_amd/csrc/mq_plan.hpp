@@ -169,4 +169,43 @@ GemmList batch_gemm(const GemmList &gl, int nvec, int64_t a_step, int64_t b_step
 // FLOPs of the reference schedule (site_hamil.hpp:28-204 for sigma, move_boundary.hpp for steps)
 double reference_sigma_flops(const mq_site_desc &d);
 
+// ---- two-site split ----
+// One slab of the map from the two-site tensor psi (canonical left-paired,
+// column-major blocks) to the working matrices of the block SVD:
+// dst[i*dst_rs + j*dst_cs] = c0 * psi[src0 + i + j*src_cs] (+ c1 * psi[src1 + ...]
+// when src1 >= 0), i < rows, j < cols.
+struct SplitSlab {
+    int64_t dst, src0, src1;
+    double c0, c1;
+    int32_t rows, cols;
+    int32_t dst_rs, dst_cs, src_cs;
+    int32_t pad;
+};
+// One both-paired block (rows x cols) and its SVD working arrays: the R x C
+// working matrix X (C = min(rows, cols), columns contiguous) at x_off is the
+// block itself (trans = 0, rows >= cols) or its transpose (trans = 1); V
+// (C x C, columns contiguous) at v_off; singular values and their column
+// permutation at s_off.
+struct SvdBlk {
+    int64_t x_off, v_off, s_off;
+    int32_t R, C;
+    int32_t rows, cols;
+    int32_t trans;
+    int32_t pad;
+};
+struct SplitMap {
+    std::vector<SplitSlab> slabs;
+    std::vector<SvdBlk> blk;                  // descending charge
+    std::vector<int32_t> charge, rows, cols;  // per block, in blk order
+    std::vector<int32_t> order;               // block indices, largest working matrix first (launch order)
+    int64_t psi_len = 0, x_total = 0, v_total = 0, s_total = 0;
+    int max_c = 0;
+};
+// The map of TwoSiteTensor::make_both_paired (ts_reduction.h unreduce_left for
+// SU2U1, then ts_reshape.h:88-145 reshape_left_to_both).
+SplitMap build_split_map(const mq_split_desc &d);
+// The same map applied on the host (test introspection): both-paired blocks,
+// row-major, one after another in blk order.
+void apply_split_map_host(const SplitMap &m, const double *psi, double *both);
+
 }  // namespace mq

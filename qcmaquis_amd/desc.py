@@ -202,6 +202,22 @@ def mpo_arrays(w):
     return a
 
 
+class mq_split_desc(C.Structure):
+    _fields_ = [("symm", C.c_int32), ("phys1", mq_index), ("phys2", mq_index), ("left", mq_index),
+                ("right", mq_index)]
+
+
+class SplitDesc:
+    """Owns the numpy arrays behind one mq_split_desc (two-site split)."""
+
+    def __init__(self, symm, phys1, phys2, left, right):
+        self.keep = [IndexArrays(phys1), IndexArrays(phys2), IndexArrays(left), IndexArrays(right)]
+        self.desc = mq_split_desc(symm, *(k.c() for k in self.keep))
+
+    def ptr(self):
+        return C.byref(self.desc)
+
+
 class SiteDesc:
     """Owns the numpy arrays behind one mq_site_desc."""
 

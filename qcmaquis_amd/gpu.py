@@ -12,7 +12,8 @@ import os
 
 import numpy as np
 
-from .desc import BoundaryLayout, SiteDesc, MQ_PREC_FP64, f64p, i32p, i64p, mq_boundary_desc
+from .desc import (BoundaryLayout, SiteDesc, SplitDesc, MQ_PREC_FP64, MQ_SYMM_SU2U1, MQ_SYMM_TWOU1, f64p, i32p, i64p,
+                   mq_boundary_desc)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "_lib", "libmqgpu.so")
@@ -79,6 +80,14 @@ def lib():
         L.mq_host_plan_destroy.argtypes = [vp]
         L.mq_host_plan_schedule.argtypes = [vp]
         L.mq_plan_schedule.argtypes = [vp]
+        L.mq_split_create.argtypes = [vp, vp, C.POINTER(vp)]
+        L.mq_split_destroy.argtypes = [vp]
+        L.mq_split_blocks.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(i32p), C.POINTER(i32p), C.POINTER(i32p),
+                                      C.POINTER(C.c_int64)]
+        L.mq_split_decompose.argtypes = [vp, vp, f64p]
+        L.mq_split_assemble.argtypes = [vp, i32p, i64p, i64p, C.c_int, vp, vp]
+        L.mq_split_sweeps.argtypes = [vp, i32p]
+        L.mq_host_split_both.argtypes = [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), vp]
         L.mq_host_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -140,6 +149,24 @@ class HostPlan:
                 self.out_layout = BoundaryLayout.from_c(d)
         finally:
             L.mq_host_plan_destroy(h)
+
+
+def host_split_both(p1, p2, left, right, psi, su2=True):
+    """The both-paired blocks of the two-site tensor psi (host, canonical
+    left-paired flat) from mq_split's gather map applied on the host
+    (mq_host_split_both, a check of the map; no GPU).  Returns a list of
+    row-major (rows, cols) arrays in mq_split_blocks order."""
+    L = lib()
+    d = SplitDesc(MQ_SYMM_SU2U1 if su2 else MQ_SYMM_TWOU1, p1, p2, left, right)
+    nb, n = C.c_int32(), C.c_int64()
+    if L.mq_host_split_both(d.ptr(), None, C.byref(nb), C.byref(n), None) != 0:
+        raise MQError(L.mq_host_last_error().decode())
+    x = np.ascontiguousarray(psi, dtype=np.float64)
+    out = np.zeros(max(n.value, 1))
+    if L.mq_host_split_both(d.ptr(), x.ctypes.data_as(C.c_void_p), C.byref(nb), C.byref(n),
+                            out.ctypes.data_as(C.c_void_p)) != 0:
+        raise MQError(L.mq_host_last_error().decode())
+    return out[:n.value]
 
 
 # --------------------------------------------------------------------------- GPU engine
@@ -338,6 +365,7 @@ class GpuEngine:
             theta, u, its = sp.jd(x, max_iter, tol, ortho)
             t2 = time.perf_counter()
             uh = u.cpu().numpy()
+            self._last_u = (uh, u)  # split_twosite of this vector uses the device copy
             t3 = time.perf_counter()
             self.flops += its * sp.work_flops
             info = sp.solve_info()
@@ -346,6 +374,86 @@ class GpuEngine:
             return theta, uh, its
         finally:
             sp.close()
+
+    def split_twosite(self, u, p1, p2, left, right, Mmax, cutoff, l2r=True, su2=True):
+        """TwoSiteTensor::split_mps_l2r / split_mps_r2l (twositetensor.hpp:141-181)
+        on the device: mq_split's map to the both-paired blocks and block Jacobi
+        SVD, the reference's global truncation rule on the singular values
+        (mps.truncation_keep), and the two site tensors assembled on the device.
+        u: the two-site tensor as a canonical left-paired flat vector -- the
+        vector the last solve returned (its device copy is used) or any host
+        array.  Returns (A, B, info) as mps.split_twosite: A left-paired site
+        tensor of (p1, left, kept bond), B right-paired of (p2, kept bond, right);
+        l2r puts the singular values on B, r2l on A.  last_split records the
+        host-side times."""
+        import time
+        from .blocks import BlockMatrix, Index
+        from .mps import MPSTensor, canonical_layout, truncation_keep
+        t = self.torch
+        L = lib()
+        t0 = time.perf_counter()
+        d = SplitDesc(MQ_SYMM_SU2U1 if su2 else MQ_SYMM_TWOU1, p1, p2, left, right)
+        sp = C.c_void_p()
+        self.check(L.mq_split_create(self.ctx, d.ptr(), C.byref(sp)), "mq_split_create")
+        try:
+            nb, n = C.c_int32(), C.c_int64()
+            ch, rw, cl = i32p(), i32p(), i32p()
+            self.check(L.mq_split_blocks(sp, C.byref(nb), C.byref(ch), C.byref(rw), C.byref(cl), C.byref(n)),
+                       "mq_split_blocks")
+            nb = nb.value
+            charges = [(ch[2 * b], ch[2 * b + 1]) for b in range(nb)]
+            rows = [rw[b] for b in range(nb)]
+            cols = [cl[b] for b in range(nb)]
+            last = getattr(self, "_last_u", None)
+            if last is not None and last[0] is u:
+                ud = last[1]
+            else:
+                ud = self.to_dev(u)
+            if ud.numel() != n.value:
+                raise MQError(f"split: the two-site vector has {ud.numel()} elements, the layout {n.value}")
+            nsv = [min(r, c) for r, c in zip(rows, cols)]
+            s = np.zeros(max(sum(nsv), 1))
+            t1 = time.perf_counter()
+            t.cuda.synchronize()
+            self.check(L.mq_split_decompose(sp, _ptr(ud), s.ctypes.data_as(f64p)), "mq_split_decompose")
+            t2 = time.perf_counter()
+            sw = np.zeros(max(nb, 1), np.int32)
+            L.mq_split_sweeps(sp, sw.ctypes.data_as(i32p))
+            self.last_split_blocks = dict(rows=rows, cols=cols, sweeps=sw[:nb].tolist())
+            svals, o = [], 0
+            for k in nsv:
+                svals.append(s[o:o + k])
+                o += k
+            keep, info = truncation_keep(svals, Mmax, cutoff)
+            mid = Index.from_dict({q: k for q, k in zip(charges, keep) if k > 0})
+            lay, na = canonical_layout(p1, left, mid)
+            aoff = {c: off for c, _, _, off in lay}
+            a_off = np.array([aoff.get(q, 0) if k else 0 for q, k in zip(charges, keep)], np.int64)
+            b_off = np.zeros(nb, np.int64)
+            nbv = 0
+            for b in range(nb):
+                b_off[b] = nbv
+                nbv += keep[b] * cols[b]
+            Ad = t.empty(max(na, 1), dtype=t.float64, device=self.device)
+            Bd = t.empty(max(nbv, 1), dtype=t.float64, device=self.device)
+            kp = np.array(keep, np.int32)
+            self.check(L.mq_split_assemble(sp, kp.ctypes.data_as(i32p), a_off.ctypes.data_as(i64p),
+                                           b_off.ctypes.data_as(i64p), 1 if l2r else 0, _ptr(Ad), _ptr(Bd)),
+                       "mq_split_assemble")
+            Ah = Ad.cpu().numpy()[:na]
+            Bh = Bd.cpu().numpy()[:nbv]
+            A = MPSTensor.from_flat(p1, left, mid, Ah)
+            Bb = BlockMatrix()
+            for b in range(nb):
+                if keep[b]:
+                    Bb.blocks[(charges[b], charges[b])] = Bh[b_off[b]:b_off[b] + keep[b] * cols[b]].reshape(
+                        keep[b], cols[b])
+            B = MPSTensor(p2, mid, right, Bb, "R")
+            t3 = time.perf_counter()
+            self.last_split = dict(t_map=t1 - t0, t_decompose=t2 - t1, t_assemble=t3 - t2)
+            return A, B, info
+        finally:
+            L.mq_split_destroy(sp)
 
     def davidson(self, symm, psi, L, R, W, max_iter=10, tol=1e-8, ortho=(), x0=None):
         """Davidson with the diagonal preconditioner (SU2U1), solve_ietl_davidson."""

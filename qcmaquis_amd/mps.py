@@ -527,31 +527,43 @@ def make_twosite_mps(m1: MPSTensor, m2: MPSTensor, su2=True) -> MPSTensor:
     return t.make_left_paired()
 
 
-def svd_truncate(M: BlockMatrix, Mmax, cutoff):
-    """Block SVD with global truncation (block_matrix_algorithms.h:211-333)."""
-    U, S, V = {}, {}, {}
-    for k, m in M.blocks.items():
-        u, s, vt = np.linalg.svd(m, full_matrices=False)
-        U[k], S[k], V[k] = u, s, vt
-    allv = np.sort(np.concatenate([S[k] for k in S]))[::-1] if S else np.zeros(0)
+def truncation_keep(svals, Mmax, cutoff):
+    """Global truncation rule of estimate_truncation (block_matrix_algorithms.h:
+    211-262): the threshold is cutoff * the largest singular value, raised to the
+    (Mmax+1)-th largest when there are more than Mmax, and every value at or
+    above it is kept (so a bond can keep Mmax + 1 states, as the reference does).
+    svals: per block, singular values in descending order.  Returns the
+    per-block kept counts and the truncation record."""
+    allv = np.sort(np.concatenate(list(svals)))[::-1] if len(svals) else np.zeros(0)
+    if allv.size == 0 or allv[0] == 0.0:
+        return [0] * len(svals), dict(bond_dimension=0, truncated_weight=0.0, smallest_ev=0.0)
     evalscut = cutoff * allv[0]
     if len(allv) > Mmax:
         evalscut = max(evalscut, allv[Mmax])
     smallest_ev = evalscut / allv[0]
     trunc = allv[allv < evalscut]
     tw = float(np.sum(trunc ** 2) / np.sum(allv ** 2))
+    keep = [_keep_count(s, evalscut) for s in svals]
+    return keep, dict(bond_dimension=int(sum(keep)), truncated_weight=tw, smallest_ev=smallest_ev)
+
+
+def svd_truncate(M: BlockMatrix, Mmax, cutoff):
+    """Block SVD with global truncation (block_matrix_algorithms.h:211-333)."""
+    U, S, V = {}, {}, {}
+    for k, m in M.blocks.items():
+        u, s, vt = np.linalg.svd(m, full_matrices=False)
+        U[k], S[k], V[k] = u, s, vt
+    keys = list(M.blocks)
+    keep, info = truncation_keep([S[k] for k in keys], Mmax, cutoff)
     Ub, Sb, Vb = BlockMatrix(), {}, BlockMatrix()
-    for k in M.blocks:
-        s = S[k]
-        keep = _keep_count(s, evalscut)
-        if keep == 0:
+    for k, kk in zip(keys, keep):
+        if kk == 0:
             continue
         lc, rc = k
-        Ub.blocks[(lc, rc)] = U[k][:, :keep]
-        Sb[rc] = s[:keep]
-        Vb.blocks[(lc, rc)] = V[k][:keep, :]
-    bond = sum(len(v) for v in Sb.values())
-    return Ub, Sb, Vb, dict(bond_dimension=bond, truncated_weight=tw, smallest_ev=smallest_ev)
+        Ub.blocks[(lc, rc)] = U[k][:, :kk]
+        Sb[rc] = S[k][:kk]
+        Vb.blocks[(lc, rc)] = V[k][:kk, :]
+    return Ub, Sb, Vb, info
 
 
 def _keep_count(s, cut):

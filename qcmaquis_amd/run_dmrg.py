@@ -1,7 +1,8 @@
 """Two-site DMRG sweeps on one GPU.
 
     python -m qcmaquis_amd.run_dmrg --L 24 --ramp 64,128,256,512,512,512 \
-        [--fcidump FILE] [--prec fp64|mixed|bf16x3|both|all|compare] [--jd-maxiter 8] [--out result.json]
+        [--fcidump FILE] [--prec fp64|mixed|bf16x3|both|all|compare|fp64,bf16x3] [--jd-maxiter 8] \
+        [--converge 1e-9 --max-sweeps 30] [--out result.json]
 
 Without --fcidump the synthetic L-orbital half-filled singlet Hamiltonian
 (qcmaquis_amd.integrals.synthetic, seed 1000 + L) is used.  The sweep order,
@@ -75,11 +76,12 @@ class _CompareEngine:
             sp.close()
 
 
-def _driver(ham, eng, m, roots, jd_maxiter, jd_tol, seed, eigensolver="jd"):
+def _driver(ham, eng, m, roots, jd_maxiter, jd_tol, seed, eigensolver="jd", split="engine"):
     from .sweep import StateAveragedDMRG
     if roots > 1:
         return StateAveragedDMRG(ham, eng, m, roots, jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed)
-    return TwoSiteDMRG(ham, eng, m, jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed, eigensolver=eigensolver)
+    return TwoSiteDMRG(ham, eng, m, jd_maxiter=jd_maxiter, jd_tol=jd_tol, seed=seed, eigensolver=eigensolver,
+                       split=split)
 
 
 def _energies(d, E):
@@ -120,18 +122,19 @@ TIMERS = ("t_site", "t_plan", "t_iter", "t_down", "t_split", "t_step")
 
 
 def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd", roots=1, converge=0.0,
-        max_sweeps=0):
+        max_sweeps=0, split="engine"):
     """Sweeps with the bond dimensions of `ramp`; with converge > 0, further sweeps
     at the last bond dimension until the sweep energy changes by less than
     `converge` (at most max_sweeps sweeps in all).  Each sweep's record sums the
     microiterations' time per part: two-site tensor and MPO (t_site), plan build
     with boundary binding (t_plan), the solve (t_iter), result download (t_down),
-    host SVD truncation (t_split) and boundary step (t_step)."""
+    SVD truncation (t_split: on the device, or with split="host" on the host) and
+    boundary step (t_step)."""
     from .gpu import GpuEngine
     eng = GpuEngine(0, prec=prec)
     eng.set_ref_flops(False)  # the sweep needs no reference FLOP counts
     try:
-        d = _driver(ham, eng, ramp[0], roots, jd_maxiter, jd_tol, seed, eigensolver)
+        d = _driver(ham, eng, ramp[0], roots, jd_maxiter, jd_tol, seed, eigensolver, split)
         out = []
         t0 = time.perf_counter()
         ms = list(ramp)
@@ -166,7 +169,8 @@ def main(argv=None):
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--fcidump")
     ap.add_argument("--ramp", default="64,64,64,64")
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "bf16x3", "both", "all", "compare"])
+    ap.add_argument("--prec", default="fp64",
+                    help="fp64, mixed, bf16x3, both (fp64,mixed), all, compare, or a comma list of forms")
     ap.add_argument("--jd-maxiter", type=int, default=8)
     ap.add_argument("--jd-tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=42)
@@ -177,6 +181,8 @@ def main(argv=None):
     ap.add_argument("--converge", type=float, default=0.0,
                     help="after the ramp, sweep at its last bond dimension until the energy changes by less")
     ap.add_argument("--max-sweeps", type=int, default=30)
+    ap.add_argument("--split", default="engine", choices=["engine", "host"],
+                    help="SVD truncation on the device (engine, default) or on the host")
     ap.add_argument("--out")
     a = ap.parse_args(argv)
     if a.fcidump:
@@ -193,15 +199,17 @@ def main(argv=None):
             with open(a.out, "w") as f:
                 json.dump(rec, f, indent=1)
         return rec
-    precs = {"fp64": [("fp64", MQ_PREC_FP64)], "mixed": [("mixed", MQ_PREC_FP32)],
-             "bf16x3": [("bf16x3", MQ_PREC_BF16X3)],
-             "both": [("fp64", MQ_PREC_FP64), ("mixed", MQ_PREC_FP32)],
-             "all": [("fp64", MQ_PREC_FP64)] + list(LOW_FORMS)}[a.prec]
+    forms = {"fp64": MQ_PREC_FP64, "mixed": MQ_PREC_FP32, "bf16x3": MQ_PREC_BF16X3}
+    names = {"both": "fp64,mixed", "all": "fp64,mixed,bf16x3"}.get(a.prec, a.prec).split(",")
+    unknown = [n for n in names if n not in forms]
+    if unknown:
+        ap.error(f"unknown form(s) {unknown} in --prec {a.prec}")
+    precs = [(n, forms[n]) for n in names]
     res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver, roots=a.roots,
-                     converge=a.converge, max_sweeps=a.max_sweeps)
+                     converge=a.converge, max_sweeps=a.max_sweeps, split=a.split)
            for name, p in precs}
     rec = dict(L=ham.L, ramp=ramp, roots=a.roots, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol,
-               eigensolver=a.eigensolver, results=res)
+               eigensolver=a.eigensolver, split=a.split, results=res)
     for name, _ in LOW_FORMS:
         if "fp64" in res and name in res:
             rec[f"delta_E_{name}"] = [m["energy"] - f["energy"] for f, m in zip(res["fp64"], res[name])]

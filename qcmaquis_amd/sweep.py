@@ -85,12 +85,17 @@ def micro_site(L, i):
 
 class TwoSiteDMRG:
     def __init__(self, ham: Hamiltonian, engine, max_bond_dimension, init_bond_dimension=5, seed=42,
-                 cutoff=1e-16, jd_tol=1e-8, jd_maxiter=10, mps=None, log=None, eigensolver="jd"):
+                 cutoff=1e-16, jd_tol=1e-8, jd_maxiter=10, mps=None, log=None, eigensolver="jd", split="engine"):
         """eigensolver: "jd" (Jacobi-Davidson, IETL_JCD) or "davidson" (diagonal-preconditioned
         Davidson, IETL_DAVIDSON, SU2 only), as parms["eigensolver"] selects in
-        optimize/ts_optimize.hpp:139-147; both take ietl_jcd_maxiter / ietl_jcd_tol."""
+        optimize/ts_optimize.hpp:139-147; both take ietl_jcd_maxiter / ietl_jcd_tol.
+        split: "engine" truncates with the engine's split_twosite when it has one
+        (the GPU engine: block SVD on the device), "host" with mps.split_twosite."""
         if eigensolver not in ("jd", "davidson"):
             raise ValueError(f"unknown eigensolver {eigensolver!r}")
+        if split not in ("engine", "host"):
+            raise ValueError(f"unknown split {split!r}")
+        self.split = split
         self.solve = engine.jd if eigensolver == "jd" else engine.davidson
         self.ham = ham
         self.engine = engine
@@ -129,10 +134,15 @@ class TwoSiteDMRG:
         theta, u, its = self.solve(ham.symm, psi, Lb, Rb, W, self.maxiter, self.tol)
         t1 = time.perf_counter()
         E = theta + ham.ecore
-        ut = MPSTensor.from_flat(psi.phys, psi.left, psi.right, u)
         l2r = forward and not change
-        A, B, info = split_twosite(ut, ham.phys, ham.phys, self.mps[site].left, self.mps[site + 1].right,
+        dev_split = getattr(self.engine, "split_twosite", None) if self.split == "engine" else None
+        if dev_split is not None:
+            A, B, info = dev_split(u, ham.phys, ham.phys, self.mps[site].left, self.mps[site + 1].right,
                                    self.m, self.cutoff, l2r=l2r, su2=ham.su2)
+        else:
+            ut = MPSTensor.from_flat(psi.phys, psi.left, psi.right, u)
+            A, B, info = split_twosite(ut, ham.phys, ham.phys, self.mps[site].left, self.mps[site + 1].right,
+                                       self.m, self.cutoff, l2r=l2r, su2=ham.su2)
         tsplit = time.perf_counter()
         self.mps[site], self.mps[site + 1] = A, B
         if l2r:
@@ -152,12 +162,14 @@ class TwoSiteDMRG:
                 right_normalize_and_return(self.mps[site])
         t2 = time.perf_counter()
         # t_jd: site problem and local solve; t_rest: truncation and boundary step, of
-        # which t_site (two-site tensor and MPO), t_split (the SVD truncation on the
-        # host) and t_step (boundary step and normalisation) are parts
+        # which t_site (two-site tensor and MPO), t_split (the SVD truncation, on the
+        # device with the GPU engine) and t_step (boundary step and normalisation) are parts
         rec = dict(sweep=sweep, micro=i, site=site, energy=E, jd_iters=its, bond=info["bond_dimension"],
                    truncated_weight=info["truncated_weight"], t_jd=t1 - t0, t_rest=t2 - t1,
                    t_site=ts - t0, t_split=tsplit - t1, t_step=t2 - tsplit, dim=len(u))
         rec.update(getattr(self.engine, "last_timing", None) or {})
+        if dev_split is not None:
+            rec.update(getattr(self.engine, "last_split", None) or {})
         self.records.append(rec)
         self.energies.append(E)
         if self.log:

@@ -146,3 +146,49 @@ def run_diag(hp, Ldata, Rdata):
     gemm(P["g2_groups"], P["g2_tiles"], T, rb, out)
     assert not np.isnan(out).any(), "diagonal element not written"
     return out
+
+
+def jacobi_svd(M, max_sweeps=60):
+    """numpy executor of the device split's block SVD (k_jacobi_svd and
+    k_split_assemble): the working matrix X is M (rows >= cols) or M^T, its
+    columns are rotated pairwise in the kernel's round-robin order until a sweep
+    rotates nothing, and the triplets come out sorted by descending singular
+    value with ties broken by column.  Returns (U, s, Vt, sweeps) with
+    M = U diag(s) Vt."""
+    rows, cols = M.shape
+    trans = cols > rows
+    X = (M.T if trans else M).astype(np.float64).copy()
+    R, C = X.shape
+    V = np.eye(C)
+    Ce = C + (C & 1)
+    tol = 2.220446049250313e-16 * np.sqrt(R)
+    sweeps, done = 0, C < 2
+    while not done and sweeps < max_sweeps:
+        rotated = False
+        for t in range(Ce - 1):
+            for k in range(Ce // 2):
+                p, q = (t, Ce - 1) if k == 0 else ((t + k) % (Ce - 1), (t - k + Ce - 1) % (Ce - 1))
+                p, q = min(p, q), max(p, q)
+                if q >= C:
+                    continue
+                a, b, g = X[:, p] @ X[:, p], X[:, q] @ X[:, q], X[:, p] @ X[:, q]
+                if abs(g) > tol * np.sqrt(a * b):
+                    zeta = (b - a) / (2.0 * g)
+                    tt = np.copysign(1.0, zeta) / (abs(zeta) + np.sqrt(1.0 + zeta * zeta))
+                    if tt != 0.0:
+                        cs = 1.0 / np.sqrt(1.0 + tt * tt)
+                        sn = cs * tt
+                        X[:, [p, q]] = X[:, [p, q]] @ np.array([[cs, sn], [-sn, cs]])
+                        V[:, [p, q]] = V[:, [p, q]] @ np.array([[cs, sn], [-sn, cs]])
+                        rotated = True
+        sweeps += 1
+        done = not rotated
+    sig = np.sqrt(np.sum(X * X, axis=0))
+    order = sorted(range(C), key=lambda j: (-sig[j], j))
+    s = sig[order]
+    inv = np.where(s > 0, 1.0 / np.where(s > 0, s, 1.0), 0.0)
+    if trans:
+        U, Vt = V[:, order], (X[:, order] * inv).T
+    else:
+        U, Vt = X[:, order] * inv, V[:, order].T
+    return U, s, Vt, (sweeps if done else -1)
