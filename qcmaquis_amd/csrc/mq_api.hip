@@ -1065,23 +1065,27 @@ struct JD {
         check(hipStreamSynchronize(s), "davidson sync");
         ptr_used = coef_used = 0;
     }
-    // dots of pairs (x_k, y_k), summed on the host in a fixed order (one synchronisation)
+    // dots of pairs (x_k, y_k), summed on the host in a fixed order (one
+    // synchronisation per kJdPartialPairs pairs)
     std::vector<double> dots(const std::vector<const double *> &xs, const std::vector<const double *> &ys) {
-        int np = (int)xs.size();
-        if (np > kJdPartialPairs) throw std::runtime_error("too many dot products in one batch");
-        std::vector<const double *> ptrs(xs);
-        ptrs.insert(ptrs.end(), ys.begin(), ys.end());
-        int slot;
-        const double *const *d = stage_ptrs(ptrs, &slot);
-        check(launch_multidot(d, d + np, np, n, nblk, p->jd_partial.p, s), "multidot");
-        check(hipMemcpyAsync(p->h_partial, p->jd_partial.p, sizeof(double) * np * nblk, hipMemcpyDeviceToHost, s),
-              "dot partials");
-        sync();
-        std::vector<double> r(np, 0.0);
-        for (int k = 0; k < np; ++k) {
-            long double acc = 0;
-            for (int b = 0; b < nblk; ++b) acc += p->h_partial[(size_t)k * nblk + b];
-            r[k] = (double)acc;
+        const int np_all = (int)xs.size();
+        std::vector<double> r(np_all, 0.0);
+        for (int k0 = 0; k0 < np_all; k0 += kJdPartialPairs) {
+            const int np = std::min(kJdPartialPairs, np_all - k0);
+            std::vector<const double *> ptrs(xs.begin() + k0, xs.begin() + k0 + np);
+            ptrs.insert(ptrs.end(), ys.begin() + k0, ys.begin() + k0 + np);
+            int slot;
+            const double *const *d = stage_ptrs(ptrs, &slot);
+            check(launch_multidot(d, d + np, np, n, nblk, p->jd_partial.p, s), "multidot");
+            check(hipMemcpyAsync(p->h_partial, p->jd_partial.p, sizeof(double) * np * nblk, hipMemcpyDeviceToHost,
+                                 s),
+                  "dot partials");
+            sync();
+            for (int k = 0; k < np; ++k) {
+                long double acc = 0;
+                for (int b = 0; b < nblk; ++b) acc += p->h_partial[(size_t)k * nblk + b];
+                r[k0 + k] = (double)acc;
+            }
         }
         return r;
     }

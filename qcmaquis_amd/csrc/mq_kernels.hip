@@ -755,14 +755,9 @@ __device__ inline double wave_sum(double v) {
 // rotates nothing.  Then sigma is sorted in descending order (ties by column)
 // into S and the column permutation into perm; sweeps[block] is the number of
 // sweeps, or -1 when max_sweeps did not converge.
-__global__ void __launch_bounds__(1024) k_jacobi_svd(const SvdBlk *__restrict__ blks, const int32_t *__restrict__ order,
-                                                     double *__restrict__ X, double *__restrict__ V,
-                                                     double *__restrict__ S, int32_t *__restrict__ perm,
-                                                     int32_t *__restrict__ sweeps, int max_sweeps) {
-    extern __shared__ double sig[];
-    __shared__ int rotated;
-    const int bi = order[blockIdx.x];
-    const SvdBlk b = blks[bi];
+__device__ __attribute__((noinline)) void jacobi_svd_plain(const SvdBlk &b, int bi, double *__restrict__ X, double *__restrict__ V,
+                                 double *__restrict__ S, int32_t *__restrict__ perm, int32_t *__restrict__ sweeps,
+                                 int max_sweeps, double *sig, int &rotated) {
     const int R = b.R, C = b.C;
     double *x = X + b.x_off, *v = V + b.v_off;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
@@ -849,6 +844,151 @@ __global__ void __launch_bounds__(1024) k_jacobi_svd(const SvdBlk *__restrict__ 
     if (tid == 0) sweeps[bi] = done ? sweep : -1;
 }
 
+// Register-batched form of jacobi_svd_plain for blocks with R <= 64 * RCH:
+// the same rotations in the same order.  A wave loads the X columns of NB
+// pairs into registers at once, reduces their 3 * NB dot products together,
+// and rotates and stores them (and the V columns, read after the rotation is
+// known), so each round trip to memory serves NB pairs instead of one.
+__device__ inline void jacobi_pair(int t, int k, int Ce, int &p, int &q) {
+    if (k == 0) {
+        p = t;
+        q = Ce - 1;
+    } else {
+        p = (t + k) % (Ce - 1);
+        q = (t - k + Ce - 1) % (Ce - 1);
+    }
+    if (p > q) {
+        const int z = p;
+        p = q;
+        q = z;
+    }
+}
+
+template <int RCH, int NB>
+__device__ __attribute__((noinline)) void jacobi_svd_rb(const SvdBlk &b, int bi, double *__restrict__ X, double *__restrict__ V,
+                              double *__restrict__ S, int32_t *__restrict__ perm, int32_t *__restrict__ sweeps,
+                              int max_sweeps, double *sig, int &rotated) {
+    const int R = b.R, C = b.C;
+    double *x = X + b.x_off, *v = V + b.v_off;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int e = tid; e < C * C; e += blockDim.x) v[e] = (e % (C + 1) == 0) ? 1.0 : 0.0;
+    const int Ce = C + (C & 1), np = Ce / 2;
+    const double tol = 2.220446049250313e-16 * sqrt((double)R);
+    int sweep = 0;
+    bool done = C < 2;
+    __syncthreads();
+    while (!done && sweep < max_sweeps) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int t = 0; t < Ce - 1; ++t) {
+            for (int k0 = wave; k0 < np; k0 += nw * NB) {
+                int P[NB], Q[NB];
+                double cp[NB][RCH], cq[NB][RCH];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    const int k = k0 + j * nw;
+                    int p = -1, q = -1;
+                    if (k < np) jacobi_pair(t, k, Ce, p, q);
+                    if (q >= C) p = q = -1;
+                    P[j] = p;
+                    Q[j] = q;
+                    const double *xp = x + (int64_t)(p < 0 ? 0 : p) * R, *xq = x + (int64_t)(q < 0 ? 0 : q) * R;
+#pragma unroll
+                    for (int c = 0; c < RCH; ++c) {
+                        const int i = lane + 64 * c;
+                        const bool in = p >= 0 && i < R;
+                        cp[j][c] = in ? xp[i] : 0.0;
+                        cq[j][c] = in ? xq[i] : 0.0;
+                    }
+                }
+                double a[NB], bb[NB], g[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    a[j] = bb[j] = g[j] = 0.0;
+#pragma unroll
+                    for (int c = 0; c < RCH; ++c) {
+                        a[j] += cp[j][c] * cp[j][c];
+                        bb[j] += cq[j][c] * cq[j][c];
+                        g[j] += cp[j][c] * cq[j][c];
+                    }
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        a[j] += __shfl_xor(a[j], o, 64);
+                        bb[j] += __shfl_xor(bb[j], o, 64);
+                        g[j] += __shfl_xor(g[j], o, 64);
+                    }
+                double cs[NB], sn[NB];
+                bool rot[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    rot[j] = false;
+                    cs[j] = 1.0;
+                    sn[j] = 0.0;
+                    if (P[j] >= 0 && fabs(g[j]) > tol * sqrt(a[j] * bb[j])) {
+                        const double zeta = (bb[j] - a[j]) / (2.0 * g[j]);
+                        const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        if (tt != 0.0) {
+                            rot[j] = true;
+                            cs[j] = 1.0 / sqrt(1.0 + tt * tt);
+                            sn[j] = cs[j] * tt;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    if (!rot[j]) continue;
+                    double *xp = x + (int64_t)P[j] * R, *xq = x + (int64_t)Q[j] * R;
+#pragma unroll
+                    for (int c = 0; c < RCH; ++c) {
+                        const int i = lane + 64 * c;
+                        if (i < R) {
+                            xp[i] = cs[j] * cp[j][c] - sn[j] * cq[j][c];
+                            xq[i] = sn[j] * cp[j][c] + cs[j] * cq[j][c];
+                        }
+                    }
+                    if (lane == 0) rotated = 1;
+                }
+                for (int i = lane; i < C; i += 64) {
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        if (!rot[j]) continue;
+                        double *vp = v + (int64_t)P[j] * C, *vq = v + (int64_t)Q[j] * C;
+                        const double u = vp[i], w = vq[i];
+                        vp[i] = cs[j] * u - sn[j] * w;
+                        vq[i] = sn[j] * u + cs[j] * w;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        ++sweep;
+        done = rotated == 0;
+        __syncthreads();
+    }
+    for (int j = wave; j < C; j += nw) {
+        const double *xj = x + (int64_t)j * R;
+        double a = 0.0;
+        for (int i = lane; i < R; i += 64) a += xj[i] * xj[i];
+        a = wave_sum(a);
+        if (lane == 0) sig[j] = sqrt(a);
+    }
+    __syncthreads();
+    for (int j = tid; j < C; j += blockDim.x) {
+        const double sj = sig[j];
+        int r = 0;
+        for (int k = 0; k < C; ++k) {
+            const double sk = sig[k];
+            r += (sk > sj) || (sk == sj && k < j);
+        }
+        S[b.s_off + r] = sj;
+        perm[b.s_off + r] = j;
+    }
+    if (tid == 0) sweeps[bi] = done ? sweep : -1;
+}
+
 // The two site tensors from the decomposition: for kept triplet k of block b
 // (column j = perm[k] of the working matrices) the left vector u (X_j / sigma
 // or V_j) goes to column k of A and the right vector w (V_j or X_j / sigma) to
@@ -889,10 +1029,31 @@ hipError_t launch_split_gather(const SplitSlab *slabs, int n, const double *psi,
     return hipGetLastError();
 }
 
+// One workgroup per block, each running the form for its row count (one launch
+// for all blocks, largest first, so that small blocks fill the CUs the large
+// ones leave free).
+__global__ void __launch_bounds__(512) k_jacobi_svd(const SvdBlk *__restrict__ blks, const int32_t *__restrict__ order,
+                                                    double *__restrict__ X, double *__restrict__ V,
+                                                    double *__restrict__ S, int32_t *__restrict__ perm,
+                                                    int32_t *__restrict__ sweeps, int max_sweeps) {
+    extern __shared__ double sig[];
+    __shared__ int rotated;
+    const int bi = order[blockIdx.x];
+    const SvdBlk b = blks[bi];
+    const int R = b.R;
+    if (R <= 128) jacobi_svd_rb<2, 8>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+    else if (R <= 256) jacobi_svd_rb<4, 8>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+    else if (R <= 384) jacobi_svd_rb<6, 5>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+    else if (R <= 512) jacobi_svd_rb<8, 4>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+    else if (R <= 768) jacobi_svd_rb<12, 2>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+    else if (R <= 1024) jacobi_svd_rb<16, 2>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+    else jacobi_svd_plain(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+}
+
 hipError_t launch_jacobi_svd(const SvdBlk *blks, const int32_t *order, int nblk, int max_c, double *X, double *V,
                              double *S, int32_t *perm, int32_t *sweeps, int max_sweeps, hipStream_t s) {
     if (nblk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_jacobi_svd, dim3(nblk), dim3(1024), sizeof(double) * std::max(1, max_c), s, blks, order, X,
+    hipLaunchKernelGGL(k_jacobi_svd, dim3(nblk), dim3(512), sizeof(double) * std::max(1, max_c), s, blks, order, X,
                        V, S, perm, sweeps, max_sweeps);
     return hipGetLastError();
 }

@@ -23,6 +23,42 @@ def load_sectors(path):
     return rec, bonds
 
 
+def stretched_sectors(path, L, m, phys, right_end, su2=True):
+    """Bond sectors of an L-orbital state modelled on a sector structure recorded
+    from a DMRG run on a shorter chain (tests/golden/make_sectors.py).
+
+    Bond i of the long chain takes the sectors of the recorded bond at the same
+    relative position, moved in particle number by the even shift that carries
+    the recorded mean filling there (scaled by N / N_recorded) to this chain's,
+    with each size scaled by m / m_recorded and capped at the sector's number of
+    states; sectors this chain does not allow are dropped.  The result keeps the
+    shape of a real ground state's bond (a few tens of sectors around the mean
+    filling, small spins) where the degeneracy-proportional structure of
+    bounded_sectors spreads m over every allowed sector."""
+    import numpy as np
+    from .mps import allowed_sectors
+    rec, bonds = load_sectors(path)
+    L0, N0 = rec["L"], rec["nelec"]
+    m0 = max(b.sum_of_sizes() for b in bonds)
+    nbar0 = np.array([sum(c[0] * n for c, n in b) / max(1, b.sum_of_sizes()) for b in bonds])
+    allowed = allowed_sectors(L, phys, right_end, 1 << 30, su2)
+    out = []
+    for i in range(L + 1):
+        x = i * L0 / L
+        j = int(round(x))
+        target = float(np.interp(x, np.arange(L0 + 1), nbar0)) * right_end[0] / N0
+        shift = 2 * int(round((target - nbar0[j]) / 2))
+        d = {}
+        for c, n in bonds[j]:
+            q = (c[0] + shift, c[1])
+            if allowed[i].has(q):
+                d[q] = min(allowed[i].size_of_block(q), max(1, int(round(n * m / m0))))
+        if not d:
+            raise ValueError(f"no recorded sector of bond {j} fits bond {i} of the L={L} chain")
+        out.append(Index.from_dict(d))
+    return out
+
+
 class SiteProblem:
     """Two-site problem: Hamiltonian, psi0, boundaries, two-site MPO, the
     site index and the MPS it was cut from (sites < site left-normalised,
