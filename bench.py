@@ -92,19 +92,47 @@ def plan_key(form, plan):
     return f"{form} | executed {plan.exec_flops:.0f} | workspace {plan.workspace_bytes}"
 
 
-def traffic_profile(a, sec_src, form, plan):
+def traffic_profile(a, sec_src, key):
     """Counter-measured HBM bytes per launch of the sigma kernels for this
-    workload, form and plan, from the committed profile (None when the run
-    builds a plan that has not been profiled)."""
+    workload and plan (key: plan_key of the form's plan), from the committed
+    profile (None when the run builds a plan that has not been profiled)."""
     if not (sec_src == "tests/golden/sectors_L24_m512.json" and a.L == 24 and a.symm == "su2u1"):
         return None
     path = os.path.join(ROOT, "profiles", "sigma_L24_m512_traffic.json")
     if not os.path.exists(path):
         return None
-    rec = json.load(open(path)).get(plan_key(form, plan))
+    rec = json.load(open(path)).get(key)
     if rec is None:
         return None
     return dict(rec, file=os.path.relpath(path, ROOT))
+
+
+def config_records():
+    """Per-form summaries of the other workloads (TwoU1 L=40 m=1024; SU2U1 L=50
+    m=1024, 4 roots), read from the JSON lines their own runs of this script
+    wrote under profiles/ (python bench.py --config 3 / --config 4): recorded
+    by those runs, not measured by this one."""
+    out = {}
+    for cfg in (3, 4):
+        path = os.path.join(ROOT, "profiles", f"bench_config{cfg}.json")
+        if not os.path.exists(path):
+            continue
+        rec = json.loads(open(path).read().strip().splitlines()[-1])
+        forms = [k for k in ("fp64", "mixed", "bf16x3") if isinstance(rec.get(k), dict)]
+        blk = rec.get("block_davidson") or {}
+        out[str(cfg)] = dict(
+            recorded_by=f"python bench.py --config {cfg}", file=os.path.relpath(path, ROOT),
+            workload=rec["config"]["workload"], sectors=rec["config"].get("sectors"),
+            psi_len=rec["config"].get("psi_len"), mpo_bond=rec["config"].get("mpo_bond"),
+            flops_per_sigma=rec.get("flops_per_sigma"),
+            forms={k: dict(ms_per_sigma=rec[k]["ms_per_sigma"], tflops=rec[k]["tflops"],
+                           ms_per_davidson_iter=rec[k].get("ms_per_davidson_iter"),
+                           ms_per_block_iter=(blk.get(k) or {}).get("ms_per_block_iter"),
+                           delta_E_vs_fp64_Eh=rec.get(f"delta_E_{k}_vs_fp64_Eh"),
+                           block_max_delta_E_vs_fp64=(blk.get(k) or {}).get("max_delta_E_vs_fp64"))
+                   for k in forms},
+            cpu_baseline=rec.get("cpu_baseline"), parity_max_rel_err_vs_oracle=rec.get("parity_max_rel_err_vs_oracle"))
+    return out or None
 
 
 def main():
@@ -129,7 +157,17 @@ def main():
                     help="use the default schedule instead of tuning it per form on the workload")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="first run a plain device copy of this many GiB (reference for HBM counters)")
+    ap.add_argument("--lean", action="store_true",
+                    help="free each form's plan before building the next (workloads near the HBM size)")
+    ap.add_argument("--config", type=int, choices=[1, 3, 4], default=None,
+                    help="preset workload: 1 the SU2U1 L=24 m=512 centre (default), 3 TwoU1 L=40 m=1024 decay 8, "
+                         "4 SU2U1 L=50 m=1024 on the stretched sector model, FP64 and BF16x3, 4 roots, lean")
     a = ap.parse_args()
+    if a.config == 3:
+        a.L, a.m, a.symm, a.decay = 40, 1024, "2u1", 8.0
+    elif a.config == 4:
+        a.L, a.m, a.symm, a.decay, a.sectors, a.roots = 50, 1024, "su2u1", None, "stretched", 4
+        a.forms, a.lean, a.no_autotune = "fp64,bf16x3", True, True
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sigma-vector path is built and timed on a single MI355X; "
               "run it with --gpus 1", file=sys.stderr)
@@ -142,7 +180,7 @@ def main():
 
     from qcmaquis_amd.autotune import DEFAULT as DEFAULT_SCHEDULE, table_json, tune
     from qcmaquis_amd.gpu import GpuEngine
-    from qcmaquis_amd.workload import centre_problem, load_sectors
+    from qcmaquis_amd.workload import centre_problem, load_sectors, stretched_sectors
 
     eng = GpuEngine(0)
     if a.copy_calibration > 0:
@@ -161,7 +199,15 @@ def main():
     sectors, sec_src = None, f"degeneracy-proportional sectors capped at m={a.m}"
     if a.sectors is None and a.symm == "su2u1" and a.L == 24 and a.decay is None:
         a.sectors = os.path.join(ROOT, "tests", "golden", "sectors_L24_m512.json")
-    if a.sectors and os.path.exists(a.sectors):
+    if a.sectors == "stretched":
+        # the recorded L=24 m=512 ground-state structure stretched to this chain
+        from qcmaquis_amd.operators import su2_phys_index, twou1_phys_index
+        if a.symm != "su2u1":
+            raise SystemExit("--sectors stretched models SU2U1 chains")
+        rec_path = os.path.join(ROOT, "tests", "golden", "sectors_L24_m512.json")
+        sectors = stretched_sectors(rec_path, a.L, a.m, su2_phys_index(), (a.L, 0))
+        sec_src = f"{os.path.relpath(rec_path, ROOT)} stretched to L={a.L}, m={a.m} (workload.stretched_sectors)"
+    elif a.sectors and os.path.exists(a.sectors):
         rec, sectors = load_sectors(a.sectors)
         if rec["L"] != a.L or rec.get("symm", "su2u1") != a.symm or rec.get("decay") != a.decay:
             raise SystemExit(f"{a.sectors} does not describe L={a.L} {a.symm} decay={a.decay}")
@@ -184,21 +230,14 @@ def main():
             log(f"autotune {name}: {tuned[name][0].label()} ({tuned[name][1][0][1]:.3f} ms; default "
                 f"{[r[1] for r in tuned[name][1] if r[0] == DEFAULT_SCHEDULE][0]:.3f} ms) "
                 f"in {time.perf_counter() - t0:.1f}s")
-    plans = {}
+    # one form at a time: its plan is built, measured (sigma timing, the site solve
+    # and the block solve) and, with --lean, freed before the next form's
+    plans, pinfo = {}, {}
     t_plan = 0.0
-    for name, _ in FORMS:
-        eng.set_schedule(tuned[name][0])
-        t0 = time.perf_counter()
-        plans[name] = eng.sigma_plan(ham.symm, psi, Lb, Rb, W)
-        t_plan = max(t_plan, time.perf_counter() - t0)
-        log(f"{name} plan {time.perf_counter() - t0:.2f}s: F {plans[name].ref_flops / 1e9:.3f} GF, executed "
-            f"{plans[name].exec_flops / 1e9:.3f} GF, workspace {plans[name].workspace_bytes / 1e9:.2f} GB")
-    sp = plans["fp64"]
     x = eng.to_dev(psi.to_flat())
     y = torch.empty_like(x)
 
-    def measure(name, prec):
-        sp = plans[name]
+    def measure(sp, prec):
         torch.cuda.synchronize()
         for _ in range(a.warmup):
             sp.apply(x, y, prec)
@@ -237,13 +276,6 @@ def main():
         ms_jd = (time.perf_counter() - t0) / max(its, 1) * 1e3
         return dict(dt=dt, avg=avg, work=work, theta=theta, its=its, ms_jd=ms_jd, y=ycopy, bind_ms=bind_ms)
 
-    res = {}
-    for name, prec in FORMS:
-        res[name] = measure(name, prec)
-        log(f"{name} stages ms: " + ", ".join(f"{n} {v:.3f}" for n, v in zip(STAGES, res[name]["avg"])))
-    r64 = res["fp64"]
-    LOW = [n for n, _ in FORMS if n != "fp64"]
-    sig_err = {k: float(np.abs(res[k]["y"] - r64["y"]).max() / np.abs(r64["y"]).max()) for k in LOW}
     # energy agreement: every form solves the site problem from the same start
     # vector (the MPS guess) to |r| <= 1e-8 |theta| (the reference's default
     # ietl_jcd_tol, DmrgParameters.h:53; the eigenvalue error is then about
@@ -251,61 +283,67 @@ def main():
     # iterations; each record keeps the iterations in the form, the FP64
     # continuation iterations, the form's own last Ritz value and the wall time
     JD_SOLVE, RESTARTS, TOL = 50, 8, 1e-8
-    conv, dE = None, None
-    if not a.no_delta_e:
-        conv = {}
-        for name, prec in FORMS:
-            p_ = plans[name]
-            u, rec = x, dict(form_iterations=0, fp64_iterations=0, restarts=0, converged=False)
-            eng.sync()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for r_ in range(RESTARTS):
-                th, u, its = p_.jd(u, JD_SOLVE, TOL, prec=prec)
-                info = p_.solve_info()
-                if r_ == 0:
-                    rec["theta_form_first_solve"] = info["theta_form"] + ham.ecore
-                    rec["theta_rayleigh_fp64_first_solve"] = info["theta_rayleigh_fp64"] + ham.ecore
-                rec["form_iterations"] += info["form_iterations"]
-                rec["fp64_iterations"] += info["fp64_iterations"]
-                rec["restarts"] = r_
-                done = info["fp64_iterations"] < JD_SOLVE if prec != MQ_PREC_FP64 and info["fp64_iterations"] > 0 \
-                    else info["form_iterations"] < JD_SOLVE
-                if done:
-                    rec["converged"] = True
-                    break
-            eng.sync()
-            rec["solve_s"] = time.perf_counter() - t0
-            rec["theta"] = th + ham.ecore
-            conv[name] = rec
-        dE = {k: abs(conv[k]["theta"] - conv["fp64"]["theta"]) for k in LOW}
-        for name, _ in FORMS:
+
+    def solve(p_, prec):
+        u, rec = x, dict(form_iterations=0, fp64_iterations=0, restarts=0, converged=False)
+        eng.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r_ in range(RESTARTS):
+            th, u, its = p_.jd(u, JD_SOLVE, TOL, prec=prec)
+            info = p_.solve_info()
+            if r_ == 0:
+                rec["theta_form_first_solve"] = info["theta_form"] + ham.ecore
+                rec["theta_rayleigh_fp64_first_solve"] = info["theta_rayleigh_fp64"] + ham.ecore
+            rec["form_iterations"] += info["form_iterations"]
+            rec["fp64_iterations"] += info["fp64_iterations"]
+            rec["restarts"] = r_
+            done = info["fp64_iterations"] < JD_SOLVE if prec != MQ_PREC_FP64 and info["fp64_iterations"] > 0 \
+                else info["form_iterations"] < JD_SOLVE
+            if done:
+                rec["converged"] = True
+                break
+        eng.sync()
+        rec["solve_s"] = time.perf_counter() - t0
+        rec["theta"] = th + ham.ecore
+        return rec
+
+    res, conv = {}, ({} if not a.no_delta_e else None)
+    block = {"roots": a.roots} if a.roots > 0 else None
+    X0, ref = None, None
+    for name, prec in FORMS:
+        eng.set_schedule(tuned[name][0])
+        t0 = time.perf_counter()
+        bp = eng.sigma_plan(ham.symm, psi, Lb, Rb, W)
+        t_plan = max(t_plan, time.perf_counter() - t0)
+        pinfo[name] = dict(ref_flops=bp.ref_flops, exec_flops=bp.exec_flops, workspace_bytes=bp.workspace_bytes,
+                           min_bytes=bp.min_bytes, n=bp.n, key=plan_key(name, bp))
+        log(f"{name} plan {time.perf_counter() - t0:.2f}s: F {bp.ref_flops / 1e9:.3f} GF, executed "
+            f"{bp.exec_flops / 1e9:.3f} GF, FP64 workspace {bp.workspace_bytes / 1e9:.2f} GB")
+        res[name] = measure(bp, prec)
+        log(f"{name} stages ms: " + ", ".join(f"{n} {v:.3f}" for n, v in zip(STAGES, res[name]["avg"])))
+        if conv is not None:
+            conv[name] = solve(bp, prec)
             c_ = conv[name]
             log(f"solve {name}: theta {c_['theta']:.12f}, {c_['form_iterations']} iterations in the form + "
                 f"{c_['fp64_iterations']} FP64, {c_['solve_s']:.2f} s, converged {c_['converged']}"
-                + (f", dE vs fp64 {dE[name]:.2e}, sigma rel err {sig_err[name]:.2e}" if name in dE else ""))
-    ok = ["fp64"] + [k for k in LOW if dE is not None and dE[k] <= 1e-8]
-    prec_name = min(ok, key=lambda k: res[k]["dt"])
-    ch = res[prec_name]
-    dt, avg, work = ch["dt"], ch["avg"], ch["work"]
-
-    # block Davidson over `roots` roots: ms per block iteration (one batched sigma of
-    # the block) in each form, and the converged roots against the FP64 ones
-    block = None
-    if a.roots > 0:
-        from qcmaquis_amd.gpu import independent_starts
-        k = a.roots
-        rng = np.random.default_rng(11)
-        # the MPS guess and random vectors on the coordinates H_eff couples (the
-        # structurally zero columns of H_eff have eigenvalue 0, below every physical
-        # root of this Hamiltonian)
-        live = sp.live_mask()
-        X0 = independent_starts([psi.to_flat()] + [rng.standard_normal(sp.n) for _ in range(k - 1)], live=live)
-        X0 = torch.as_tensor(X0).to(eng.device)
-        block = {"roots": k, "dead_coordinates": int((~live).sum())}
-        ref = None
-        for name, prec in FORMS:
-            bp = plans[name]
+                + (f", dE vs fp64 {abs(c_['theta'] - conv['fp64']['theta']):.2e}" if name != "fp64" else ""))
+        # block Davidson over `roots` roots: ms per block iteration (one batched sigma
+        # of the block) in each form, and the converged roots against the FP64 ones
+        if block is not None:
+            from qcmaquis_amd.gpu import independent_starts
+            k = a.roots
+            bp.release()  # the single-root solve's arenas, before the batched ones
+            if X0 is None:
+                rng = np.random.default_rng(11)
+                # the MPS guess and random vectors on the coordinates H_eff couples (the
+                # structurally zero columns of H_eff have eigenvalue 0, below every
+                # physical root of this Hamiltonian)
+                live = bp.live_mask()
+                X0 = independent_starts([psi.to_flat()] + [rng.standard_normal(bp.n) for _ in range(k - 1)],
+                                        live=live)
+                X0 = torch.as_tensor(X0).to(eng.device)
+                block["dead_coordinates"] = int((~live).sum())
             diag = bp.diagonal() if ham.symm == "su2u1" else None
             bp.block_davidson(X0, 1, 0.0, prec=prec, diag=diag)  # builds the plan's batched task lists
             eng.sync()
@@ -316,7 +354,19 @@ def main():
             torch.cuda.synchronize()
             ms_it = (time.perf_counter() - t0) / max(its, 1) * 1e3
             t0 = time.perf_counter()
-            th, _, itc = bp.block_davidson(X0, 400, 1e-9, prec=prec, diag=diag)
+            if a.lean:
+                # restarted from the current Ritz vectors every 40 iterations, with a
+                # progress line per restart (a sigma takes tenths of a second here)
+                Xc, itc = X0, 0
+                for r_ in range(10):
+                    th, Xc, k_ = bp.block_davidson(Xc, 40, 1e-9, prec=prec, diag=diag)
+                    itc += k_
+                    log(f"  block Davidson {name}: restart {r_}, {itc} iterations, lowest root {th[0]:.9f} "
+                        f"({time.perf_counter() - t0:.0f}s)")
+                    if k_ < 40:
+                        break
+            else:
+                th, _, itc = bp.block_davidson(X0, 400, 1e-9, prec=prec, diag=diag)
             eng.sync()
             t_conv = time.perf_counter() - t0
             th = [t + ham.ecore for t in th]
@@ -325,13 +375,29 @@ def main():
             info = bp.solve_info()
             block[name] = dict(ms_per_block_iter=ms_it, converged_iterations=itc, converge_s=t_conv, energies=th,
                                form_iterations=info["form_iterations"], fp64_iterations=info["fp64_iterations"],
-                               max_delta_E_vs_fp64=max(abs(x - y) for x, y in zip(th, ref)))
+                               max_delta_E_vs_fp64=max(abs(x_ - y_) for x_, y_ in zip(th, ref)))
             if conv is not None:
                 block[name]["root0_minus_single_root_theta"] = th[0] - conv["fp64"]["theta"]
             bp.release()  # the batched arenas of one form at a time
             log(f"block Davidson {k} roots {name}: {ms_it:.2f} ms/iteration; converged in {itc} iterations "
                 f"({t_conv:.2f} s), roots {', '.join(f'{e:.9f}' for e in th)}, max dE vs fp64 "
                 f"{block[name]['max_delta_E_vs_fp64']:.2e}")
+        if a.lean:
+            bp.close()
+        else:
+            plans[name] = bp
+    fp = pinfo["fp64"]
+    r64 = res["fp64"]
+    LOW = [n for n, _ in FORMS if n != "fp64"]
+    sig_err = {k: float(np.abs(res[k]["y"] - r64["y"]).max() / np.abs(r64["y"]).max()) for k in LOW}
+    dE = None if conv is None else {k: abs(conv[k]["theta"] - conv["fp64"]["theta"]) for k in LOW}
+    for k in LOW:
+        log(f"{k}: sigma rel err vs fp64 {sig_err[k]:.2e}")
+    ok = ["fp64"] + [k for k in LOW if dE is not None and dE[k] <= 1e-8]
+    prec_name = min(ok, key=lambda k: res[k]["dt"])
+    ch = res[prec_name]
+    dt, avg, work = ch["dt"], ch["avg"], ch["work"]
+    if block is not None:
         for name in LOW:
             block[name]["speedup_vs_fp64"] = block["fp64"]["ms_per_block_iter"] / block[name]["ms_per_block_iter"]
 
@@ -341,10 +407,10 @@ def main():
     # on the time, F / P_MFMA(form) or B_min / BW_HBM, and achieved = F / t with t
     # the kernel time of one sigma from HIP events on the library stream
     peak_f = {"fp64": PEAK_FP64_TFLOPS, "mixed": PEAK_FP32_TFLOPS, "bf16x3": PEAK_BF16X3_TFLOPS}[prec_name]
-    F, Bmin = sp.ref_flops, sp.min_bytes
+    F, Bmin = fp["ref_flops"], fp["min_bytes"]
     t_sig = sum(avg[1:]) * 1e-3
     kern = sigma_kernels(prec_name)
-    prof = traffic_profile(a, sec_src, prec_name, plans[prec_name])
+    prof = traffic_profile(a, sec_src, pinfo[prec_name]["key"])
     traffic = None
     if prof is not None and all(k in prof["kernels"] for k in kern.values()):
         traffic = sum(prof["kernels"][k]["traffic_bytes"] * prof["kernels"][k].get("per_sigma", 1.0)
@@ -381,8 +447,8 @@ def main():
     def summary(name):
         r = res[name]
         pk = {"fp64": PEAK_FP64_TFLOPS, "mixed": PEAK_FP32_TFLOPS, "bf16x3": PEAK_BF16X3_TFLOPS}[name]
-        out_ = dict(tflops=sp.ref_flops / r["dt"] / 1e12, ms_per_sigma=r["dt"] * 1e3,
-                    frac_of_mfma_peak=sp.ref_flops / (sum(r["avg"][1:]) * 1e-3) / 1e12 / pk,
+        out_ = dict(tflops=F / r["dt"] / 1e12, ms_per_sigma=r["dt"] * 1e3,
+                    frac_of_mfma_peak=F / (sum(r["avg"][1:]) * 1e-3) / 1e12 / pk,
                     ms_per_davidson_iter=r["ms_jd"], stages_ms=dict(zip(STAGES, r["avg"])),
                     stage_implementation_bytes=dict(zip(STAGES[2:], r["work"][3:6])),
                     theta_after_jd_iters=r["theta"] + ham.ecore, bind_ms=r["bind_ms"])
@@ -395,7 +461,7 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": sp.ref_flops / dt / 1e12,
+        "value": F / dt / 1e12,
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -411,29 +477,30 @@ def main():
                         f"m={a.m}{'' if a.decay is None else f' decay={a.decay:g}'}, centre bond "
                         f"(sites {site},{site + 1})",
             "sectors": sec_src,
-            "psi_len": int(sp.n),
+            "psi_len": int(fp["n"]),
             "mpo_bond": [int(W.row_dim), int(W.col_dim)],
             "form": prec_name,
             "parallelism": "single GPU",
         },
         "ms_per_davidson_iter": ch["ms_jd"],
         "davidson_iters": ch["its"],
-        "flops_per_sigma": sp.ref_flops,
-        "executed_flops_per_sigma": plans[prec_name].exec_flops,
+        "flops_per_sigma": F,
+        "executed_flops_per_sigma": pinfo[prec_name]["exec_flops"],
         "plan_build_s": t_plan,
         "schedule": {k: asdict(v[0]) for k, v in tuned.items()},
-        "plan_keys": {k: plan_key(k, p_) for k, p_ in plans.items()},
+        "plan_keys": {k: v["key"] for k, v in pinfo.items()},
         "autotune": None if a.no_autotune else {k: table_json(v[1][:5]) for k, v in tuned.items()},
         "fp64": summary("fp64"),
         **{k: summary(k) for k in LOW},
         **{f"{k}_speedup_vs_fp64": r64["dt"] / res[k]["dt"] for k in LOW},
         **{f"delta_E_{k}_vs_fp64_Eh": None if dE is None else dE[k] for k in LOW},
-        "min_bytes_per_sigma": sp.min_bytes,
+        "min_bytes_per_sigma": Bmin,
         **{f"{k}_sigma_max_rel_err": sig_err[k] for k in LOW},
         "block_davidson": block,
         "parity_max_rel_err_vs_oracle": err,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "other_configs": config_records() if a.config in (None, 1) else None,
     }
     print(json.dumps(out), flush=True)
     for p_ in plans.values():

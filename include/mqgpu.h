@@ -324,9 +324,12 @@ int mq_block_davidson(mq_plan *plan, const double *L, const double *R,
 int mq_plan_solve_info(const mq_plan *plan, double info[4]);
 
 /* Free the work arrays a plan keeps between calls (batched task lists and
- * arenas, Davidson vectors, the captured sigma graph); they are rebuilt by the
- * next call that needs them.  No reference counterpart: for callers that hold
- * many plans at once and need the device memory back. */
+ * arenas, a sigma plan's FP64 intermediates, Davidson vectors, the captured
+ * sigma graph); they are rebuilt by the next call that needs them.  No
+ * reference counterpart: for callers that hold many plans at once and need the
+ * device memory back.  A sigma plan allocates its FP64 intermediates with its
+ * first FP64 sigma, and a batched sigma whose per-vector arenas would not fit
+ * in free device memory runs its vectors one after another (same results). */
 int mq_plan_release(mq_plan *plan);
 
 /* -- two-site split on the device ---------------------------------------- */

@@ -3,6 +3,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -156,12 +157,11 @@ struct mq_plan {
         DevGemm g1, g2;
     };
     std::map<int, std::unique_ptr<BatchLists>> blists;
-    struct BatchArenas {
-        int cap = 0;
+    struct BatchArenas {  // vectors the FP64 (cap) and FP32 (cap32) arenas hold
+        int cap = 0, cap32 = 0, tcap = 0;
         int64_t ws = 0;
         DevArray<double> prp, t, c, ws64, tmp;  // tmp: rbtm plans' right-paired sigma
         DevArray<float> prp32, t32, c32, ws32;
-        int cap32 = 0;
     } barena;
     // captured sigma stages on fixed staging vectors (mq_sigma_graph)
     hipGraphExec_t gexec = nullptr;
@@ -293,8 +293,13 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         p->la.alloc(h.n_la);
         p->rb.alloc(h.n_rb);
         p->prp.alloc(h.n_prp);
-        p->t.alloc(h.n_t);
-        p->c.alloc(h.n_c);
+        // sigma plans allocate the FP64 T and C arenas with their first FP64 sigma
+        // (the mixed forms use FP32 ones), so a plan run only in a mixed form never
+        // holds them
+        if (kind != PLAN_SIGMA) {
+            p->t.alloc(h.n_t);
+            p->c.alloc(h.n_c);
+        }
         p->tmp.alloc(h.n_tmp);
         p->ws.alloc(std::max(h.g1.ws, h.g2.ws));
         for (auto &st : h.wapply.strips) {
@@ -367,16 +372,57 @@ static void bind(mq_plan *p, const double *L, const double *R, hipStream_t s) {
     p->conv32 = false;
 }
 
+// Frees the batched arenas (rebuilt by the next batched sigma that needs them).
+static void free_batch_arenas(mq_plan *p) {
+    check(hipStreamSynchronize(p->ctx->stream), "release sync");
+    auto &A = p->barena;
+    A.prp.free_();
+    A.t.free_();
+    A.c.free_();
+    A.ws64.free_();
+    A.prp32.free_();
+    A.t32.free_();
+    A.c32.free_();
+    A.ws32.free_();
+    A.tmp.free_();
+    A.cap = A.cap32 = A.tcap = 0;
+    A.ws = 0;
+}
+
+// Allocates a plan's single-vector arenas with f; when device memory is short,
+// frees the batched arenas (a block solve switching form holds the other form's)
+// and tries once more.
+static void alloc_or_free_batches(mq_plan *p, const std::function<void()> &f) {
+    try {
+        f();
+    } catch (std::runtime_error &) {
+        (void)hipGetLastError();  // the failed allocation's error must not reach the next launch check
+        free_batch_arenas(p);
+        f();
+    }
+}
+
+// FP64 T and C arenas of a sigma plan (allocated with its first FP64 sigma)
+static void ensure64(mq_plan *p) {
+    HostPlan &h = p->hp;
+    alloc_or_free_batches(p, [&] {
+        if (!p->t.p && h.n_t) p->t.alloc(h.n_t);
+        if (!p->c.p && h.n_c) p->c.alloc(h.n_c);
+    });
+}
+
 // FP32 panels of the current binding (mixed form)
 static void bind32(mq_plan *p, hipStream_t s) {
     HostPlan &h = p->hp;
     if (!p->t32.p && h.n_t) {
-        p->la32.alloc(h.n_la);
-        p->rb32.alloc(h.n_rb);
-        p->prp32.alloc(h.n_prp);
-        p->t32.alloc(h.n_t);
-        p->c32.alloc(h.n_c);
-        p->ws32.alloc(std::max(h.g1.ws, h.g2.ws));
+        alloc_or_free_batches(p, [&] {
+            p->la32.alloc(h.n_la);
+            p->rb32.alloc(h.n_rb);
+            p->prp32.alloc(h.n_prp);
+            p->t32.alloc(h.n_t);
+            p->c32.alloc(h.n_c);
+            p->ws32.alloc(std::max(h.g1.ws, h.g2.ws));
+        });
     }
     if (p->conv32) return;
     check(launch_to_f32(p->la.p, p->la32.p, h.n_la, s), "LA to FP32");
@@ -389,6 +435,8 @@ static void sigma_stages(mq_plan *p, const double *x, double *y, int prec, hipEv
     hipStream_t s = p->ctx->stream;
     HostPlan &h = p->hp;
     const bool x3 = prec == MQ_PREC_BF16X3;
+    if (prec == MQ_PREC_FP64 && ((!p->t.p && h.n_t) || (!p->c.p && h.n_c)))
+        throw std::runtime_error("sigma: FP64 arenas not allocated");
     auto mark = [&](int i) {
         if (ev) check(hipEventRecord(ev[i], s), "record");
     };
@@ -460,6 +508,7 @@ static void sigma_launch(mq_plan *p, const double *L, const double *R, const dou
     hipStream_t s = p->ctx->stream;
     bind(p, L, R, s);
     if (prec != MQ_PREC_FP64) bind32(p, s);
+    else ensure64(p);
     sigma_stages(p, x, y, prec, nullptr);
 }
 
@@ -487,26 +536,42 @@ static mq_plan::BatchLists &batch_setup(mq_plan *p, int nvec, bool f32) {
         slot = std::move(B);
     }
     auto &A = p->barena;
-    if (A.cap < nvec || A.ws < slot->ws) {
-        check(hipStreamSynchronize(s), "batch arenas");
-        const int cap = std::max(A.cap, nvec);
-        A.prp.alloc((size_t)h.n_prp * cap);
-        A.t.alloc((size_t)h.n_t * cap);
-        A.c.alloc((size_t)h.n_c * cap);
-        A.tmp.alloc((size_t)h.n_tmp * cap);
-        A.ws = std::max(A.ws, slot->ws);
-        A.ws64.alloc((size_t)A.ws);
-        A.cap = cap;
-        A.cap32 = 0;
+    if (A.ws < slot->ws) {
+        A.ws = slot->ws;
+        A.cap = A.cap32 = 0;  // the split-K workspaces are reallocated with the arenas
     }
-    if (f32 && A.cap32 < A.cap) {
-        A.prp32.alloc((size_t)h.n_prp * A.cap);
-        A.t32.alloc((size_t)h.n_t * A.cap);
-        A.c32.alloc((size_t)h.n_c * A.cap);
+    if (A.tcap < nvec) {  // FP64 output staging of rbtm plans, used by every form
+        check(hipStreamSynchronize(s), "batch arenas");
+        A.tmp.alloc((size_t)h.n_tmp * nvec);
+        A.tcap = nvec;
+    }
+    if (!f32 && A.cap < nvec) {
+        check(hipStreamSynchronize(s), "batch arenas");
+        A.prp.alloc((size_t)h.n_prp * nvec);
+        A.t.alloc((size_t)h.n_t * nvec);
+        A.c.alloc((size_t)h.n_c * nvec);
+        A.ws64.alloc((size_t)A.ws);
+        A.cap = nvec;
+    }
+    if (f32 && A.cap32 < nvec) {
+        check(hipStreamSynchronize(s), "batch arenas");
+        A.prp32.alloc((size_t)h.n_prp * nvec);
+        A.t32.alloc((size_t)h.n_t * nvec);
+        A.c32.alloc((size_t)h.n_c * nvec);
         A.ws32.alloc((size_t)A.ws);
-        A.cap32 = A.cap;
+        A.cap32 = nvec;
     }
     return *slot;
+}
+
+// Device bytes the batch arenas of nvec vectors in a form would add to what the
+// plan holds now.
+static double batch_arena_bytes(mq_plan *p, int nvec, bool f32) {
+    HostPlan &h = p->hp;
+    auto &A = p->barena;
+    const int have = f32 ? A.cap32 : A.cap;
+    if (have >= nvec) return 0.0;
+    return (double)nvec * (h.n_prp + h.n_t + h.n_c) * (f32 ? 4.0 : 8.0) + (double)nvec * h.n_tmp * 8.0;
 }
 
 // Y(:, j) = H X(:, j) for nvec vectors stored one after another: one launch per
@@ -516,6 +581,20 @@ static void sigma_batch_launch(mq_plan *p, const double *L, const double *R, con
     hipStream_t s = p->ctx->stream;
     bind(p, L, R, s);
     if (prec != MQ_PREC_FP64) bind32(p, s);
+    else ensure64(p);
+    // when the batch arenas would not fit in free device memory (with 10 % to
+    // spare), the vectors run one after another through the plan's own arenas:
+    // the same task lists on the same data, so the results are identical
+    const double need = batch_arena_bytes(p, nvec, prec != MQ_PREC_FP64);
+    if (need > 0) {
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "memory info");
+        if (need > 0.9 * (double)free_b) {
+            const int64_t n = p->hp.vec_len;
+            for (int j = 0; j < nvec; ++j) sigma_stages(p, X + j * n, Y + j * n, prec, nullptr);
+            return;
+        }
+    }
     mq_plan::BatchLists &Bl = batch_setup(p, nvec, prec != MQ_PREC_FP64);
     auto &A = p->barena;
     struct {
@@ -587,6 +666,7 @@ static void sigma_graph_launch(mq_plan *p, const double *L, const double *R, con
     hipStream_t s = p->ctx->stream;
     bind(p, L, R, s);
     if (prec != MQ_PREC_FP64) bind32(p, s);
+    else ensure64(p);  // before the capture: no allocation inside it
     const int64_t n = p->hp.vec_len;
     if (!p->gx.p && n) {
         p->gx.alloc(n);
@@ -709,6 +789,7 @@ int mq_sigma_timed(mq_plan *plan, const double *L, const double *R, const double
         check(hipEventRecord(ev[0], s), "record");
         bind(plan, L, R, s);
         if (prec != MQ_PREC_FP64) bind32(plan, s);
+        else ensure64(plan);
         check(hipEventRecord(ev[1], s), "record");
         sigma_stages(plan, x, y, prec, ev);
         check(hipEventSynchronize(ev[5]), "sync");
@@ -920,17 +1001,11 @@ int mq_plan_release(mq_plan *plan) {
         check(hipSetDevice(plan->ctx->device), "hipSetDevice");
         check(hipStreamSynchronize(plan->ctx->stream), "release sync");
         plan->blists.clear();
-        auto &A = plan->barena;
-        A.prp.free_();
-        A.t.free_();
-        A.c.free_();
-        A.ws64.free_();
-        A.prp32.free_();
-        A.t32.free_();
-        A.c32.free_();
-        A.ws32.free_();
-        A.cap = A.cap32 = 0;
-        A.ws = 0;
+        free_batch_arenas(plan);
+        if (plan->hp.kind == PLAN_SIGMA) {  // re-allocated by the next FP64 sigma
+            plan->t.free_();
+            plan->c.free_();
+        }
         plan->jd_vecs.free_();
         plan->jd_cap = 0;
         if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);

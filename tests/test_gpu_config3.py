@@ -7,7 +7,7 @@ capped at m=1024, random MPS values (seed 42), the centre two-site problem
 Bars: sigma <= 1e-12 relative to the oracle with identical algorithmic FLOPs;
 the FP32 and BF16x3 sigma within 1e-5 and 2e-5; the lowest root of the
 4-root block Davidson (FP32 form, live-masked starts) equals the single-root
-Jacobi-Davidson eigenvalue to 1e-8 Eh."""
+Jacobi-Davidson eigenvalue (restarted FP64 solves to 1e-9) to 1e-8 Eh."""
 import numpy as np
 import pytest
 
@@ -15,13 +15,17 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def centre_l40(gpu_engine):
+def centre_l40(gpu_engine, request):
     import time
     from qcmaquis_amd.workload import centre_problem
     t0 = time.perf_counter()
+    capman = request.config.pluginmanager.getplugin("capturemanager")
 
     def log(msg):
-        print(f"[{time.perf_counter() - t0:6.1f}s] {msg}", flush=True)
+        # the build takes minutes (the 3.8M-term MPO, 38 boundary steps): its stages
+        # go to the terminal with pytest's output capture suspended
+        with capman.global_and_fixture_disabled():
+            print(f"[{time.perf_counter() - t0:6.1f}s] {msg}", flush=True)
 
     log("building the TwoU1 L=40 m=1024 decay=8 centre problem")
     p = centre_problem(gpu_engine, 40, m=1024, symm="2u1", decay=8, log=log)
@@ -66,12 +70,21 @@ def test_config3_block_roots(centre_l40, gpu_engine):
     sp = gpu_engine.sigma_plan(p.ham.symm, p.psi, p.L, p.R, p.W)
     try:
         x0 = p.psi.to_flat()
-        theta, _, its = sp.jd(gpu_engine.to_dev(x0), 200, 1e-10, prec=MQ_PREC_FP64)
+        # restarted single-root solve from the MPS guess: 100 iterations per restart
+        # until a restart converges (the random-MPS guess is far from the ground state)
+        u, its = gpu_engine.to_dev(x0), 0
+        for _ in range(12):
+            theta, u, k = sp.jd(u, 100, 1e-9, prec=MQ_PREC_FP64)
+            its += k
+            if k < 100:
+                break
+        assert k < 100, "single-root solve did not converge"
         X0 = independent_starts([x0] * 4, seed=3, live=sp.live_mask())
-        th, _, bits = sp.block_davidson(gpu_engine.torch.as_tensor(X0).to(gpu_engine.device), 400, 1e-8,
+        th, _, bits = sp.block_davidson(gpu_engine.torch.as_tensor(X0).to(gpu_engine.device), 600, 1e-9,
                                         prec=MQ_PREC_FP32)
     finally:
         sp.close()
     print(f"single root {theta:.10f} ({its} its), block roots {np.round(th, 8)} ({bits} its)", flush=True)
     assert np.all(np.diff(th) >= -1e-10)
+    assert bits < 600
     assert abs(th[0] - theta) <= 1e-8
