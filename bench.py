@@ -160,14 +160,19 @@ def main():
     ap.add_argument("--lean", action="store_true",
                     help="free each form's plan before building the next (workloads near the HBM size)")
     ap.add_argument("--config", type=int, choices=[1, 3, 4], default=None,
-                    help="preset workload: 1 the SU2U1 L=24 m=512 centre (default), 3 TwoU1 L=40 m=1024 decay 8, "
-                         "4 SU2U1 L=50 m=1024 on the stretched sector model, FP64 and BF16x3, 4 roots, lean")
+                    help="preset workload: 1 the SU2U1 L=24 m=512 centre (default), 3 TwoU1 L=40 m=1024 decay 8 "
+                         "(lean, default schedule), 4 SU2U1 L=50 m=1024 on the stretched sector model, FP64 and "
+                         "BF16x3, 4 roots (lean, default schedule)")
+    # a preset sets the defaults of its workload; flags given explicitly still win
+    pre = argparse.ArgumentParser(add_help=False)
+    pre.add_argument("--config", type=int, default=None)
+    cfg = pre.parse_known_args()[0].config
+    if cfg == 3:
+        ap.set_defaults(L=40, m=1024, symm="2u1", decay=8.0, lean=True, no_autotune=True)
+    elif cfg == 4:
+        ap.set_defaults(L=50, m=1024, symm="su2u1", decay=None, sectors="stretched", roots=4, forms="fp64,bf16x3",
+                        lean=True, no_autotune=True)
     a = ap.parse_args()
-    if a.config == 3:
-        a.L, a.m, a.symm, a.decay = 40, 1024, "2u1", 8.0
-    elif a.config == 4:
-        a.L, a.m, a.symm, a.decay, a.sectors, a.roots = 50, 1024, "su2u1", None, "stretched", 4
-        a.forms, a.lean, a.no_autotune = "fp64,bf16x3", True, True
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sigma-vector path is built and timed on a single MI355X; "
               "run it with --gpus 1", file=sys.stderr)
@@ -355,25 +360,30 @@ def main():
             ms_it = (time.perf_counter() - t0) / max(its, 1) * 1e3
             t0 = time.perf_counter()
             if a.lean:
-                # restarted from the current Ritz vectors every 40 iterations, with a
+                # restarted from the current Ritz vectors every 100 iterations, with a
                 # progress line per restart (a sigma takes tenths of a second here)
-                Xc, itc = X0, 0
-                for r_ in range(10):
-                    th, Xc, k_ = bp.block_davidson(Xc, 40, 1e-9, prec=prec, diag=diag)
+                Xc, itc, conv_ = X0, 0, False
+                for r_ in range(8):
+                    th, Xc, k_ = bp.block_davidson(Xc, 100, 1e-9, prec=prec, diag=diag)
                     itc += k_
                     log(f"  block Davidson {name}: restart {r_}, {itc} iterations, lowest root {th[0]:.9f} "
                         f"({time.perf_counter() - t0:.0f}s)")
-                    if k_ < 40:
+                    if k_ < 100:
+                        conv_ = True
                         break
+                if not conv_:
+                    log(f"  block Davidson {name}: not converged in {itc} iterations")
             else:
                 th, _, itc = bp.block_davidson(X0, 400, 1e-9, prec=prec, diag=diag)
+                conv_ = itc < 400
             eng.sync()
             t_conv = time.perf_counter() - t0
             th = [t + ham.ecore for t in th]
             if ref is None:
                 ref = th
             info = bp.solve_info()
-            block[name] = dict(ms_per_block_iter=ms_it, converged_iterations=itc, converge_s=t_conv, energies=th,
+            block[name] = dict(ms_per_block_iter=ms_it, converged=conv_, converged_iterations=itc, converge_s=t_conv,
+                               energies=th,
                                form_iterations=info["form_iterations"], fp64_iterations=info["fp64_iterations"],
                                max_delta_E_vs_fp64=max(abs(x_ - y_) for x_, y_ in zip(th, ref)))
             if conv is not None:

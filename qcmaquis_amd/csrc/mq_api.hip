@@ -1889,7 +1889,7 @@ extern "C" int mq_block_davidson(mq_plan *plan, const double *L, const double *R
 // Sweeps of the Jacobi SVD before a block is reported as not converged (the
 // sweeps converge quadratically; blocks of a few hundred columns take 6-12).
 static constexpr int kSvdMaxSweeps = 60;
-// singular values of one block kept in LDS by the Jacobi kernel
+// singular values of one block kept in LDS while they are sorted
 static constexpr int kSvdMaxCols = 8192;
 
 struct mq_split {
@@ -1898,6 +1898,8 @@ struct mq_split {
     DevArray<SplitSlab> slabs;
     DevArray<SvdBlk> blk;
     DevArray<int32_t> order, perm, sweeps, keep;
+    DevArray<int32_t> slot_blk, slot_k, done, rot;  // Jacobi rounds: one wave per column pair slot
+    int nslots = 0, rounds = 0;                     // rounds per sweep: the largest block's Ce - 1
     DevArray<int64_t> aoff, boff;
     DevArray<double> X, V, S;
     std::vector<int32_t> last_sweeps;  // Jacobi sweeps per block of the last decomposition
@@ -1922,6 +1924,22 @@ int mq_split_create(mq_ctx *ctx, const mq_split_desc *desc, mq_split **out) {
         sp->slabs.upload(m.slabs, s);
         sp->blk.upload(m.blk, s);
         sp->order.upload(m.order, s);
+        // pair slots of every block, the largest blocks first
+        std::vector<int32_t> sb, sk;
+        for (int32_t b : m.order) {
+            const int C = m.blk[b].C, Ce = C + (C & 1);
+            if (C < 2) continue;
+            for (int k = 0; k < Ce / 2; ++k) {
+                sb.push_back(b);
+                sk.push_back(k);
+            }
+            sp->rounds = std::max(sp->rounds, Ce - 1);
+        }
+        sp->nslots = (int)sb.size();
+        sp->slot_blk.upload(sb, s);
+        sp->slot_k.upload(sk, s);
+        sp->done.alloc(m.blk.size());
+        sp->rot.alloc(m.blk.size());
         sp->X.alloc((size_t)m.x_total);
         sp->V.alloc((size_t)m.v_total);
         sp->S.alloc((size_t)m.s_total);
@@ -1960,9 +1978,23 @@ int mq_split_decompose(mq_split *sp, const double *psi, double *s_out) {
     try {
         if (m.x_total) check(hipMemsetAsync(sp->X.p, 0, sizeof(double) * m.x_total, s), "split clear");
         check(launch_split_gather(sp->slabs.p, (int)m.slabs.size(), psi, sp->X.p, s), "split gather");
-        check(launch_jacobi_svd(sp->blk.p, sp->order.p, (int)m.blk.size(), m.max_c, sp->X.p, sp->V.p, sp->S.p,
-                                sp->perm.p, sp->sweeps.p, kSvdMaxSweeps, s),
-              "jacobi svd");
+        const int nb = (int)m.blk.size();
+        check(launch_jacobi_init(sp->blk.p, nb, sp->V.p, sp->done.p, sp->rot.p, sp->sweeps.p, s), "jacobi init");
+        std::vector<int32_t> dn(nb);
+        for (int sw = 0; sw < kSvdMaxSweeps && sp->nslots > 0; ++sw) {
+            for (int t = 0; t < sp->rounds; ++t)
+                check(launch_jacobi_round(sp->blk.p, sp->slot_blk.p, sp->slot_k.p, sp->nslots, t, sp->X.p, sp->V.p,
+                                          sp->done.p, sp->rot.p, s),
+                      "jacobi round");
+            check(launch_jacobi_sweep_end(nb, sw, sp->done.p, sp->rot.p, sp->sweeps.p, s), "jacobi sweep end");
+            check(hipMemcpyAsync(dn.data(), sp->done.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s),
+                  "jacobi done");
+            check(hipStreamSynchronize(s), "jacobi sweep");
+            if (std::all_of(dn.begin(), dn.end(), [](int32_t d) { return d != 0; })) break;
+        }
+        check(launch_jacobi_finish(sp->blk.p, nb, m.max_c, sp->X.p, sp->S.p, sp->perm.p, sp->done.p, sp->sweeps.p,
+                                   s),
+              "jacobi finish");
         std::vector<int32_t> sw(m.blk.size());
         if (!sw.empty())
             check(hipMemcpyAsync(sw.data(), sp->sweeps.p, sizeof(int32_t) * sw.size(), hipMemcpyDeviceToHost, s),

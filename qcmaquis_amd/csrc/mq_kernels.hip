@@ -743,112 +743,17 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// One-sided (Hestenes) Jacobi SVD of one working matrix X (R x C, C <= R,
-// columns contiguous) per workgroup: sweeps of plane rotations that make the
-// columns of X pairwise orthogonal while V (C x C, from the identity) collects
-// them, so that X V_final = U diag(sigma) with sigma_j the final column norms.
-// Each sweep visits every column pair once in round-robin order: round t pairs
-// column Ce-1 (a padding column when C is odd) with column t, and column
-// (t + k) mod (Ce-1) with (t - k) mod (Ce-1) for k = 1 .. Ce/2-1; the pairs of
-// a round are disjoint, one per wave.  A pair is rotated when
-// |x_p . x_q| > eps sqrt(R) |x_p| |x_q|; the sweeps stop when a whole sweep
-// rotates nothing.  Then sigma is sorted in descending order (ties by column)
-// into S and the column permutation into perm; sweeps[block] is the number of
-// sweeps, or -1 when max_sweeps did not converge.
-__device__ __attribute__((noinline)) void jacobi_svd_plain(const SvdBlk &b, int bi, double *__restrict__ X, double *__restrict__ V,
-                                 double *__restrict__ S, int32_t *__restrict__ perm, int32_t *__restrict__ sweeps,
-                                 int max_sweeps, double *sig, int &rotated) {
-    const int R = b.R, C = b.C;
-    double *x = X + b.x_off, *v = V + b.v_off;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    for (int e = tid; e < C * C; e += blockDim.x) v[e] = (e % (C + 1) == 0) ? 1.0 : 0.0;
-    const int Ce = C + (C & 1);
-    const double tol = 2.220446049250313e-16 * sqrt((double)R);
-    int sweep = 0;
-    bool done = C < 2;
-    __syncthreads();
-    while (!done && sweep < max_sweeps) {
-        if (tid == 0) rotated = 0;
-        __syncthreads();
-        for (int t = 0; t < Ce - 1; ++t) {
-            for (int k = wave; k < Ce / 2; k += nw) {
-                int p, q;
-                if (k == 0) {
-                    p = t;
-                    q = Ce - 1;
-                } else {
-                    p = (t + k) % (Ce - 1);
-                    q = (t - k + Ce - 1) % (Ce - 1);
-                }
-                if (p > q) {
-                    const int z = p;
-                    p = q;
-                    q = z;
-                }
-                if (q >= C) continue;
-                double *xp = x + (int64_t)p * R, *xq = x + (int64_t)q * R;
-                double a = 0.0, bb = 0.0, g = 0.0;
-                for (int i = lane; i < R; i += 64) {
-                    const double u = xp[i], w = xq[i];
-                    a += u * u;
-                    bb += w * w;
-                    g += u * w;
-                }
-                a = wave_sum(a);
-                bb = wave_sum(bb);
-                g = wave_sum(g);
-                if (fabs(g) > tol * sqrt(a * bb)) {
-                    const double zeta = (bb - a) / (2.0 * g);
-                    const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    if (tt != 0.0) {
-                        const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
-                        for (int i = lane; i < R; i += 64) {
-                            const double u = xp[i], w = xq[i];
-                            xp[i] = cs * u - sn * w;
-                            xq[i] = sn * u + cs * w;
-                        }
-                        double *vp = v + (int64_t)p * C, *vq = v + (int64_t)q * C;
-                        for (int i = lane; i < C; i += 64) {
-                            const double u = vp[i], w = vq[i];
-                            vp[i] = cs * u - sn * w;
-                            vq[i] = sn * u + cs * w;
-                        }
-                        if (lane == 0) rotated = 1;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        ++sweep;
-        done = rotated == 0;
-        __syncthreads();
-    }
-    for (int j = wave; j < C; j += nw) {
-        const double *xj = x + (int64_t)j * R;
-        double a = 0.0;
-        for (int i = lane; i < R; i += 64) a += xj[i] * xj[i];
-        a = wave_sum(a);
-        if (lane == 0) sig[j] = sqrt(a);
-    }
-    __syncthreads();
-    for (int j = tid; j < C; j += blockDim.x) {
-        const double sj = sig[j];
-        int r = 0;
-        for (int k = 0; k < C; ++k) {
-            const double sk = sig[k];
-            r += (sk > sj) || (sk == sj && k < j);
-        }
-        S[b.s_off + r] = sj;
-        perm[b.s_off + r] = j;
-    }
-    if (tid == 0) sweeps[bi] = done ? sweep : -1;
-}
-
-// Register-batched form of jacobi_svd_plain for blocks with R <= 64 * RCH:
-// the same rotations in the same order.  A wave loads the X columns of NB
-// pairs into registers at once, reduces their 3 * NB dot products together,
-// and rotates and stores them (and the V columns, read after the rotation is
-// known), so each round trip to memory serves NB pairs instead of one.
+// One-sided (Hestenes) Jacobi SVD of the working matrices X (R x C, C <= R,
+// columns contiguous), all blocks together: sweeps of plane rotations that make
+// the columns of each X pairwise orthogonal while its V (C x C, from the
+// identity) collects them, so that X V_final = U diag(sigma) with sigma_j the
+// final column norms.  A sweep visits every column pair of a block once, in
+// round-robin order: round t pairs column Ce-1 (a padding column when C is odd)
+// with column t, and column (t + k) mod (Ce-1) with (t - k) mod (Ce-1) for
+// k = 1 .. Ce/2-1.  The pairs of a round are disjoint, so one launch runs round
+// t of every block with one wave per pair, across the whole GPU; a pair is
+// rotated when |x_p . x_q| > eps sqrt(R) |x_p| |x_q|, and a block is done after
+// a sweep that rotates none of its pairs.
 __device__ inline void jacobi_pair(int t, int k, int Ce, int &p, int &q) {
     if (k == 0) {
         p = t;
@@ -864,112 +769,93 @@ __device__ inline void jacobi_pair(int t, int k, int Ce, int &p, int &q) {
     }
 }
 
-template <int RCH, int NB>
-__device__ __attribute__((noinline)) void jacobi_svd_rb(const SvdBlk &b, int bi, double *__restrict__ X, double *__restrict__ V,
-                              double *__restrict__ S, int32_t *__restrict__ perm, int32_t *__restrict__ sweeps,
-                              int max_sweeps, double *sig, int &rotated) {
-    const int R = b.R, C = b.C;
-    double *x = X + b.x_off, *v = V + b.v_off;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    for (int e = tid; e < C * C; e += blockDim.x) v[e] = (e % (C + 1) == 0) ? 1.0 : 0.0;
-    const int Ce = C + (C & 1), np = Ce / 2;
-    const double tol = 2.220446049250313e-16 * sqrt((double)R);
-    int sweep = 0;
-    bool done = C < 2;
-    __syncthreads();
-    while (!done && sweep < max_sweeps) {
-        if (tid == 0) rotated = 0;
-        __syncthreads();
-        for (int t = 0; t < Ce - 1; ++t) {
-            for (int k0 = wave; k0 < np; k0 += nw * NB) {
-                int P[NB], Q[NB];
-                double cp[NB][RCH], cq[NB][RCH];
-#pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    const int k = k0 + j * nw;
-                    int p = -1, q = -1;
-                    if (k < np) jacobi_pair(t, k, Ce, p, q);
-                    if (q >= C) p = q = -1;
-                    P[j] = p;
-                    Q[j] = q;
-                    const double *xp = x + (int64_t)(p < 0 ? 0 : p) * R, *xq = x + (int64_t)(q < 0 ? 0 : q) * R;
-#pragma unroll
-                    for (int c = 0; c < RCH; ++c) {
-                        const int i = lane + 64 * c;
-                        const bool in = p >= 0 && i < R;
-                        cp[j][c] = in ? xp[i] : 0.0;
-                        cq[j][c] = in ? xq[i] : 0.0;
-                    }
-                }
-                double a[NB], bb[NB], g[NB];
-#pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    a[j] = bb[j] = g[j] = 0.0;
-#pragma unroll
-                    for (int c = 0; c < RCH; ++c) {
-                        a[j] += cp[j][c] * cp[j][c];
-                        bb[j] += cq[j][c] * cq[j][c];
-                        g[j] += cp[j][c] * cq[j][c];
-                    }
-                }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-                    for (int j = 0; j < NB; ++j) {
-                        a[j] += __shfl_xor(a[j], o, 64);
-                        bb[j] += __shfl_xor(bb[j], o, 64);
-                        g[j] += __shfl_xor(g[j], o, 64);
-                    }
-                double cs[NB], sn[NB];
-                bool rot[NB];
-#pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    rot[j] = false;
-                    cs[j] = 1.0;
-                    sn[j] = 0.0;
-                    if (P[j] >= 0 && fabs(g[j]) > tol * sqrt(a[j] * bb[j])) {
-                        const double zeta = (bb[j] - a[j]) / (2.0 * g[j]);
-                        const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                        if (tt != 0.0) {
-                            rot[j] = true;
-                            cs[j] = 1.0 / sqrt(1.0 + tt * tt);
-                            sn[j] = cs[j] * tt;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    if (!rot[j]) continue;
-                    double *xp = x + (int64_t)P[j] * R, *xq = x + (int64_t)Q[j] * R;
-#pragma unroll
-                    for (int c = 0; c < RCH; ++c) {
-                        const int i = lane + 64 * c;
-                        if (i < R) {
-                            xp[i] = cs[j] * cp[j][c] - sn[j] * cq[j][c];
-                            xq[i] = sn[j] * cp[j][c] + cs[j] * cq[j][c];
-                        }
-                    }
-                    if (lane == 0) rotated = 1;
-                }
-                for (int i = lane; i < C; i += 64) {
-#pragma unroll
-                    for (int j = 0; j < NB; ++j) {
-                        if (!rot[j]) continue;
-                        double *vp = v + (int64_t)P[j] * C, *vq = v + (int64_t)Q[j] * C;
-                        const double u = vp[i], w = vq[i];
-                        vp[i] = cs[j] * u - sn[j] * w;
-                        vq[i] = sn[j] * u + cs[j] * w;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        ++sweep;
-        done = rotated == 0;
-        __syncthreads();
+// V = I for every block; done / rotated flags cleared (blocks of one column are done)
+__global__ void __launch_bounds__(256) k_jacobi_init(const SvdBlk *__restrict__ blks, double *__restrict__ V,
+                                                     int32_t *__restrict__ done, int32_t *__restrict__ rot,
+                                                     int32_t *__restrict__ sweeps) {
+    const SvdBlk b = blks[blockIdx.x];
+    double *v = V + b.v_off;
+    const int C = b.C;
+    for (int e = threadIdx.x; e < C * C; e += blockDim.x) v[e] = (e % (C + 1) == 0) ? 1.0 : 0.0;
+    if (threadIdx.x == 0) {
+        done[blockIdx.x] = C < 2;
+        rot[blockIdx.x] = 0;
+        sweeps[blockIdx.x] = 0;
     }
+}
+
+// round t of a sweep: wave w of the launch takes pair slot w (block slot_blk[w],
+// pair index slot_k[w])
+__global__ void __launch_bounds__(256) k_jacobi_round(const SvdBlk *__restrict__ blks,
+                                                      const int32_t *__restrict__ slot_blk,
+                                                      const int32_t *__restrict__ slot_k, int nslots, int t,
+                                                      double *__restrict__ X, double *__restrict__ V,
+                                                      const int32_t *__restrict__ done, int32_t *__restrict__ rot) {
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= nslots) return;
+    const int bi = slot_blk[w];
+    if (done[bi]) return;
+    const SvdBlk b = blks[bi];
+    const int R = b.R, C = b.C, Ce = C + (C & 1);
+    if (t >= Ce - 1) return;
+    int p, q;
+    jacobi_pair(t, slot_k[w], Ce, p, q);
+    if (q >= C) return;
+    double *xp = X + b.x_off + (int64_t)p * R, *xq = X + b.x_off + (int64_t)q * R;
+    double a = 0.0, bb = 0.0, g = 0.0;
+    for (int i = lane; i < R; i += 64) {
+        const double u = xp[i], v = xq[i];
+        a += u * u;
+        bb += v * v;
+        g += u * v;
+    }
+    a = wave_sum(a);
+    bb = wave_sum(bb);
+    g = wave_sum(g);
+    const double tol = 2.220446049250313e-16 * sqrt((double)R);
+    if (!(fabs(g) > tol * sqrt(a * bb))) return;
+    const double zeta = (bb - a) / (2.0 * g);
+    const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+    if (tt == 0.0) return;
+    const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
+    for (int i = lane; i < R; i += 64) {
+        const double u = xp[i], v = xq[i];
+        xp[i] = cs * u - sn * v;
+        xq[i] = sn * u + cs * v;
+    }
+    double *vp = V + b.v_off + (int64_t)p * C, *vq = V + b.v_off + (int64_t)q * C;
+    for (int i = lane; i < C; i += 64) {
+        const double u = vp[i], v = vq[i];
+        vp[i] = cs * u - sn * v;
+        vq[i] = sn * u + cs * v;
+    }
+    if (lane == 0) rot[bi] = 1;
+}
+
+// end of sweep s: blocks that rotated nothing are done after s + 1 sweeps
+__global__ void __launch_bounds__(256) k_jacobi_sweep_end(int nblk, int s, int32_t *__restrict__ done,
+                                                          int32_t *__restrict__ rot, int32_t *__restrict__ sweeps) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    if (!done[b] && !rot[b]) {
+        done[b] = 1;
+        sweeps[b] = s + 1;
+    }
+    rot[b] = 0;
+}
+
+// singular values (final column norms) sorted descending, ties by column, with
+// their column permutation; a block that did not converge reports sweeps = -1
+__global__ void __launch_bounds__(256) k_jacobi_finish(const SvdBlk *__restrict__ blks, const double *__restrict__ X,
+                                                       double *__restrict__ S, int32_t *__restrict__ perm,
+                                                       const int32_t *__restrict__ done,
+                                                       int32_t *__restrict__ sweeps) {
+    extern __shared__ double sig[];
+    const SvdBlk b = blks[blockIdx.x];
+    const int R = b.R, C = b.C;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
     for (int j = wave; j < C; j += nw) {
-        const double *xj = x + (int64_t)j * R;
+        const double *xj = X + b.x_off + (int64_t)j * R;
         double a = 0.0;
         for (int i = lane; i < R; i += 64) a += xj[i] * xj[i];
         a = wave_sum(a);
@@ -986,7 +872,7 @@ __device__ __attribute__((noinline)) void jacobi_svd_rb(const SvdBlk &b, int bi,
         S[b.s_off + r] = sj;
         perm[b.s_off + r] = j;
     }
-    if (tid == 0) sweeps[bi] = done ? sweep : -1;
+    if (tid == 0 && !done[blockIdx.x]) sweeps[blockIdx.x] = -1;
 }
 
 // The two site tensors from the decomposition: for kept triplet k of block b
@@ -1029,32 +915,33 @@ hipError_t launch_split_gather(const SplitSlab *slabs, int n, const double *psi,
     return hipGetLastError();
 }
 
-// One workgroup per block, each running the form for its row count (one launch
-// for all blocks, largest first, so that small blocks fill the CUs the large
-// ones leave free).
-__global__ void __launch_bounds__(512) k_jacobi_svd(const SvdBlk *__restrict__ blks, const int32_t *__restrict__ order,
-                                                    double *__restrict__ X, double *__restrict__ V,
-                                                    double *__restrict__ S, int32_t *__restrict__ perm,
-                                                    int32_t *__restrict__ sweeps, int max_sweeps) {
-    extern __shared__ double sig[];
-    __shared__ int rotated;
-    const int bi = order[blockIdx.x];
-    const SvdBlk b = blks[bi];
-    const int R = b.R;
-    if (R <= 128) jacobi_svd_rb<2, 8>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
-    else if (R <= 256) jacobi_svd_rb<4, 8>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
-    else if (R <= 384) jacobi_svd_rb<6, 5>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
-    else if (R <= 512) jacobi_svd_rb<8, 4>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
-    else if (R <= 768) jacobi_svd_rb<12, 2>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
-    else if (R <= 1024) jacobi_svd_rb<16, 2>(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
-    else jacobi_svd_plain(b, bi, X, V, S, perm, sweeps, max_sweeps, sig, rotated);
+hipError_t launch_jacobi_init(const SvdBlk *blks, int nblk, double *V, int32_t *done, int32_t *rot, int32_t *sweeps,
+                              hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_init, dim3(nblk), dim3(256), 0, s, blks, V, done, rot, sweeps);
+    return hipGetLastError();
 }
 
-hipError_t launch_jacobi_svd(const SvdBlk *blks, const int32_t *order, int nblk, int max_c, double *X, double *V,
-                             double *S, int32_t *perm, int32_t *sweeps, int max_sweeps, hipStream_t s) {
+hipError_t launch_jacobi_round(const SvdBlk *blks, const int32_t *slot_blk, const int32_t *slot_k, int nslots, int t,
+                               double *X, double *V, const int32_t *done, int32_t *rot, hipStream_t s) {
+    if (nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_round, dim3((nslots + 3) / 4), dim3(256), 0, s, blks, slot_blk, slot_k, nslots, t, X,
+                       V, done, rot);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_sweep_end(int nblk, int sweep, int32_t *done, int32_t *rot, int32_t *sweeps,
+                                   hipStream_t s) {
     if (nblk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_jacobi_svd, dim3(nblk), dim3(512), sizeof(double) * std::max(1, max_c), s, blks, order, X,
-                       V, S, perm, sweeps, max_sweeps);
+    hipLaunchKernelGGL(k_jacobi_sweep_end, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, sweep, done, rot, sweeps);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_finish(const SvdBlk *blks, int nblk, int max_c, const double *X, double *S, int32_t *perm,
+                                const int32_t *done, int32_t *sweeps, hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_finish, dim3(nblk), dim3(256), sizeof(double) * std::max(1, max_c), s, blks, X, S,
+                       perm, done, sweeps);
     return hipGetLastError();
 }
 

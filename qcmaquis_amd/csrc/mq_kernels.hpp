@@ -48,11 +48,19 @@ hipError_t launch_diag_scale(const double *x, const double *u, double f, const d
 hipError_t launch_lincomb(const double *const *vs, const double *coef, int nv, int64_t n, double beta, double *out,
                           hipStream_t s);
 // two-site split: gather psi into the SVD working matrices (X must be zeroed
-// first), one-sided Jacobi SVD per block (order: launch order of the blocks,
-// max_c: the largest C), and the assembly of the two site tensors
+// first) and the assembly of the two site tensors
 hipError_t launch_split_gather(const SplitSlab *slabs, int n, const double *psi, double *X, hipStream_t s);
-hipError_t launch_jacobi_svd(const SvdBlk *blks, const int32_t *order, int nblk, int max_c, double *X, double *V,
-                             double *S, int32_t *perm, int32_t *sweeps, int max_sweeps, hipStream_t s);
+// Jacobi SVD of every block: init (V = I, flags), one launch per round of a sweep
+// (one wave per column pair of every block), the end-of-sweep flag update, and
+// the sorted singular values with their permutation
+hipError_t launch_jacobi_init(const SvdBlk *blks, int nblk, double *V, int32_t *done, int32_t *rot, int32_t *sweeps,
+                              hipStream_t s);
+hipError_t launch_jacobi_round(const SvdBlk *blks, const int32_t *slot_blk, const int32_t *slot_k, int nslots, int t,
+                               double *X, double *V, const int32_t *done, int32_t *rot, hipStream_t s);
+hipError_t launch_jacobi_sweep_end(int nblk, int sweep, int32_t *done, int32_t *rot, int32_t *sweeps,
+                                   hipStream_t s);
+hipError_t launch_jacobi_finish(const SvdBlk *blks, int nblk, int max_c, const double *X, double *S, int32_t *perm,
+                                const int32_t *done, int32_t *sweeps, hipStream_t s);
 hipError_t launch_split_assemble(const SvdBlk *blks, int nblk, int max_keep, const int32_t *keep,
                                  const int64_t *a_off, const int64_t *b_off, int l2r, const double *X,
                                  const double *V, const double *S, const int32_t *perm, double *A, double *B,
