@@ -630,8 +630,13 @@ struct MBP {
     bool su2;
     mutable std::vector<BlockMatrix> data;
     mutable std::vector<char> pop;
+    // single-use columns: FLOPs of computing each, and whether a task has used it yet
+    // (the reference computes such a column on its first use only)
+    mutable std::vector<double> lazy_flops;
+    mutable std::vector<char> used;
     MBP(const BlockMatrix &bm_, const Boundary &R_, const MPO &mpo_, const Index &ref_, bool su2_)
-        : R(R_), mpo(mpo_), bm(bm_), ref(ref_), su2(su2_), data(R_.size()), pop(R_.size(), 0) {}
+        : R(R_), mpo(mpo_), bm(bm_), ref(ref_), su2(su2_), data(R_.size()), pop(R_.size(), 0),
+          lazy_flops(R_.size(), 0.0), used(R_.size(), 0) {}
     void one(int b2, BlockMatrix &out) const {
         if (mpo.right_skip(b2)) {
             BlockMatrix trv = transpose(R[mpo.right_herm[b2]]);
@@ -655,6 +660,21 @@ struct MBP {
             pop[b2] = 1;
         }
         return data[b2];
+    }
+    // Computes a single-use column ahead of its use without counting its FLOPs;
+    // mark_used then counts them once, at the first task that reads the column,
+    // as the reference's lazy at() spends them.
+    void prefetch(int b2) const {
+        Counter c, *parent = g_counter;
+        g_counter = parent ? &c : nullptr;
+        at(b2);
+        g_counter = parent;
+        lazy_flops[b2] = c.flops;
+    }
+    void mark_used(int b2) const {
+        if (mpo.col_nnz[b2] != 1 || used[b2]) return;
+        used[b2] = 1;
+        if (g_counter) g_counter->flops += lazy_flops[b2];
     }
 };
 
@@ -738,7 +758,10 @@ static void rbtm_axpy(TaskMap &tasks, BlockMatrix &ret, const Index &out_right_i
         size_t o = ret.insert(kv.first.first, kv.first.second, ot[0].l_size,
                               out_right_i.size_of_block(kv.first.second));
         Matrix &buf = ret.b[o].m;
-        for (auto &mt : ot) task_axpy(mt, buf.v.data(), t.at(mt.b2).b[mt.k].m.v.data() + mt.in_offset);
+        for (auto &mt : ot) {
+            t.mark_used(mt.b2);
+            task_axpy(mt, buf.v.data(), t.at(mt.b2).b[mt.k].m.v.data() + mt.in_offset);
+        }
     }
 }
 
@@ -755,7 +778,10 @@ static void rbtm_axpy_gemm(int b1, TaskMap &tasks, BlockMatrix &prod, const Inde
         size_t k = left_b1.find(kv.first.second, kv.first.first);
         if (k == left_b1.n()) continue;
         Matrix buf(ot[0].l_size, out_right_i.size_of_block(kv.first.second));
-        for (auto &mt : ot) task_axpy(mt, buf.v.data(), t.at(mt.b2).b[mt.k].m.v.data() + mt.in_offset);
+        for (auto &mt : ot) {
+            t.mark_used(mt.b2);
+            task_axpy(mt, buf.v.data(), t.at(mt.b2).b[mt.k].m.v.data() + mt.in_offset);
+        }
         Charge rc = kv.first.second;
         const Matrix &A = left_b1.b[k].m;
         size_t c = prod.find_or_insert(rc, rc, A.r, buf.c);
@@ -781,9 +807,10 @@ static BlockMatrix site_hamil_rbtm(const Problem &p, const BlockMatrix &ket_lp, 
     ProductBasis out_right_pb(p.phys, p.right, true);
     BlockMatrix collector;
     // Single-use columns are materialised before the parallel loop so that the
-    // lazily filled cache is never written concurrently.
+    // lazily filled cache is never written concurrently; their FLOPs count at their
+    // first use (MBP::mark_used), where the reference computes them.
     for (int b2 = 0; b2 < mpo.col_dim; ++b2)
-        if (mpo.col_nnz[b2] == 1) t.at(b2);
+        if (mpo.col_nnz[b2] == 1) t.prefetch(b2);
     omp_loop_counted(mpo.row_dim, [&](int b1) {
         TaskMap tasks(cmp_pair);
         BlockMatrix tmp2;
@@ -848,7 +875,7 @@ Boundary overlap_mpo_right_step(const Problem &p, const BlockMatrix &ket_lp, con
     MBP t(ket_lp, R, mpo, indexForTrim, su2);
     t.populate();
     for (int b2 = 0; b2 < mpo.col_dim; ++b2)
-        if (mpo.col_nnz[b2] == 1) t.at(b2);
+        if (mpo.col_nnz[b2] == 1) t.prefetch(b2);
     Index out_right_i = product(adjoin(p.phys), p.right);
     Index bra_left_i = p.left;
     common_subset(out_right_i, bra_left_i);

@@ -211,3 +211,30 @@ def test_fullsize_block_roots_are_physical(centre, gpu_engine):
         assert abs(roots[0] - th) <= 1e-9, name
         assert min(roots) >= th - 1e-9, name
     assert max(abs(a - b) for a, b in zip(res["fp64"], res["bf16x3"])) <= 1e-8
+
+
+@pytest.mark.parametrize("site", [3, 7, 15, 19])
+def test_fullsize_sigma_off_centre_vs_oracle(site, centre, gpu_engine, oracle_engine):
+    """The L=24 m=512 sigma at sites away from the centre (both halves of the
+    chain, where the left and right boundaries have very different MPO bond
+    dimensions and block counts), FP64 and BF16x3, against the oracle; the
+    centre problem's Hamiltonian and MPS are reused."""
+    from qcmaquis_amd.desc import MQ_PREC_BF16X3
+    from qcmaquis_amd.workload import centre_problem, load_sectors
+    _, sec = load_sectors(os.path.join(HERE, "golden", "sectors_L24_m512.json"))
+    p = centre_problem(gpu_engine, 24, sectors=sec, site=site, ham=centre.ham)
+    x = np.random.default_rng(site).standard_normal(p.psi.to_flat().size)
+    sp = gpu_engine.sigma_plan(p.ham.symm, p.psi, p.L, p.R, p.W)
+    try:
+        xd = gpu_engine.to_dev(x)
+        y1 = sp.apply(xd).cpu().numpy()
+        f1 = sp.ref_flops
+        y3 = sp.apply(xd, prec=MQ_PREC_BF16X3).cpu().numpy()
+    finally:
+        sp.close()
+    y0, f0 = oracle_engine.sigma(p.ham.symm, p.psi, _host(p.L), _host(p.R), p.W, x=x)
+    print(f"site {site}: psi {x.size}, F {f0 / 1e9:.2f} GF, fp64 rel err {_rel(y1, y0):.2e}, "
+          f"bf16x3 rel err {_rel(y3, y0):.2e}", flush=True)
+    assert _rel(y1, y0) <= 1e-12
+    assert f1 == f0
+    assert _rel(y3, y0) <= 2e-5
