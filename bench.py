@@ -86,10 +86,14 @@ def sigma_kernels(form):
             "split_reduce": f"k_split_reduce<1, {t}, double>"}
 
 
-def plan_key(form, plan):
+def plan_key(form, plan, sched=None):
     """Identifies a plan's task lists for a form: schedules that build the same
-    lists (e.g. fused-column ratios that fuse the same columns) share it."""
-    return f"{form} | executed {plan.exec_flops:.0f} | workspace {plan.workspace_bytes}"
+    lists (e.g. fused-column ratios that fuse the same columns) share it; a
+    fixed GEMM tile shape is part of the key."""
+    key = f"{form} | executed {plan.exec_flops:.0f} | workspace {plan.workspace_bytes}"
+    if sched is not None and (sched.tile_m or sched.tile_n):
+        key += f" | tiles {sched.tile_m}x{sched.tile_n}"
+    return key
 
 
 def traffic_profile(a, sec_src, key):
@@ -322,7 +326,7 @@ def main():
         bp = eng.sigma_plan(ham.symm, psi, Lb, Rb, W)
         t_plan = max(t_plan, time.perf_counter() - t0)
         pinfo[name] = dict(ref_flops=bp.ref_flops, exec_flops=bp.exec_flops, workspace_bytes=bp.workspace_bytes,
-                           min_bytes=bp.min_bytes, n=bp.n, key=plan_key(name, bp))
+                           min_bytes=bp.min_bytes, n=bp.n, key=plan_key(name, bp, tuned[name][0]))
         log(f"{name} plan {time.perf_counter() - t0:.2f}s: F {bp.ref_flops / 1e9:.3f} GF, executed "
             f"{bp.exec_flops / 1e9:.3f} GF, FP64 workspace {bp.workspace_bytes / 1e9:.2f} GB")
         res[name] = measure(bp, prec)

@@ -165,8 +165,12 @@ const char *mq_last_error(const mq_ctx *ctx);
  * (non-abelian/site_hamil.hpp:35): the rbtm form of site_hamil_rbtm
  * (site_hamil.hpp:149-204) where (rows - one_rows) >= (cols - one_cols) for
  * SU2U1, lbtm elsewhere. */
+/* MQ_OPT_TILE_N: column tile of every GEMM group, a multiple of 16 from 16 to
+ * 128, or 0 (default: per group, the width that wastes the fewest padded
+ * columns).  Tile shapes change the launch, not the results. */
 enum mq_option { MQ_OPT_FUSE = 0, MQ_OPT_FUSE_RATIO = 1, MQ_OPT_FUSE_MIN = 2, MQ_OPT_KCHUNK = 3,
-                 MQ_OPT_GRAPH = 4, MQ_OPT_TILE_M = 5, MQ_OPT_REF_FLOPS = 6, MQ_OPT_SCHEDULE = 7 };
+                 MQ_OPT_GRAPH = 4, MQ_OPT_TILE_M = 5, MQ_OPT_REF_FLOPS = 6, MQ_OPT_SCHEDULE = 7,
+                 MQ_OPT_TILE_N = 8 };
 enum mq_schedule { MQ_SCHED_LBTM = 0, MQ_SCHED_REFERENCE = 1 };
 int mq_ctx_set_option(mq_ctx *ctx, int option, int value);
 /* HIP stream (hipStream_t) the context launches on. */

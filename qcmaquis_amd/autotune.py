@@ -1,13 +1,15 @@
 """Schedule autotuner for the sigma-vector plans.
 
-A plan's schedule has three choices that the task lists depend on but the
+A plan's schedule has four choices that the task lists depend on but the
 results do not (they agree to round-off):
   - the fused-column rule: which MPO columns are summed on the left boundary
     first (mode 0 none, 1 by an entries/positions ratio and a minimum entry
     count, 2 all), trading W-application traffic for GEMM flops;
   - the split-K chunk of the output GEMM groups;
   - the GEMM row tile (0: 128 rows unless a group fits 64; or 64 for every group,
-    twice the tiles with half the accumulators each).
+    twice the tiles with half the accumulators each);
+  - the GEMM column tile (0: per group the width that wastes the fewest padded
+    columns; or one width, 32, 64 or 128, for every group).
 The tuner builds the plan of one site problem under each candidate schedule,
 times a few sigma vectors in the requested precision with HIP events on the
 library stream, and keeps the fastest.  Results are cached per site shape
@@ -29,9 +31,10 @@ class Schedule:
     kchunk: int = 2048
     tile_m: int = 0
     sched: int = 0  # MQ_OPT_SCHEDULE: 0 lbtm everywhere, 1 the reference's lbtm/rbtm choice
+    tile_n: int = 0  # MQ_OPT_TILE_N: 0 the padding-minimising width per group, else 16..128
 
     def label(self):
-        tm = f" tile_m={self.tile_m}" if self.tile_m else ""
+        tm = (f" tile_m={self.tile_m}" if self.tile_m else "") + (f" tile_n={self.tile_n}" if self.tile_n else "")
         rule = " sched=reference" if self.sched else ""
         if self.fuse != 1:
             return f"fuse={self.fuse} kchunk={self.kchunk}{tm}{rule}"
@@ -43,8 +46,8 @@ DEFAULT = Schedule()
 
 def candidates():
     """Fused-column rules around the default (ratio x minimum entries), the two
-    extremes, split-K chunks of 1024 / 2048 / 4096 and the 64-row tile for the
-    default rule."""
+    extremes, split-K chunks of 1024 / 2048 / 4096, the 64-row tile and fixed
+    column tiles of 32 / 64 / 128 (and 64 x 64) for the default rule."""
     out = [DEFAULT, Schedule(fuse=0), Schedule(fuse=2)]
     for ratio in (2, 3, 6, 8):
         for mn in (1, 64):
@@ -54,6 +57,9 @@ def candidates():
     for kc in (1024, 4096):
         out.append(Schedule(kchunk=kc))
     out.append(Schedule(tile_m=64))
+    for tn in (32, 64, 128):
+        out.append(Schedule(tile_n=tn))
+    out.append(Schedule(tile_m=64, tile_n=64))
     return out
 
 

@@ -47,6 +47,7 @@ struct mq_ctx {
     FuseRule fuse;  // MQ_OPT_FUSE, MQ_OPT_FUSE_RATIO, MQ_OPT_FUSE_MIN
     int kchunk = 2048;  // MQ_OPT_KCHUNK
     int tile_m = 0;     // MQ_OPT_TILE_M (0: automatic)
+    int tile_n = 0;     // MQ_OPT_TILE_N (0: automatic)
     int ref_flops = 1;  // MQ_OPT_REF_FLOPS
     int graph = 0;      // MQ_OPT_GRAPH
     int schedule = SCHED_LBTM;  // MQ_OPT_SCHEDULE
@@ -238,6 +239,10 @@ int mq_ctx_set_option(mq_ctx *ctx, int option, int value) {
         ctx->tile_m = value;
         return MQ_OK;
     }
+    if (option == MQ_OPT_TILE_N && (value == 0 || (value >= 16 && value <= 128 && value % 16 == 0))) {
+        ctx->tile_n = value;
+        return MQ_OK;
+    }
     if (option == MQ_OPT_GRAPH && (value == 0 || value == 1)) {
         ctx->graph = value;
         return MQ_OK;
@@ -268,7 +273,7 @@ static int make_plan(mq_ctx *ctx, const mq_site_desc *desc, mq_plan **out, int k
         check(hipSetDevice(ctx->device), "hipSetDevice");
         std::unique_ptr<mq_plan> p(new mq_plan());
         p->ctx = ctx;
-        const int tm = ctx->tile_m, tn = 0, kc = ctx->kchunk;
+        const int tm = ctx->tile_m, tn = ctx->tile_n, kc = ctx->kchunk;
         if (kind == PLAN_SIGMA)
             p->hp = build_sigma_plan(*desc, tm, tn, kc, ctx->fuse, ctx->ref_flops != 0, ctx->schedule);
         else if (kind == PLAN_LSTEP) p->hp = build_lstep_plan(*desc, tm, tn, kc, ctx->fuse, ctx->ref_flops != 0);

@@ -233,11 +233,12 @@ class GpuEngine:
 
     def set_schedule(self, rule):
         """Plan-construction schedule of the plans built afterwards: a
-        Schedule (fused-column rule, split-K chunk, GEMM row tile)."""
+        Schedule (fused-column rule, split-K chunk, GEMM row and column tiles)."""
         self.set_fuse(rule.fuse, rule.ratio, rule.min_entries)
         self.check(lib().mq_ctx_set_option(self.ctx, 3, rule.kchunk), "mq_ctx_set_option")
         self.check(lib().mq_ctx_set_option(self.ctx, 5, rule.tile_m), "mq_ctx_set_option")
         self.check(lib().mq_ctx_set_option(self.ctx, 7, rule.sched), "mq_ctx_set_option")
+        self.check(lib().mq_ctx_set_option(self.ctx, 8, rule.tile_n), "mq_ctx_set_option")
         self.schedule = rule
 
     def close(self):

@@ -1,8 +1,9 @@
 """The two-site split on the device (GpuEngine.split_twosite: mq_split's gather
 map, block Jacobi SVD and assembly) against the host split (mps.split_twosite,
 LAPACK SVD): the same kept bond and truncated weight, the same truncated
-two-site tensor A.B block by block, and orthonormal site tensors; and whole
-sweeps whose truncation runs on the device reproduce the host-truncated sweeps'
+two-site tensor A.B block by block, and orthonormal site tensors, up to the
+L=50 m=1024 centre tensor (blocks of several hundred columns); and whole sweeps
+whose truncation runs on the device reproduce the host-truncated sweeps'
 energies."""
 import numpy as np
 import pytest
@@ -12,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("l2r", [True, False])
 @pytest.mark.parametrize("symm,L,m,site,Mmax", [("su2u1", 12, 128, 5, 100), ("su2u1", 24, 512, 11, 400),
-                                                ("2u1", 12, 128, 5, 100)])
+                                                ("2u1", 12, 128, 5, 100), ("stretched", 50, 1024, 24, 1024)])
 def test_device_split_matches_host(gpu_engine, symm, L, m, site, Mmax, l2r):
     from test_split_cpu import _problem
     from qcmaquis_amd.mps import MPSTensor, split_twosite
