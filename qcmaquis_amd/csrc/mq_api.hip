@@ -1902,7 +1902,7 @@ struct mq_split {
     SplitMap map;
     DevArray<SplitSlab> slabs;
     DevArray<SvdBlk> blk;
-    DevArray<int32_t> order, perm, sweeps, keep;
+    DevArray<int32_t> perm, sweeps, keep;
     DevArray<int32_t> slot_blk, slot_k, done, rot;  // Jacobi rounds: one wave per column pair slot
     int nslots = 0, rounds = 0;                     // rounds per sweep: the largest block's Ce - 1
     DevArray<int64_t> aoff, boff;
@@ -1928,7 +1928,6 @@ int mq_split_create(mq_ctx *ctx, const mq_split_desc *desc, mq_split **out) {
         hipStream_t s = ctx->stream;
         sp->slabs.upload(m.slabs, s);
         sp->blk.upload(m.blk, s);
-        sp->order.upload(m.order, s);
         // pair slots of every block, the largest blocks first
         std::vector<int32_t> sb, sk;
         for (int32_t b : m.order) {

@@ -197,7 +197,7 @@ struct SplitMap {
     std::vector<SplitSlab> slabs;
     std::vector<SvdBlk> blk;                  // descending charge
     std::vector<int32_t> charge, rows, cols;  // per block, in blk order
-    std::vector<int32_t> order;               // block indices, largest working matrix first (launch order)
+    std::vector<int32_t> order;               // block indices, largest working matrix first (pair-slot order)
     int64_t psi_len = 0, x_total = 0, v_total = 0, s_total = 0;
     int max_c = 0;
 };
