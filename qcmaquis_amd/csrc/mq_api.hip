@@ -72,7 +72,8 @@ struct DevArray {
             }
         }
     }
-    void upload(const std::vector<T> &v, hipStream_t s) {
+    template <class A>
+    void upload(const std::vector<T, A> &v, hipStream_t s) {
         alloc(v.size());
         if (!v.empty() && hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s) != hipSuccess)
             throw std::runtime_error("hipMemcpyAsync failed");

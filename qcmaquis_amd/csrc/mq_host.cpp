@@ -9,30 +9,47 @@ Mpo to_mpo(const mq_mpo_desc &d) {
     m.cols = d.col_dim;
     m.col_b1.resize(d.col_dim);
     m.row_b2.resize(d.row_dim);
+    m.row_ent.resize(d.row_dim);
+    m.term_ptr.push_back(0);
     for (int b2 = 0; b2 < d.col_dim; ++b2)
         for (int e = d.col_ptr[b2]; e < d.col_ptr[b2 + 1]; ++e) {
             int b1 = d.row_idx[e];
             if (b1 < 0 || b1 >= d.row_dim) throw std::runtime_error("mpo: row index out of range");
+            // columns are walked in order, so row_b2[b1] stays ascending; a repeat
+            // of (b1, b2) would have been added to it just before
+            if (!m.row_b2[b1].empty() && m.row_b2[b1].back() == b2)
+                throw std::runtime_error("mpo: repeated (row, column) entry");
             m.col_b1[b2].push_back(b1);
             m.row_b2[b1].push_back(b2);
-            auto &t = m.at[{b1, b2}];
+            m.row_ent[b1].push_back((int)m.term_ptr.size() - 1);
             for (int q = d.term_ptr[e]; q < d.term_ptr[e + 1]; ++q) {
                 if (d.term_op[q] < 0 || d.term_op[q] >= d.n_ops) throw std::runtime_error("mpo: operator id out of range");
-                t.push_back({d.term_op[q], d.term_scale[q]});
+                m.term.push_back({d.term_op[q], d.term_scale[q]});
             }
+            m.term_ptr.push_back((int)m.term.size());
         }
-    for (auto &r : m.row_b2) std::sort(r.begin(), r.end());
+    if (d.n_ops < 0 || d.op_bptr[0] != 0) throw std::runtime_error("mpo: bad operator block pointers");
+    for (int o = 0; o < d.n_ops; ++o)
+        if (d.op_bptr[o + 1] < d.op_bptr[o]) throw std::runtime_error("mpo: bad operator block pointers");
+    const int nblk = d.op_bptr[d.n_ops];
+    if (d.opb_eptr[0] != 0) throw std::runtime_error("mpo: bad operator entry pointers");
+    for (int bb = 0; bb < nblk; ++bb)
+        if (d.opb_eptr[bb + 1] < d.opb_eptr[bb]) throw std::runtime_error("mpo: bad operator entry pointers");
+    const int nent = d.opb_eptr[nblk];
+    m.ent.resize(nent);
+    for (int e = 0; e < nent; ++e)
+        m.ent[e] = {d.ent_row[e], d.ent_col[e], d.ent_rspin[e], d.ent_cspin[e], d.ent_coeff[e]};
+    m.blk.resize(nblk);
+    for (int bb = 0; bb < nblk; ++bb) {
+        OpBlk &ob = m.blk[bb];
+        ob.in = Q(d.opb_in[2 * bb], d.opb_in[2 * bb + 1]);
+        ob.out = Q(d.opb_out[2 * bb], d.opb_out[2 * bb + 1]);
+        ob.e = {m.ent.data() + d.opb_eptr[bb], m.ent.data() + d.opb_eptr[bb + 1]};
+    }
     m.ops.resize(d.n_ops);
     for (int o = 0; o < d.n_ops; ++o) {
         m.ops[o].spin = d.op_spin[o];
-        for (int bb = d.op_bptr[o]; bb < d.op_bptr[o + 1]; ++bb) {
-            OpBlk ob;
-            ob.in = Q(d.opb_in[2 * bb], d.opb_in[2 * bb + 1]);
-            ob.out = Q(d.opb_out[2 * bb], d.opb_out[2 * bb + 1]);
-            for (int e = d.opb_eptr[bb]; e < d.opb_eptr[bb + 1]; ++e)
-                ob.e.push_back({d.ent_row[e], d.ent_col[e], d.ent_rspin[e], d.ent_cspin[e], d.ent_coeff[e]});
-            m.ops[o].blocks.push_back(ob);
-        }
+        m.ops[o].blocks = {m.blk.data() + d.op_bptr[o], m.blk.data() + d.op_bptr[o + 1]};
     }
     m.lspin.assign(d.left_spin, d.left_spin + d.row_dim);
     m.rspin.assign(d.right_spin, d.right_spin + d.col_dim);
@@ -42,10 +59,8 @@ Mpo to_mpo(const mq_mpo_desc &d) {
     m.rphase.assign(d.right_phase, d.right_phase + d.col_dim);
     m.row_nnz.assign(d.row_dim, 0);
     m.col_nnz.assign(d.col_dim, 0);
-    for (auto &kv : m.at) {
-        m.row_nnz[kv.first.first]++;
-        m.col_nnz[kv.first.second]++;
-    }
+    for (int b = 0; b < m.rows; ++b) m.row_nnz[b] = (int)m.row_b2[b].size();
+    for (int b = 0; b < m.cols; ++b) m.col_nnz[b] = (int)m.col_b1[b].size();
     m.one_rows = (int)std::count(m.row_nnz.begin(), m.row_nnz.end(), 1);
     m.one_cols = (int)std::count(m.col_nnz.begin(), m.col_nnz.end(), 1);
     for (int b = 0; b < m.rows; ++b)
@@ -72,6 +87,10 @@ BLayout to_layout(const mq_boundary_desc &d) {
                 throw std::runtime_error("boundary block outside its arena");
             L.b[b].push_back(x);
         }
+    L.desc.assign(d.aux_dim, 1);
+    for (int b = 0; b < d.aux_dim; ++b)
+        for (size_t k = 1; k < L.b[b].size(); ++k)
+            if (!(QPair{L.b[b][k].lc, L.b[b][k].rc} < QPair{L.b[b][k - 1].lc, L.b[b][k - 1].rc})) L.desc[b] = 0;
     return L;
 }
 

@@ -116,24 +116,59 @@ struct Ent {
     int row, col, rspin, cspin;
     double coeff;
 };
+// read-only view of a contiguous range
+template <class T>
+struct Span {
+    const T *b = nullptr, *e_ = nullptr;
+    const T *begin() const { return b; }
+    const T *end() const { return e_; }
+    size_t size() const { return (size_t)(e_ - b); }
+    bool empty() const { return b == e_; }
+    const T &operator[](size_t i) const { return b[i]; }
+};
 struct OpBlk {
     Q in, out;
-    std::vector<Ent> e;
+    Span<Ent> e;  // into Mpo::ent
 };
 struct Op {
     int spin = 0;
-    std::vector<OpBlk> blocks;
+    Span<OpBlk> blocks;  // into Mpo::blk
 };
 struct TermD {
     int op;
     double scale;
 };
 
+struct TermSpan {
+    const TermD *b, *e;
+    const TermD *begin() const { return b; }
+    const TermD *end() const { return e; }
+};
+
+// Move-only: its operators hold views into its own ent and blk arrays (a move
+// keeps the arrays' storage, a copy would not).
 struct Mpo {
+    Mpo() = default;
+    Mpo(const Mpo &) = delete;
+    Mpo &operator=(const Mpo &) = delete;
+    Mpo(Mpo &&) = default;
+    Mpo &operator=(Mpo &&) = default;
     int rows = 0, cols = 0;
-    std::vector<std::vector<int>> col_b1, row_b2;
-    std::map<std::pair<int, int>, std::vector<TermD>> at;
+    std::vector<std::vector<int>> col_b1, row_b2;  // row_b2[b1] ascending
+    std::vector<std::vector<int>> row_ent;         // entry id of (b1, row_b2[b1][k])
+    std::vector<int> term_ptr;                      // terms of entry e: term[term_ptr[e] .. term_ptr[e + 1])
+    std::vector<TermD> term;
+    // terms of the nonzero (b1, b2); throws std::out_of_range for a zero entry
+    TermSpan terms(int b1, int b2) const {
+        const std::vector<int> &r = row_b2[b1];
+        auto it = std::lower_bound(r.begin(), r.end(), b2);
+        if (it == r.end() || *it != b2) throw std::out_of_range("mpo: no entry at (b1, b2)");
+        const int e = row_ent[b1][it - r.begin()];
+        return {term.data() + term_ptr[e], term.data() + term_ptr[e + 1]};
+    }
     std::vector<Op> ops;
+    std::vector<OpBlk> blk;  // operator blocks, operator after operator
+    std::vector<Ent> ent;    // sparse entries, block after block
     std::vector<int> lspin, rspin, lherm, rherm, lphase, rphase, row_nnz, col_nnz;
     int one_rows = 0, one_cols = 0;
     bool lskip(int b) const { return lherm[b] < b; }
@@ -151,9 +186,17 @@ struct BBlk {
 struct BLayout {
     int aux = 0;
     std::vector<std::vector<BBlk>> b;  // per bond, descending (lc, rc)
+    std::vector<char> desc;            // per bond: blocks strictly descending in (lc, rc)
     int64_t total = 0;
     const BBlk *find(int bond, Q lc, Q rc) const {
-        for (auto &x : b[bond])
+        const std::vector<BBlk> &v = b[bond];
+        if (!desc.empty() && desc[bond]) {
+            const QPair key{lc, rc};
+            auto it = std::lower_bound(v.begin(), v.end(), key,
+                                       [](const BBlk &x, const QPair &k) { return k < QPair{x.lc, x.rc}; });
+            return it != v.end() && it->lc == lc && it->rc == rc ? &*it : nullptr;
+        }
+        for (auto &x : v)
             if (x.lc == lc && x.rc == rc) return &x;
         return nullptr;
     }

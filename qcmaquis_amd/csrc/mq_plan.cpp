@@ -115,12 +115,13 @@ struct Site {
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-// f(i) for i < n on up to 16 host threads (dynamic chunks); f must only write
-// data owned by index i.  Exceptions are rethrown on the calling thread.
+// f(i) for i < n on up to 16 host threads, at least `grain` indices per thread,
+// taken `chunk` at a time; f must only write data owned by index i.  Exceptions
+// are rethrown on the calling thread.
 template <class F>
-void parallel_for(int n, F &&f) {
+void parallel_for(int n, F &&f, int grain = 64, int chunk = 16) {
     int nt = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
-    nt = std::min(nt, std::max(1, n / 64));
+    nt = std::min(nt, std::max(1, n / grain));
     if (nt <= 1) {
         for (int i = 0; i < n; ++i) f(i);
         return;
@@ -131,9 +132,9 @@ void parallel_for(int n, F &&f) {
     auto work = [&]() {
         try {
             for (;;) {
-                int i0 = next.fetch_add(16);
+                int i0 = next.fetch_add(chunk);
                 if (i0 >= n) break;
-                for (int i = i0; i < std::min(n, i0 + 16); ++i) f(i);
+                for (int i = i0; i < std::min(n, i0 + chunk); ++i) f(i);
             }
         } catch (...) {
             std::lock_guard<std::mutex> lk(mu);
@@ -338,61 +339,75 @@ ColProg column_program(const Site &s, bool skip_rskip_cols, FuseRule fuse) {
         int b1;
         double w;
     };
-    std::vector<E> ents;
+    // per column, in parallel: its direct terms (cp.col[b2]) or its fused groups
+    std::vector<std::vector<FGroup>> groups(s.W.cols);
+    std::vector<char> fused_col(s.W.cols, 0);
+    parallel_for(
+        s.W.cols,
+        [&](int b2) {
+            if (skip_rskip_cols && s.W.rskip(b2)) return;
+            thread_local std::vector<E> ents;
+            ents.clear();
+            for (int b1 : s.W.col_b1[b2])
+                for (auto &term : s.W.terms(b1, b2)) {
+                    const Op &op = s.W.ops[term.op];
+                    for (auto &ob : s.opb[term.op])
+                        for (auto &e : ob.b->e) {
+                            if (ob.pin > 255 || ob.pout > 255 || e.row > 255 || e.col > 255 || e.rspin > 3 ||
+                                e.cspin > 3 || op.spin > 127 || s.W.lspin[b1] > 127)
+                                throw std::runtime_error("operator entry out of the fused-key range");
+                            uint64_t key = ((uint64_t)ob.pin << 56) | ((uint64_t)ob.pout << 48) |
+                                           ((uint64_t)e.row << 40) | ((uint64_t)e.col << 32) |
+                                           ((uint64_t)e.rspin << 30) | ((uint64_t)e.cspin << 28) |
+                                           ((uint64_t)op.spin << 21) | ((uint64_t)s.W.lspin[b1] << 14);
+                            ents.push_back({key, b1, e.coeff * term.scale});
+                        }
+                }
+            std::stable_sort(ents.begin(), ents.end(), [](const E &x, const E &y) { return x.key < y.key; });
+            size_t ngroups = 0;
+            for (size_t i = 0; i < ents.size(); ++i)
+                if (i == 0 || ents[i].key != ents[i - 1].key) ++ngroups;
+            bool fused = (fuse.mode == 2 && !ents.empty()) ||
+                         (fuse.mode == 1 && !ents.empty() && ents.size() >= (size_t)fuse.min_entries &&
+                          ents.size() >= (size_t)fuse.ratio * ngroups);
+            if (!fused) {
+                for (int b1 : s.W.col_b1[b2])
+                    for (auto &term : s.W.terms(b1, b2)) cp.col[b2].push_back({b1, term.op, term.scale, s.W.lspin[b1]});
+                return;
+            }
+            fused_col[b2] = 1;
+            size_t i = 0;
+            while (i < ents.size()) {
+                size_t j = i;
+                while (j < ents.size() && ents[j].key == ents[i].key) ++j;
+                uint64_t k = ents[i].key;
+                FGroup g;
+                g.b2 = b2;
+                g.pin = (int)(k >> 56);
+                g.pout = (int)((k >> 48) & 255);
+                g.row = (int)((k >> 40) & 255);
+                g.col = (int)((k >> 32) & 255);
+                g.rspin = (int)((k >> 30) & 3);
+                g.cspin = (int)((k >> 28) & 3);
+                g.k = (int)((k >> 21) & 127);
+                g.a = (int)((k >> 14) & 127);
+                for (size_t q = i; q < j; ++q) g.mem.push_back({ents[q].b1, ents[q].w});
+                groups[b2].push_back(std::move(g));
+                i = j;
+            }
+        },
+        8, 1);
+    // in column order: the rows with direct terms, the group ids
     for (int b2 = 0; b2 < s.W.cols; ++b2) {
         if (skip_rskip_cols && s.W.rskip(b2)) continue;
-        ents.clear();
-        for (int b1 : s.W.col_b1[b2])
-            for (auto &term : s.W.at.at({b1, b2})) {
-                const Op &op = s.W.ops[term.op];
-                for (auto &ob : s.opb[term.op])
-                    for (auto &e : ob.b->e) {
-                        if (ob.pin > 255 || ob.pout > 255 || e.row > 255 || e.col > 255 || e.rspin > 3 ||
-                            e.cspin > 3 || op.spin > 127 || s.W.lspin[b1] > 127)
-                            throw std::runtime_error("operator entry out of the fused-key range");
-                        uint64_t key = ((uint64_t)ob.pin << 56) | ((uint64_t)ob.pout << 48) |
-                                       ((uint64_t)e.row << 40) | ((uint64_t)e.col << 32) |
-                                       ((uint64_t)e.rspin << 30) | ((uint64_t)e.cspin << 28) |
-                                       ((uint64_t)op.spin << 21) | ((uint64_t)s.W.lspin[b1] << 14);
-                        ents.push_back({key, b1, e.coeff * term.scale});
-                    }
-            }
-        std::stable_sort(ents.begin(), ents.end(), [](const E &x, const E &y) { return x.key < y.key; });
-        size_t ngroups = 0;
-        for (size_t i = 0; i < ents.size(); ++i)
-            if (i == 0 || ents[i].key != ents[i - 1].key) ++ngroups;
-        bool fused = (fuse.mode == 2 && !ents.empty()) ||
-                     (fuse.mode == 1 && !ents.empty() && ents.size() >= (size_t)fuse.min_entries &&
-                      ents.size() >= (size_t)fuse.ratio * ngroups);
-        if (!fused) {
-            for (int b1 : s.W.col_b1[b2]) {
-                for (auto &term : s.W.at.at({b1, b2})) cp.col[b2].push_back({b1, term.op, term.scale, s.W.lspin[b1]});
-                cp.direct[b1] = 1;
-            }
+        if (!fused_col[b2]) {
+            for (int b1 : s.W.col_b1[b2]) cp.direct[b1] = 1;
             continue;
         }
-        size_t i = 0;
-        while (i < ents.size()) {
-            size_t j = i;
-            while (j < ents.size() && ents[j].key == ents[i].key) ++j;
-            uint64_t k = ents[i].key;
-            FGroup g;
-            g.b2 = b2;
-            g.pin = (int)(k >> 56);
-            g.pout = (int)((k >> 48) & 255);
-            g.row = (int)((k >> 40) & 255);
-            g.col = (int)((k >> 32) & 255);
-            g.rspin = (int)((k >> 30) & 3);
-            g.cspin = (int)((k >> 28) & 3);
-            g.k = (int)((k >> 21) & 127);
-            g.a = (int)((k >> 14) & 127);
-            for (size_t q = i; q < j; ++q) {
-                g.mem.push_back({ents[q].b1, ents[q].w});
-                cp.member[ents[q].b1] = 1;
-            }
+        for (auto &g : groups[b2]) {
+            for (auto &m : g.mem) cp.member[m.first] = 1;
             cp.fcol[b2].push_back((int)cp.fg.size());
             cp.fg.push_back(std::move(g));
-            i = j;
         }
     }
     return cp;
@@ -496,6 +511,21 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
         for (int b1 = 0; b1 < nb1; ++b1)
             if (real[b1] && (!cp || cp->direct[b1]))
                 for (auto &a : blk[b1]) rows_of[a.lk].push_back({a.row, s.lsize(a.lb)});
+        std::vector<int64_t> map0(s.nl + 1, (int64_t)g1->rowmap.size());
+        for (int lk = 0; lk < s.nl; ++lk) map0[lk + 1] = map0[lk] + t.M[lk];
+        g1->rowmap.resize((size_t)map0[s.nl]);
+        parallel_for(
+            s.nl,
+            [&](int lk) {
+                RowMap *rm = g1->rowmap.data() + map0[lk];
+                std::fill_n(rm, t.M[lk], RowMap{-1, 0, 0});
+                for (auto &rb : rows_of[lk])
+                    for (int r = 0; r < rb.second; ++r)
+                        rm[rb.first + r] = {(int64_t)rb.first * s.pbr_sz_l[lk] + r, rb.second, 0};
+                for (int r = 0; r < t.M[lk]; ++r)
+                    if (rm[r].off < 0) throw std::runtime_error("T panel row without a block");
+            },
+            1, 1);
         for (int lk = 0; lk < s.nl; ++lk) {
             if (t.M[lk] == 0) continue;
             GemmGroup g{};
@@ -508,14 +538,7 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
             g.lda = t.Mtot[lk];
             g.ldb = g.K;
             g.ldc = g.M;
-            g.rowmap = (int64_t)g1->rowmap.size() + 1;
-            g1->rowmap.resize(g1->rowmap.size() + g.M, RowMap{-1, 0, 0});
-            RowMap *rm = g1->rowmap.data() + (g.rowmap - 1);
-            for (auto &rb : rows_of[lk])
-                for (int r = 0; r < rb.second; ++r)
-                    rm[rb.first + r] = {(int64_t)rb.first * g.N + r, rb.second, 0};
-            for (int r = 0; r < g.M; ++r)
-                if (rm[r].off < 0) throw std::runtime_error("T panel row without a block");
+            g.rowmap = map0[lk] + 1;
             add_gemm(*g1, g, tm, tn, kc);
         }
     }
@@ -655,14 +678,13 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     }
     if (comb) {
         if (comb_key.size() != comb->strips.size()) throw std::runtime_error("combination strip list out of step");
-        std::vector<size_t> ord(comb->strips.size());
-        for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
-            uint64_t kx = (comb_key[x] << 32) | (comb_key[x] >> 32), ky = (comb_key[y] << 32) | (comb_key[y] >> 32);
-            return kx < ky;
-        });
-        std::vector<Strip> sorted(ord.size());
-        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = comb->strips[ord[i]];
+        // stable counting sort by (lk, lb)
+        std::vector<int> start((size_t)nl * nl + 1, 0);
+        for (uint64_t k : comb_key) ++start[(size_t)(uint32_t)k * nl + (k >> 32) + 1];
+        for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+        TaskVec<Strip> sorted(comb->strips.size());
+        for (size_t i = 0; i < comb_key.size(); ++i)
+            sorted[start[(size_t)(uint32_t)comb_key[i] * nl + (comb_key[i] >> 32)]++] = comb->strips[i];
         comb->strips.swap(sorted);
     }
     // row maps of the fused panels: each panel's rows at a known offset, filled in parallel
@@ -670,13 +692,16 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     for (size_t k = 0; k < panels.size(); ++k) map0[k + 1] = map0[k] + panels[k].rows;
     if (g1) {
         g1->rowmap.resize((size_t)map0.back());
-        parallel_for((int)panels.size(), [&](int k) {
-            const Panel &pn = panels[k];
-            const int rsz = s.rsize(pn.ar);
-            RowMap *rm = g1->rowmap.data() + map0[k];
-            for (auto &rb : pn.blocks)
-                for (int r = 0; r < rb.second; ++r) *rm++ = {(int64_t)rb.first * rsz + r, rb.second, 0};
-        });
+        parallel_for(
+            (int)panels.size(),
+            [&](int k) {
+                const Panel &pn = panels[k];
+                const int rsz = s.rsize(pn.ar);
+                RowMap *rm = g1->rowmap.data() + map0[k];
+                for (auto &rb : pn.blocks)
+                    for (int r = 0; r < rb.second; ++r) *rm++ = {(int64_t)rb.first * rsz + r, rb.second, 0};
+            },
+            1, 1);
     }
     for (size_t k = 0; k < panels.size(); ++k) {
         const Panel &pn = panels[k];
@@ -784,7 +809,7 @@ void lbtm_column(const Site &s, const TLay &t, const ColProg *cp, const std::vec
 std::vector<CTerm> reference_terms(const Site &s, int b2) {
     std::vector<CTerm> v;
     for (int b1 : s.W.col_b1[b2])
-        for (auto &term : s.W.at.at({b1, b2})) v.push_back({b1, term.op, term.scale, s.W.lspin[b1]});
+        for (auto &term : s.W.terms(b1, b2)) v.push_back({b1, term.op, term.scale, s.W.lspin[b1]});
     return v;
 }
 
@@ -824,24 +849,38 @@ struct Seg {
     double rscale;
 };
 
+// Row strips (row0, rows) of each output block al, in row order: the (phys, lb,
+// phys state) partition of the block's rows, computed once per block.
+struct RowStrips {
+    std::vector<std::vector<std::pair<int, int>>> of;
+    std::vector<char> done;
+    explicit RowStrips(const Site &s) : of(s.nr), done(s.nr, 0) {}
+    const std::vector<std::pair<int, int>> &get(const Site &s, int al) {
+        if (!done[al]) {
+            auto &rs = of[al];
+            for (int p = 0; p < s.np; ++p) {
+                int lb = s.rmp_l[(size_t)al * s.np + p];
+                if (lb < 0) continue;
+                int lsz = s.lsize(lb);
+                for (int ss = 0; ss < s.phys.s[p]; ++ss) rs.push_back({s.pbl[(size_t)p * s.nl + lb] + ss * lsz, lsz});
+            }
+            std::sort(rs.begin(), rs.end());
+            done[al] = 1;
+        }
+        return of[al];
+    }
+};
+
 // W-application strips of one segment; dst offsets are relative to the al block start
-// (patched with the C arena offset of al once every K_al is known).
-void emit_segment_strips(const Site &s, const Seg &sg, const Raw *r0, const Raw *r1, StripList &sl,
+// (patched with the C arena offset of al once every K_al is known).  The row strips
+// partition the al block; they are walked in row order alongside the sources, which
+// are sorted by row0 inside the segment.
+void emit_segment_strips(const Site &s, const Seg &sg, const Raw *r0, const Raw *r1, RowStrips &rows, StripList &sl,
                          std::vector<int> &strip_al) {
     int xi = s.x_of_r[sg.al];
     int rows_al = s.X.blk[xi].rows;
     int rsz = s.rsize(sg.ar);
-    // the row strips (s, lb, ss) partition the al block; walk them in row order
-    // alongside the sources, which are sorted by row0 inside the segment
-    thread_local std::vector<std::pair<int, int>> rs;  // (row0, rows)
-    rs.clear();
-    for (int p = 0; p < s.np; ++p) {
-        int lb = s.rmp_l[(size_t)sg.al * s.np + p];
-        if (lb < 0) continue;
-        int lsz = s.lsize(lb);
-        for (int ss = 0; ss < s.phys.s[p]; ++ss) rs.push_back({s.pbl[(size_t)p * s.nl + lb] + ss * lsz, lsz});
-    }
-    std::sort(rs.begin(), rs.end());
+    const std::vector<std::pair<int, int>> &rs = rows.get(s, sg.al);
     const Raw *q = r0;
     for (auto &r : rs) {
         Strip st;
@@ -860,6 +899,31 @@ void emit_segment_strips(const Site &s, const Seg &sg, const Raw *r0, const Raw 
         strip_al.push_back(sg.al);
     }
     if (q != r1) throw std::runtime_error("W-application source outside its segment");
+}
+
+// The columns' W-application sources (lbtm_column), gathered and sorted in parallel
+// a batch of columns at a time (bounded memory) and handed to emit(b2, raw, x) in
+// column order, x = extra(b2) computed beside the gathering.  skip_rskip: the
+// columns stored as their hermitian partner are left out.
+template <class Extra, class Emit>
+void column_sources(const Site &s, const TLay &t, const ColProg &cp, bool skip_rskip, Extra &&extra, Emit &&emit) {
+    const int batch = 512;
+    std::vector<std::vector<Raw>> raws(std::min(batch, s.W.cols));
+    std::vector<double> ex(raws.size(), 0.0);
+    for (int c0 = 0; c0 < s.W.cols; c0 += batch) {
+        const int nb = std::min(batch, s.W.cols - c0);
+        parallel_for(
+            nb,
+            [&](int k) {
+                const int b2 = c0 + k;
+                if (skip_rskip && s.W.rskip(b2)) return;
+                lbtm_column(s, t, &cp, cp.col[b2], b2, raws[k]);
+                ex[k] = extra(b2);
+            },
+            8, 1);
+        for (int k = 0; k < nb; ++k)
+            if (!(skip_rskip && s.W.rskip(c0 + k))) emit(c0 + k, raws[k], ex[k]);
+    }
 }
 
 void patch_strips(StripList &sl, const std::vector<int> &strip_al, const std::vector<int64_t> &c_off) {
@@ -961,31 +1025,32 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     std::vector<int> K(s.nr, 0);
     std::vector<Seg> segs;
     std::vector<int> strip_al;
-    std::vector<Raw> raw;
     // reference schedule (site_hamil.hpp:35): when it is lbtm its FLOPs are counted in this pass
     const bool ref_lbtm =
         ref_count && (s.symm != MQ_SYMM_SU2U1 || (s.W.rows - s.W.one_rows) < (s.W.cols - s.W.one_cols));
     TLay tall;
     if (ref_lbtm) tall = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, tm, tn, kc);
     double ref_axpy = 0, ref_trim = 0;
-    for (int b2 = 0; b2 < s.W.cols; ++b2) {
-        lbtm_column(s, t, &cp, cp.col[b2], b2, raw);
-        if (ref_lbtm) ref_axpy += reference_axpy_flops(s, tall, b2);
-        size_t i = 0;
-        while (i < raw.size()) {
-            size_t j = i;
-            while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
-            Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
-            if (r_operand(s, R, b2, s.right.q[sg.ar], s.right.q[sg.al], &sg.rblk, &sg.rtrans, &sg.rscale)) {
-                sg.koff = K[sg.al];
-                K[sg.al] += s.rsize(sg.ar);
-                segs.push_back(sg);
-                emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, P.wapply, strip_al);
-                ref_trim += 2.0 * s.X.blk[s.x_of_r[sg.al]].rows * s.rsize(sg.ar) * s.rsize(sg.al);
+    RowStrips rstrips(s);
+    column_sources(
+        s, t, cp, false, [&](int b2) { return ref_lbtm ? reference_axpy_flops(s, tall, b2) : 0.0; },
+        [&](int b2, const std::vector<Raw> &raw, double axpy) {
+            ref_axpy += axpy;
+            size_t i = 0;
+            while (i < raw.size()) {
+                size_t j = i;
+                while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
+                Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
+                if (r_operand(s, R, b2, s.right.q[sg.ar], s.right.q[sg.al], &sg.rblk, &sg.rtrans, &sg.rscale)) {
+                    sg.koff = K[sg.al];
+                    K[sg.al] += s.rsize(sg.ar);
+                    segs.push_back(sg);
+                    emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, rstrips, P.wapply, strip_al);
+                    ref_trim += 2.0 * s.X.blk[s.x_of_r[sg.al]].rows * s.rsize(sg.ar) * s.rsize(sg.al);
+                }
+                i = j;
             }
-            i = j;
-        }
-    }
+        });
     std::vector<int64_t> c_off(s.nr, 0), rb_off(s.nr, 0);
     int64_t ct = 0, rbt = 0;
     for (auto &xb : s.X.blk) {
@@ -1079,26 +1144,26 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     std::vector<int> K(s.nr, 0);
     std::vector<Seg> segs;
     std::vector<int> strip_al;
-    std::vector<Raw> raw;
-    for (int b2 = 0; b2 < s.W.cols; ++b2) {
-        if (s.W.rskip(b2)) continue;
-        lbtm_column(s, t, &cp, cp.col[b2], b2, raw);
-        if (ref_count) axf += reference_axpy_flops(s, tall, b2);
-        size_t i = 0;
-        while (i < raw.size()) {
-            size_t j = i;
-            while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
-            Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
-            // spin filter of SU2::gemm(transpose(contr), bra_conj, ret[b2], spin(b2)) (move_boundary.hpp:165)
-            if (!s.su2 || tri(spin2(s.right.q[sg.ar]), s.W.rspin[b2], spin2(s.right.q[sg.al]))) {
-                sg.koff = K[sg.al];
-                K[sg.al] += s.rsize(sg.ar);
-                segs.push_back(sg);
-                emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, P.wapply, strip_al);
+    RowStrips rstrips(s);
+    column_sources(
+        s, t, cp, true, [&](int b2) { return ref_count ? reference_axpy_flops(s, tall, b2) : 0.0; },
+        [&](int b2, const std::vector<Raw> &raw, double axpy) {
+            axf += axpy;
+            size_t i = 0;
+            while (i < raw.size()) {
+                size_t j = i;
+                while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
+                Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
+                // spin filter of SU2::gemm(transpose(contr), bra_conj, ret[b2], spin(b2)) (move_boundary.hpp:165)
+                if (!s.su2 || tri(spin2(s.right.q[sg.ar]), s.W.rspin[b2], spin2(s.right.q[sg.al]))) {
+                    sg.koff = K[sg.al];
+                    K[sg.al] += s.rsize(sg.ar);
+                    segs.push_back(sg);
+                    emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, rstrips, P.wapply, strip_al);
+                }
+                i = j;
             }
-            i = j;
-        }
-    }
+        });
     std::vector<int64_t> c_off(s.nr, 0), tmp_off(s.nr, 0);
     int64_t ct = 0, tt = 0;
     for (auto &xb : s.X.blk) {
@@ -1283,7 +1348,7 @@ RbtmCore rbtm_core(const Site &s, const BLayout &R, const BLayout *Lsig, HostPla
         int a = s.W.lspin[b1];
         for (int b2 : s.W.row_b2[b1]) {
             int ap = s.W.rspin[b2];
-            for (auto &term : s.W.at.at({b1, b2})) {
+            for (auto &term : s.W.terms(b1, b2)) {
                 const Op &op = s.W.ops[term.op];
                 for (auto &t : tb[b2]) {
                     int jp = spin2(s.right.q[t.c]), ip = spin2(s.right.q[t.cp]);
@@ -1655,7 +1720,7 @@ double reference_sigma_flops(const mq_site_desc &d) {
         }
         std::unordered_map<uint64_t, std::vector<std::pair<double, int>>> groups;  // (or, ol)
         for (int b2 : s.W.row_b2[b1]) {
-            for (auto &term : s.W.at.at({b1, b2})) {
+            for (auto &term : s.W.terms(b1, b2)) {
                 for (auto &pr : tblocks[b2]) {
                     for (auto &ob : s.opb[term.op]) {
                         int ol = s.rmp_l[(size_t)pr.first * s.np + ob.pin];
@@ -1784,7 +1849,7 @@ HostPlan build_diag_plan(const mq_site_desc &d, int tm, int tn, int kc) {
             for (auto &a_ : acc) a_.clear();
             for (int b1 : s.W.col_b1[b2]) {
                 const int a = s.W.lspin[b1];
-                for (const TermD &term : s.W.at.at({b1, b2})) {
+                for (const TermD &term : s.W.terms(b1, b2)) {
                     const int opk = s.W.ops[term.op].spin;
                     for (const Site::OpB &ob : s.opb[term.op]) {
                         if (ob.pin != ob.pout) continue;

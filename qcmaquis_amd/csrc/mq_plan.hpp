@@ -8,6 +8,31 @@ namespace mq {
 
 // ---- device task records (plain old data, copied to the GPU as arrays) ----
 
+// Allocator whose resize() leaves the records uninitialised: the builder writes
+// every record of a task list it sizes, often from several threads, and zero-
+// filling lists of 10^6-10^7 records first would be one more pass over them on
+// one thread.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using TaskVec = std::vector<T, NoInitAlloc<T>>;
+
 // dst[i + j*ldd] = scale * (trans ? src[j + i*lds] : src[i + j*lds]), i < rows, j < cols
 struct Copy2D {
     int64_t dst, src;
@@ -67,7 +92,7 @@ struct Src {
 struct GemmList {
     std::vector<GemmGroup> groups;
     std::vector<GemmTile> tiles;
-    std::vector<RowMap> rowmap;  // row placements of the groups with rowmap >= 0
+    TaskVec<RowMap> rowmap;  // row placements of the groups with rowmap >= 0
     int64_t ws = 0;  // workspace elements for split partials
     double flops() const {
         double f = 0;
@@ -80,8 +105,8 @@ struct GemmList {
 // together at most kPackElems elements (the fragmented blocks of small sectors).
 constexpr int kPackStrips = 64, kPackElems = 1024;
 struct StripList {
-    std::vector<Strip> strips;
-    std::vector<Src> srcs;
+    TaskVec<Strip> strips;
+    TaskVec<Src> srcs;
     std::vector<int> packs;  // pack p runs strips [packs[p], packs[p + 1])
 };
 // (re)compute the packs of a strip list, keeping the strip order
