@@ -1331,100 +1331,123 @@ RbtmCore rbtm_core(const Site &s, const BLayout &R, const BLayout *Lsig, HostPla
         double coef;
     };
     out.Mr.assign(s.nl, 0);
-    std::vector<RR> raw;
-    for (int b1 = 0; b1 < s.W.rows; ++b1) {
-        const bool lskip = s.W.lskip(b1);
-        if (!Lsig && lskip) continue;
-        // (bra, ket) left ids of the L blocks the sigma's charge_gemm can use
-        std::unordered_map<uint64_t, char> lblk;
-        if (Lsig)
-            for (auto &x : Lsig->b[lskip ? s.W.lherm[b1] : b1]) {
-                int p = s.li.get(x.lc), q = s.li.get(x.rc);
-                if (p < 0 || q < 0) continue;
-                int bra = lskip ? p : q, ket = lskip ? q : p;
-                lblk[((uint64_t)bra << 32) | (uint32_t)ket] = 1;
-            }
-        raw.clear();
-        int a = s.W.lspin[b1];
-        for (int b2 : s.W.row_b2[b1]) {
-            int ap = s.W.rspin[b2];
-            for (auto &term : s.W.terms(b1, b2)) {
-                const Op &op = s.W.ops[term.op];
-                for (auto &t : tb[b2]) {
-                    int jp = spin2(s.right.q[t.c]), ip = spin2(s.right.q[t.cp]);
-                    for (auto &ob : s.opb[term.op]) {
-                        const OpBlk &wb = *ob.b;
-                        int pin = ob.pin, pout = ob.pout;
-                        int ol = s.rmp_l[(size_t)t.c * s.np + pin];
-                        if (ol < 0) continue;
-                        int orr = s.rmp_l[(size_t)t.cp * s.np + pout];
-                        if (orr < 0 || s.pbr_sz_l[orr] == 0) continue;  // out_right_i
-                        int j = spin2(s.left.q[ol]), i = spin2(s.left.q[orr]);
-                        if (s.su2 && !tri(j, a, i)) continue;
-                        if (Lsig && !lblk.count(((uint64_t)orr << 32) | (uint32_t)ol)) continue;
-                        int lsz = s.lsize(ol), rsz = s.rsize(t.cp);
-                        static const std::array<double, 4> one{1.0, 1.0, 1.0, 1.0};
-                        const std::array<double, 4> &cpl =
-                            s.su2 ? coupling(j, std::abs(j - jp), jp, a, op.spin, ap, i, std::abs(i - ip), ip) : one;
-                        int ilo = s.pbl[(size_t)pin * s.nl + ol];
-                        int oro = s.pbr[(size_t)pout * s.nr + t.cp];
-                        for (auto &e : wb.e) {
-                            raw.push_back({orr, ol, oro + e.col * rsz, t.off + ilo + e.row * lsz, t.ld,
-                                           e.coeff * term.scale * (s.su2 ? cpl[case_of(e.rspin, e.cspin)] : 1.0)});
-                            out.axf += 2.0 * lsz * rsz;
+    // per MPO row, in parallel a batch of rows at a time (bounded memory): the
+    // row's sources, sorted and merged; then strips in row order
+    const int batch = 256;
+    std::vector<std::vector<RR>> raws(std::min(batch, std::max(s.W.rows, 1)));
+    std::vector<double> axfs(raws.size(), 0.0);
+    for (int r0 = 0; r0 < s.W.rows; r0 += batch) {
+        const int nb = std::min(batch, s.W.rows - r0);
+        parallel_for(
+            nb,
+            [&](int k) {
+                const int b1 = r0 + k;
+                std::vector<RR> &raw = raws[k];
+                double &axf = axfs[k];
+                raw.clear();
+                axf = 0.0;
+                const bool lskip = s.W.lskip(b1);
+                if (!Lsig && lskip) return;
+                // (bra, ket) left ids of the L blocks the sigma's charge_gemm can use
+                std::unordered_map<uint64_t, char> lblk;
+                if (Lsig)
+                    for (auto &x : Lsig->b[lskip ? s.W.lherm[b1] : b1]) {
+                        int p = s.li.get(x.lc), q = s.li.get(x.rc);
+                        if (p < 0 || q < 0) continue;
+                        int bra = lskip ? p : q, ket = lskip ? q : p;
+                        lblk[((uint64_t)bra << 32) | (uint32_t)ket] = 1;
+                    }
+                int a = s.W.lspin[b1];
+                for (int b2 : s.W.row_b2[b1]) {
+                    int ap = s.W.rspin[b2];
+                    for (auto &term : s.W.terms(b1, b2)) {
+                        const Op &op = s.W.ops[term.op];
+                        for (auto &t : tb[b2]) {
+                            int jp = spin2(s.right.q[t.c]), ip = spin2(s.right.q[t.cp]);
+                            for (auto &ob : s.opb[term.op]) {
+                                const OpBlk &wb = *ob.b;
+                                int pin = ob.pin, pout = ob.pout;
+                                int ol = s.rmp_l[(size_t)t.c * s.np + pin];
+                                if (ol < 0) continue;
+                                int orr = s.rmp_l[(size_t)t.cp * s.np + pout];
+                                if (orr < 0 || s.pbr_sz_l[orr] == 0) continue;  // out_right_i
+                                int j = spin2(s.left.q[ol]), i = spin2(s.left.q[orr]);
+                                if (s.su2 && !tri(j, a, i)) continue;
+                                if (Lsig && !lblk.count(((uint64_t)orr << 32) | (uint32_t)ol)) continue;
+                                int lsz = s.lsize(ol), rsz = s.rsize(t.cp);
+                                static const std::array<double, 4> one{1.0, 1.0, 1.0, 1.0};
+                                const std::array<double, 4> &cpl =
+                                    s.su2 ? coupling(j, std::abs(j - jp), jp, a, op.spin, ap, i, std::abs(i - ip), ip)
+                                          : one;
+                                int ilo = s.pbl[(size_t)pin * s.nl + ol];
+                                int oro = s.pbr[(size_t)pout * s.nr + t.cp];
+                                for (auto &e : wb.e) {
+                                    const double c = s.su2 ? cpl[case_of(e.rspin, e.cspin)] : 1.0;
+                                    raw.push_back({orr, ol, oro + e.col * rsz, t.off + ilo + e.row * lsz, t.ld,
+                                                   e.coeff * term.scale * c});
+                                    axf += 2.0 * lsz * rsz;
+                                }
+                            }
                         }
                     }
                 }
-            }
-        }
-        std::sort(raw.begin(), raw.end(), [](const RR &x, const RR &y) {
-            if (x.orr != y.orr) return x.orr < y.orr;
-            if (x.ol != y.ol) return x.ol < y.ol;
-            if (x.col0 != y.col0) return x.col0 < y.col0;
-            return x.off < y.off;
-        });
-        size_t w = 0;
-        for (size_t r = 0; r < raw.size(); ++r) {
-            if (w > 0 && raw[w - 1].orr == raw[r].orr && raw[w - 1].ol == raw[r].ol && raw[w - 1].col0 == raw[r].col0 &&
-                raw[w - 1].off == raw[r].off)
-                raw[w - 1].coef += raw[r].coef;
-            else
-                raw[w++] = raw[r];
-        }
-        raw.resize(w);
-        size_t i = 0;
-        while (i < raw.size()) {
-            size_t j = i;
-            while (j < raw.size() && raw[j].orr == raw[i].orr && raw[j].ol == raw[i].ol) ++j;
-            int orr = raw[i].orr, ol = raw[i].ol;
-            // spin filter of SU2::gemm(ret[b1], transpose(bra_conj), tmp, spin(b1)) (move_boundary.hpp:224);
-            // the sigma's tasks passed the same triangle test (apply_op.hpp:163)
-            if (!s.su2 || tri(spin2(s.left.q[ol]), a, spin2(s.left.q[orr]))) {
-                RowBlk rb{b1, ol, orr, out.Mr[orr]};
-                out.Mr[orr] += s.lsize(ol);
-                out.rowblks.push_back(rb);
-                int lsz = s.lsize(ol);
-                Q orq = s.left.q[orr];
-                for (int p = 0; p < s.np; ++p) {
-                    int r = s.ri.get(orq + s.phys.q[p]);
-                    if (r < 0) continue;
-                    int rsz = s.rsize(r);
-                    for (int ss = 0; ss < s.phys.s[p]; ++ss) {
-                        int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
-                        Strip st;
-                        st.dst = 0;  // set below, once every BUF height is known
-                        st.rows = lsz;
-                        st.cols = rsz;
-                        st.ldd = 0;
-                        st.s0 = (int)P.wapply.srcs.size();
-                        for (size_t q = i; q < j; ++q)
-                            if (raw[q].col0 == col0) P.wapply.srcs.push_back({raw[q].off, raw[q].ld, raw[q].coef});
-                        st.s1 = (int)P.wapply.srcs.size();
-                        P.wapply.strips.push_back(st);
+                std::sort(raw.begin(), raw.end(), [](const RR &x, const RR &y) {
+                    if (x.orr != y.orr) return x.orr < y.orr;
+                    if (x.ol != y.ol) return x.ol < y.ol;
+                    if (x.col0 != y.col0) return x.col0 < y.col0;
+                    return x.off < y.off;
+                });
+                size_t w = 0;
+                for (size_t r = 0; r < raw.size(); ++r) {
+                    if (w > 0 && raw[w - 1].orr == raw[r].orr && raw[w - 1].ol == raw[r].ol &&
+                        raw[w - 1].col0 == raw[r].col0 && raw[w - 1].off == raw[r].off)
+                        raw[w - 1].coef += raw[r].coef;
+                    else
+                        raw[w++] = raw[r];
+                }
+                raw.resize(w);
+            },
+            4, 1);
+        for (int k = 0; k < nb; ++k) {
+            const int b1 = r0 + k;
+            if (!Lsig && s.W.lskip(b1)) continue;
+            const std::vector<RR> &raw = raws[k];
+            const int a = s.W.lspin[b1];
+            out.axf += axfs[k];
+            size_t i = 0;
+            while (i < raw.size()) {
+                size_t j = i;
+                while (j < raw.size() && raw[j].orr == raw[i].orr && raw[j].ol == raw[i].ol) ++j;
+                int orr = raw[i].orr, ol = raw[i].ol;
+                // spin filter of SU2::gemm(ret[b1], transpose(bra_conj), tmp, spin(b1)) (move_boundary.hpp:224);
+                // the sigma's tasks passed the same triangle test (apply_op.hpp:163)
+                if (!s.su2 || tri(spin2(s.left.q[ol]), a, spin2(s.left.q[orr]))) {
+                    RowBlk rb{b1, ol, orr, out.Mr[orr]};
+                    out.Mr[orr] += s.lsize(ol);
+                    out.rowblks.push_back(rb);
+                    int lsz = s.lsize(ol);
+                    Q orq = s.left.q[orr];
+                    for (int p = 0; p < s.np; ++p) {
+                        int r = s.ri.get(orq + s.phys.q[p]);
+                        if (r < 0) continue;
+                        int rsz = s.rsize(r);
+                        for (int ss = 0; ss < s.phys.s[p]; ++ss) {
+                            int col0 = s.pbr[(size_t)p * s.nr + r] + ss * rsz;
+                            Strip st;
+                            st.dst = 0;  // set below, once every BUF height is known
+                            st.rows = lsz;
+                            st.cols = rsz;
+                            st.ldd = 0;
+                            st.s0 = (int)P.wapply.srcs.size();
+                            for (size_t q = i; q < j; ++q)
+                                if (raw[q].col0 == col0) P.wapply.srcs.push_back({raw[q].off, raw[q].ld, raw[q].coef});
+                            st.s1 = (int)P.wapply.srcs.size();
+                            P.wapply.strips.push_back(st);
+                        }
                     }
                 }
+                i = j;
             }
-            i = j;
         }
     }
     out.buf_off.assign(s.nl, -1);
