@@ -43,10 +43,10 @@ class SiteOp:
         return sorted(self.blocks.keys(), reverse=True)
 
     def scaled(self, s):
-        r = self.copy()
-        for k in r.blocks:
-            r.blocks[k] = r.blocks[k] * s
-        return r
+        # blocks and spin-basis lists are never changed in place (iadd replaces
+        # them), so the scaled operator shares the lists and owns its dicts
+        sb = None if self.spin_basis is None else dict(self.spin_basis)
+        return SiteOp({k: v * s for k, v in self.blocks.items()}, self.twoS, self.action, sb)
 
     def iadd(self, other):
         """SiteOperator += (site_operator.hpp:92-103, extend_spin_basis)."""
@@ -107,10 +107,12 @@ class SiteOp:
                 rs = [abs(cin[1])] * m.shape[0]
                 cs = [abs(cout[1])] * m.shape[1]
             ents = []
-            for i in range(m.shape[0]):
-                for j in range(m.shape[1]):
-                    if m[i, j] != 0.0:
-                        ents.append((i, j, rs[i] if i < len(rs) else 0, cs[j] if j < len(cs) else 0, float(m[i, j])))
+            ncs = len(cs)
+            for i, row in enumerate(m.tolist()):
+                ri = rs[i] if i < len(rs) else 0
+                for j, v in enumerate(row):
+                    if v != 0.0:
+                        ents.append((i, j, ri, cs[j] if j < ncs else 0, v))
             out.append((cin, cout, m.shape[0], m.shape[1], ents))
         self._sparse = out
         return out
