@@ -14,6 +14,7 @@
 
 #include <array>
 #include <atomic>
+#include <cstdlib>
 #include <exception>
 #include <mutex>
 #include <thread>
@@ -115,13 +116,23 @@ struct Site {
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-// f(i) for i < n on up to 16 host threads, at least `grain` indices per thread,
-// taken `chunk` at a time; f must only write data owned by index i.  Exceptions
-// are rethrown on the calling thread.
+// Host threads of the plan builder: MQ_HOST_THREADS (1..64) when set, else the
+// hardware's, at most 16.  Plans do not depend on it.
+int host_threads() {
+    static const int nt = [] {
+        const char *e = std::getenv("MQ_HOST_THREADS");
+        if (e && std::atoi(e) >= 1) return std::min(64, std::atoi(e));
+        return (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    }();
+    return nt;
+}
+
+// f(i) for i < n on the host threads, at least `grain` indices per thread, taken
+// `chunk` at a time; f must only write data owned by index i.  Exceptions are
+// rethrown on the calling thread.
 template <class F>
 void parallel_for(int n, F &&f, int grain = 64, int chunk = 16) {
-    int nt = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
-    nt = std::min(nt, std::max(1, n / grain));
+    int nt = std::min(host_threads(), std::max(1, n / grain));
     if (nt <= 1) {
         for (int i = 0; i < n; ++i) f(i);
         return;

@@ -121,3 +121,22 @@ def test_rbtm_plans_match_oracle(fixtures, oracle_engine):
                 assert abs(h1.exec_flops - fl) <= 0.05 * fl, (site, h1.exec_flops, fl)
             d.microiteration(1, site)
     assert n_rbtm >= 4
+
+
+def test_plans_do_not_depend_on_the_host_thread_count():
+    """The builder gathers columns, rows and fused groups on several host threads
+    and emits them in a fixed order: plans built on 1 and on 8 threads are equal
+    field by field (every task list, arena size and FLOP count)."""
+    import json
+    import subprocess
+    import sys
+    res = {}
+    for nt in ("1", "8"):
+        env = dict(os.environ, MQ_HOST_THREADS=nt)
+        p = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "plan_digest.py")], env=env,
+                           capture_output=True, text=True, timeout=600)
+        assert p.returncode == 0, p.stderr[-2000:]
+        res[nt] = json.loads(p.stdout.strip().splitlines()[-1])
+    assert len(res["1"]) > 100
+    diff = [k for k in res["1"] if res["1"][k] != res["8"].get(k)]
+    assert not diff, diff[:10]
