@@ -88,9 +88,10 @@ def _energies(d, E):
     return list(d.root_energies[-1]) if hasattr(d, "root_energies") else [E]
 
 
-def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print, roots=1):
+def run_compare(ham, ramp, jd_maxiter, jd_tol, seed, log=print, roots=1, mpo_workers=None):
     """FP64 sweeps; every local problem is also solved in the mixed and BF16x3 forms."""
     from .gpu import GpuEngine
+    ham.prepare_twosite(workers=mpo_workers, log=log)  # forks: before the GPU is initialised
     eng = _CompareEngine(GpuEngine(0, prec=MQ_PREC_FP64))
     eng.set_ref_flops(False)
     try:
@@ -122,15 +123,17 @@ TIMERS = ("t_site", "t_plan", "t_iter", "t_down", "t_split", "t_step")
 
 
 def run(ham, prec, ramp, jd_maxiter, jd_tol, seed, log=print, eigensolver="jd", roots=1, converge=0.0,
-        max_sweeps=0, split="engine"):
+        max_sweeps=0, split="engine", mpo_workers=None):
     """Sweeps with the bond dimensions of `ramp`; with converge > 0, further sweeps
     at the last bond dimension until the sweep energy changes by less than
     `converge` (at most max_sweeps sweeps in all).  Each sweep's record sums the
     microiterations' time per part: two-site tensor and MPO (t_site), plan build
     with boundary binding (t_plan), the solve (t_iter), result download (t_down),
     SVD truncation (t_split: on the device, or with split="host" on the host) and
-    boundary step (t_step)."""
+    boundary step (t_step).  The two-site MPOs are built first, on mpo_workers
+    processes (default min(16, CPUs); 1 builds them in the sweep as needed)."""
     from .gpu import GpuEngine
+    ham.prepare_twosite(workers=mpo_workers, log=log)  # forks: before the GPU is initialised
     eng = GpuEngine(0, prec=prec)
     eng.set_ref_flops(False)  # the sweep needs no reference FLOP counts
     try:
@@ -183,6 +186,8 @@ def main(argv=None):
     ap.add_argument("--max-sweeps", type=int, default=30)
     ap.add_argument("--split", default="engine", choices=["engine", "host"],
                     help="SVD truncation on the device (engine, default) or on the host")
+    ap.add_argument("--mpo-workers", type=int, default=None,
+                    help="processes that build the two-site MPOs before the sweeps (default min(16, CPUs); 1: none)")
     ap.add_argument("--out")
     a = ap.parse_args(argv)
     if a.fcidump:
@@ -192,7 +197,7 @@ def main(argv=None):
         ham = Hamiltonian(a.L, a.L, 0, synthetic(a.L, seed=1000 + a.L))
     ramp = [int(v) for v in a.ramp.split(",")]
     if a.prec == "compare":
-        res = run_compare(ham, ramp, a.jd_maxiter, a.jd_tol, a.seed, roots=a.roots)
+        res = run_compare(ham, ramp, a.jd_maxiter, a.jd_tol, a.seed, roots=a.roots, mpo_workers=a.mpo_workers)
         rec = dict(L=ham.L, ramp=ramp, roots=a.roots, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol, mode="compare",
                    results=res)
         if a.out:
@@ -206,7 +211,7 @@ def main(argv=None):
         ap.error(f"unknown form(s) {unknown} in --prec {a.prec}")
     precs = [(n, forms[n]) for n in names]
     res = {name: run(ham, p, ramp, a.jd_maxiter, a.jd_tol, a.seed, eigensolver=a.eigensolver, roots=a.roots,
-                     converge=a.converge, max_sweeps=a.max_sweeps, split=a.split)
+                     converge=a.converge, max_sweeps=a.max_sweeps, split=a.split, mpo_workers=a.mpo_workers)
            for name, p in precs}
     rec = dict(L=ham.L, ramp=ramp, roots=a.roots, jd_maxiter=a.jd_maxiter, jd_tol=a.jd_tol,
                eigensolver=a.eigensolver, split=a.split, results=res)

@@ -69,10 +69,64 @@ class Hamiltonian:
     def ts_mpo(self, p):
         w = self._ts.get(p)
         if w is None:
-            w = make_twosite_mpo(self.mpo[p], self.mpo[p + 1], self.phys, self.phys, su2=self.su2,
-                                 progress=getattr(self, "_progress", None))
+            w = self._make_ts(p)
             self._ts[p] = w
         return w
+
+    def _make_ts(self, p):
+        return make_twosite_mpo(self.mpo[p], self.mpo[p + 1], self.phys, self.phys, su2=self.su2,
+                                progress=getattr(self, "_progress", None))
+
+    def prepare_twosite(self, workers=None, log=None):
+        """Build the flattened two-site MPO of every site pair (the arrays the GPU
+        plans read) in `workers` forked processes, default min(16, CPUs).  ts_mpo(p)
+        then returns a FlatTwoSiteMPO, which builds the full tensor in this process
+        only if something other than a plan asks for it.  Call it before the GPU is
+        initialised: the workers are forked from this process."""
+        import multiprocessing as mp
+        import os
+        global _PREP_HAM
+        todo = [p for p in range(self.L - 1) if p not in self._ts]
+        n = min(len(todo), workers or min(16, os.cpu_count() or 1))
+        if n <= 1:
+            return
+        t0 = time.perf_counter()
+        _PREP_HAM = self
+        try:
+            with mp.get_context("fork").Pool(n) as pool:
+                for p, arrays in pool.imap_unordered(_flat_twosite, todo):
+                    self._ts[p] = FlatTwoSiteMPO(self, p, arrays)
+        finally:
+            _PREP_HAM = None
+        if log:
+            log(f"two-site MPOs of {len(todo)} site pairs built on {n} processes in {time.perf_counter() - t0:.1f} s")
+
+
+_PREP_HAM = None  # the Hamiltonian whose two-site MPOs forked workers build
+
+
+def _flat_twosite(p):
+    from .desc import MPOArrays
+    return p, MPOArrays(_PREP_HAM._make_ts(p))
+
+
+class FlatTwoSiteMPO:
+    """A two-site MPO tensor held as its flattened arrays (desc.MPOArrays: what the
+    plans read); any other attribute builds the full tensor (make_twosite_mpo) on
+    first use and is read from it."""
+
+    def __init__(self, ham, p, arrays):
+        self.row_dim, self.col_dim = arrays.row_dim, arrays.col_dim
+        self._mq_arrays = arrays
+        self._ham, self._p, self._full = ham, p, None
+
+    def __getattr__(self, k):  # attributes not set in __init__
+        if k.startswith("__") or "_full" not in self.__dict__:
+            raise AttributeError(k)
+        if self._full is None:
+            self._full = self._ham._make_ts(self._p)
+            self._full._mq_arrays = self._mq_arrays
+        return getattr(self._full, k)
 
 
 def n_microiterations(L):
