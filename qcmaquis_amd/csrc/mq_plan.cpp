@@ -597,6 +597,8 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     std::vector<std::vector<std::pair<int, int>>> fb_panel(ng);  // per group block: (panel, row)
     for (int gi = 0; gi < ng; ++gi) {
         FGroup &g = cp->fg[gi];
+        g.blk.reserve(cand[gi].size());
+        fb_panel[gi].reserve(cand[gi].size());
         for (auto &c : cand[gi]) {
             int &pid = panel_id[((size_t)c.lk * s.np + g.pin) * nrow_states + g.row];
             if (pid < 0) {
@@ -646,14 +648,25 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
     });
     std::vector<uint64_t> comb_key;  // (lk, lb): strips reading one LA_lk row range run together
     if (comb) {
-        // 4. (sequential) strips and source ranges in group order
+        // 4. strips and source ranges in group order: offsets by prefix sums, then
+        //    (parallel) each group's strips
         std::vector<int> gbase(ng + 1, (int)comb->srcs.size());
+        std::vector<size_t> sbase(ng + 1, comb->strips.size());
         for (int gi = 0; gi < ng; ++gi) {
+            int64_t nsrc = 0;
+            for (int c : cnt[gi]) nsrc += c;
+            if (gbase[gi] + nsrc > INT32_MAX) throw std::runtime_error("fused-row combination: too many sources");
+            gbase[gi + 1] = gbase[gi] + (int)nsrc;
+            sbase[gi + 1] = sbase[gi] + cp->fg[gi].blk.size();
+        }
+        comb->strips.resize(sbase[ng]);
+        comb_key.resize(sbase[ng] - sbase[0]);
+        parallel_for(ng, [&](int gi) {
             const FGroup &g = cp->fg[gi];
             int base = gbase[gi];
             for (size_t k = 0; k < g.blk.size(); ++k) {
                 const FBlk &fb = g.blk[k];
-                Strip st;
+                Strip &st = comb->strips[sbase[gi] + k];
                 st.dst = fb.lf;
                 st.rows = s.lsize(fb.lb);
                 st.cols = s.lsize(fb.lk);
@@ -661,11 +674,9 @@ TLay t_layout(const Site &s, const BLayout &L, const PRP &prp, ColProg *cp, std:
                 st.s0 = base;
                 st.s1 = base + cnt[gi][k];
                 base = st.s1;
-                comb->strips.push_back(st);
-                comb_key.push_back(((uint64_t)fb.lb << 32) | (uint32_t)fb.lk);
+                comb_key[sbase[gi] - sbase[0] + k] = ((uint64_t)fb.lb << 32) | (uint32_t)fb.lk;
             }
-            gbase[gi + 1] = base;
-        }
+        });
         comb->srcs.resize(gbase[ng]);
         // 5. (parallel) sources of each block, in member order, into the group's range
         parallel_for(ng, [&](int gi) {
@@ -882,59 +893,120 @@ struct RowStrips {
     }
 };
 
-// W-application strips of one segment; dst offsets are relative to the al block start
-// (patched with the C arena offset of al once every K_al is known).  The row strips
-// partition the al block; they are walked in row order alongside the sources, which
-// are sorted by row0 inside the segment.
-void emit_segment_strips(const Site &s, const Seg &sg, const Raw *r0, const Raw *r1, RowStrips &rows, StripList &sl,
-                         std::vector<int> &strip_al) {
-    int xi = s.x_of_r[sg.al];
-    int rows_al = s.X.blk[xi].rows;
-    int rsz = s.rsize(sg.ar);
-    const std::vector<std::pair<int, int>> &rs = rows.get(s, sg.al);
+// Walks the row strips of segment sg's output block alongside the segment's sources
+// [r0, r1), which are sorted by row0: strip(row0, rows, q0, q1) receives the sources
+// [q0, q1) whose row0 is the strip's.  Returns the sources placed.
+template <class F>
+int64_t walk_segment(const std::vector<std::pair<int, int>> &rs, const Raw *r0, const Raw *r1, F &&strip) {
     const Raw *q = r0;
+    int64_t placed = 0;
     for (auto &r : rs) {
-        Strip st;
-        st.dst = (int64_t)sg.koff * rows_al + r.first;
-        st.rows = r.second;
-        st.cols = rsz;
-        st.ldd = rows_al;
-        st.s0 = (int)sl.srcs.size();
         while (q < r1 && q->row0 < r.first) ++q;
-        while (q < r1 && q->row0 == r.first) {
-            sl.srcs.push_back({q->off, q->ld, q->coef});
-            ++q;
-        }
-        st.s1 = (int)sl.srcs.size();
-        sl.strips.push_back(st);
-        strip_al.push_back(sg.al);
+        const Raw *q0 = q;
+        while (q < r1 && q->row0 == r.first) ++q;
+        strip(r.first, r.second, q0, q);
+        placed += q - q0;
     }
     if (q != r1) throw std::runtime_error("W-application source outside its segment");
+    return placed;
 }
 
-// The columns' W-application sources (lbtm_column), gathered and sorted in parallel
-// a batch of columns at a time (bounded memory) and handed to emit(b2, raw, x) in
-// column order, x = extra(b2) computed beside the gathering.  skip_rskip: the
-// columns stored as their hermitian partner are left out.
-template <class Extra, class Emit>
-void column_sources(const Site &s, const TLay &t, const ColProg &cp, bool skip_rskip, Extra &&extra, Emit &&emit) {
+// The W-application strips of every column (lbtm_column), in column order, which is
+// the plan's strip and source order.  A batch of columns at a time:
+//  1. (parallel) each column's sources are gathered and sorted, cut into segments
+//     (al, ar), which accept(sg) keeps or drops (and may complete, e.g. with the R
+//     operand), and counted; x = extra(b2) is computed beside;
+//  2. (in column order) each kept segment gets its K offset inside its al block
+//     (koff) and its first strip and source; on_seg(sg) sees them in order;
+//  3. (parallel) the strips and sources are written at their places.
+// Strip dst offsets are relative to the al block start (patch_strips adds the C
+// arena offset of al once every K_al is known).  skip_rskip: the columns stored as
+// their hermitian partner are left out.  Returns the sum of extra over the columns.
+template <class Extra, class Accept, class OnSeg>
+double column_strips(const Site &s, const TLay &t, const ColProg &cp, bool skip_rskip, Extra &&extra, Accept &&accept,
+                   OnSeg &&on_seg, std::vector<int> &K, std::vector<Seg> &segs, StripList &sl,
+                   std::vector<int> &strip_al) {
+    RowStrips rows(s);
+    for (int al = 0; al < s.nr; ++al)
+        if (s.x_of_r[al] >= 0) rows.get(s, al);  // filled here: read-only in the parallel phases
+    struct CSeg {
+        Seg sg;
+        size_t i, j;          // its sources in the column's raw list
+        int64_t strip0, src0;  // its first strip and source in the plan's lists
+    };
     const int batch = 512;
-    std::vector<std::vector<Raw>> raws(std::min(batch, s.W.cols));
-    std::vector<double> ex(raws.size(), 0.0);
+    const int nbuf = std::min(batch, s.W.cols);
+    std::vector<std::vector<Raw>> raws(nbuf);
+    std::vector<std::vector<CSeg>> cs(nbuf);
+    std::vector<double> ex(nbuf, 0.0);
+    double extra_sum = 0.0;
     for (int c0 = 0; c0 < s.W.cols; c0 += batch) {
         const int nb = std::min(batch, s.W.cols - c0);
         parallel_for(
             nb,
             [&](int k) {
                 const int b2 = c0 + k;
+                cs[k].clear();
                 if (skip_rskip && s.W.rskip(b2)) return;
-                lbtm_column(s, t, &cp, cp.col[b2], b2, raws[k]);
+                std::vector<Raw> &raw = raws[k];
+                lbtm_column(s, t, &cp, cp.col[b2], b2, raw);
                 ex[k] = extra(b2);
+                size_t i = 0;
+                while (i < raw.size()) {
+                    size_t j = i;
+                    while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
+                    CSeg c{Seg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0}, i, j, 0, 0};
+                    if (accept(c.sg)) cs[k].push_back(c);
+                    i = j;
+                }
             },
             8, 1);
-        for (int k = 0; k < nb; ++k)
-            if (!(skip_rskip && s.W.rskip(c0 + k))) emit(c0 + k, raws[k], ex[k]);
+        int64_t strip_n = (int64_t)sl.strips.size(), src_n = (int64_t)sl.srcs.size();
+        for (int k = 0; k < nb; ++k) {
+            const int b2 = c0 + k;
+            if (skip_rskip && s.W.rskip(b2)) continue;
+            for (auto &c : cs[k]) {
+                const std::vector<std::pair<int, int>> &rs = rows.of[c.sg.al];
+                c.sg.koff = K[c.sg.al];
+                K[c.sg.al] += s.rsize(c.sg.ar);
+                c.strip0 = strip_n;
+                c.src0 = src_n;
+                strip_n += (int64_t)rs.size();
+                src_n += walk_segment(rs, raws[k].data() + c.i, raws[k].data() + c.j,
+                                      [](int, int, const Raw *, const Raw *) {});
+                segs.push_back(c.sg);
+                on_seg(c.sg);
+            }
+            extra_sum += ex[k];
+        }
+        if (src_n > INT32_MAX) throw std::runtime_error("W application: too many sources");
+        sl.strips.resize((size_t)strip_n);
+        sl.srcs.resize((size_t)src_n);
+        strip_al.resize((size_t)strip_n);
+        parallel_for(
+            nb,
+            [&](int k) {
+                for (const CSeg &c : cs[k]) {
+                    const int rows_al = s.X.blk[s.x_of_r[c.sg.al]].rows;
+                    const int rsz = s.rsize(c.sg.ar);
+                    int64_t st_i = c.strip0, src_i = c.src0;
+                    walk_segment(rows.of[c.sg.al], raws[k].data() + c.i, raws[k].data() + c.j,
+                                 [&](int row0, int nrows, const Raw *q0, const Raw *q1) {
+                                     Strip &st = sl.strips[st_i];
+                                     st.dst = (int64_t)c.sg.koff * rows_al + row0;
+                                     st.rows = nrows;
+                                     st.cols = rsz;
+                                     st.ldd = rows_al;
+                                     st.s0 = (int)src_i;
+                                     for (const Raw *q = q0; q < q1; ++q) sl.srcs[src_i++] = {q->off, q->ld, q->coef};
+                                     st.s1 = (int)src_i;
+                                     strip_al[st_i++] = c.sg.al;
+                                 });
+                }
+            },
+            8, 1);
     }
+    return extra_sum;
 }
 
 void patch_strips(StripList &sl, const std::vector<int> &strip_al, const std::vector<int64_t> &c_off) {
@@ -1042,26 +1114,13 @@ HostPlan build_sigma_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     TLay tall;
     if (ref_lbtm) tall = t_layout(s, L, prp, nullptr, nullptr, nullptr, nullptr, tm, tn, kc);
     double ref_axpy = 0, ref_trim = 0;
-    RowStrips rstrips(s);
-    column_sources(
+    ref_axpy = column_strips(
         s, t, cp, false, [&](int b2) { return ref_lbtm ? reference_axpy_flops(s, tall, b2) : 0.0; },
-        [&](int b2, const std::vector<Raw> &raw, double axpy) {
-            ref_axpy += axpy;
-            size_t i = 0;
-            while (i < raw.size()) {
-                size_t j = i;
-                while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
-                Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
-                if (r_operand(s, R, b2, s.right.q[sg.ar], s.right.q[sg.al], &sg.rblk, &sg.rtrans, &sg.rscale)) {
-                    sg.koff = K[sg.al];
-                    K[sg.al] += s.rsize(sg.ar);
-                    segs.push_back(sg);
-                    emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, rstrips, P.wapply, strip_al);
-                    ref_trim += 2.0 * s.X.blk[s.x_of_r[sg.al]].rows * s.rsize(sg.ar) * s.rsize(sg.al);
-                }
-                i = j;
-            }
-        });
+        [&](Seg &sg) {
+            return r_operand(s, R, sg.b2, s.right.q[sg.ar], s.right.q[sg.al], &sg.rblk, &sg.rtrans, &sg.rscale);
+        },
+        [&](const Seg &sg) { ref_trim += 2.0 * s.X.blk[s.x_of_r[sg.al]].rows * s.rsize(sg.ar) * s.rsize(sg.al); }, K,
+        segs, P.wapply, strip_al);
     std::vector<int64_t> c_off(s.nr, 0), rb_off(s.nr, 0);
     int64_t ct = 0, rbt = 0;
     for (auto &xb : s.X.blk) {
@@ -1155,26 +1214,13 @@ HostPlan build_lstep_plan(const mq_site_desc &d, int tm, int tn, int kc, FuseRul
     std::vector<int> K(s.nr, 0);
     std::vector<Seg> segs;
     std::vector<int> strip_al;
-    RowStrips rstrips(s);
-    column_sources(
+    axf = column_strips(
         s, t, cp, true, [&](int b2) { return ref_count ? reference_axpy_flops(s, tall, b2) : 0.0; },
-        [&](int b2, const std::vector<Raw> &raw, double axpy) {
-            axf += axpy;
-            size_t i = 0;
-            while (i < raw.size()) {
-                size_t j = i;
-                while (j < raw.size() && raw[j].al == raw[i].al && raw[j].ar == raw[i].ar) ++j;
-                Seg sg{b2, raw[i].al, raw[i].ar, 0, nullptr, false, 1.0};
-                // spin filter of SU2::gemm(transpose(contr), bra_conj, ret[b2], spin(b2)) (move_boundary.hpp:165)
-                if (!s.su2 || tri(spin2(s.right.q[sg.ar]), s.W.rspin[b2], spin2(s.right.q[sg.al]))) {
-                    sg.koff = K[sg.al];
-                    K[sg.al] += s.rsize(sg.ar);
-                    segs.push_back(sg);
-                    emit_segment_strips(s, sg, raw.data() + i, raw.data() + j, rstrips, P.wapply, strip_al);
-                }
-                i = j;
-            }
-        });
+        [&](Seg &sg) {
+            // spin filter of SU2::gemm(transpose(contr), bra_conj, ret[b2], spin(b2)) (move_boundary.hpp:165)
+            return !s.su2 || tri(spin2(s.right.q[sg.ar]), s.W.rspin[sg.b2], spin2(s.right.q[sg.al]));
+        },
+        [](const Seg &) {}, K, segs, P.wapply, strip_al);
     std::vector<int64_t> c_off(s.nr, 0), tmp_off(s.nr, 0);
     int64_t ct = 0, tt = 0;
     for (auto &xb : s.X.blk) {
