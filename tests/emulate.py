@@ -149,8 +149,8 @@ def run_diag(hp, Ldata, Rdata):
 
 
 def jacobi_svd(M, max_sweeps=60):
-    """numpy executor of the device split's block SVD (k_jacobi_svd and
-    k_split_assemble): the working matrix X is M (rows >= cols) or M^T, its
+    """numpy executor of the device split's block SVD (k_jacobi_init,
+    k_jacobi_round, k_jacobi_sweep_end, k_jacobi_finish and k_split_assemble): the working matrix X is M (rows >= cols) or M^T, its
     columns are rotated pairwise in the kernel's round-robin order until a sweep
     rotates nothing, and the triplets come out sorted by descending singular
     value with ties broken by column.  Returns (U, s, Vt, sweeps) with
