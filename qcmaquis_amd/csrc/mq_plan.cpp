@@ -17,10 +17,79 @@
 #include <cstdlib>
 #include <exception>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <unordered_map>
 
 namespace mq {
+
+// ---------------- task-list buffer cache
+// Released large buffers are kept (at most kTaskCacheKeep of them, kTaskCacheBytes in
+// all) and handed out again for requests they fit without wasting more than half;
+// every buffer is allocated with a capacity rounded up to 2 MiB, recorded by address.
+namespace {
+constexpr size_t kTaskCacheKeep = 16, kTaskCacheBytes = size_t(2) << 30, kTaskRound = size_t(2) << 20;
+struct TaskCache {
+    std::mutex mu;
+    std::vector<std::pair<void *, size_t>> free_;  // (buffer, capacity), oldest first
+    std::unordered_map<void *, size_t> cap;        // capacity of every buffer handed out
+    size_t held = 0;
+};
+TaskCache &task_cache() {
+    static TaskCache *c = new TaskCache();  // never destroyed: buffers may be released at exit
+    return *c;
+}
+}  // namespace
+
+void *task_alloc(size_t bytes) {
+    TaskCache &tc = task_cache();
+    {
+        std::lock_guard<std::mutex> lk(tc.mu);
+        size_t best = tc.free_.size();
+        for (size_t i = 0; i < tc.free_.size(); ++i) {
+            const size_t c = tc.free_[i].second;
+            if (c >= bytes && c <= 2 * bytes && (best == tc.free_.size() || c < tc.free_[best].second)) best = i;
+        }
+        if (best < tc.free_.size()) {
+            void *p = tc.free_[best].first;
+            tc.held -= tc.free_[best].second;
+            tc.cap[p] = tc.free_[best].second;
+            tc.free_.erase(tc.free_.begin() + best);
+            return p;
+        }
+    }
+    const size_t c = (bytes + kTaskRound - 1) / kTaskRound * kTaskRound;
+    void *p = std::malloc(c);
+    if (!p) throw std::bad_alloc();
+    std::lock_guard<std::mutex> lk(tc.mu);
+    tc.cap[p] = c;
+    return p;
+}
+
+void task_free(void *p, size_t) noexcept {
+    if (!p) return;
+    TaskCache &tc = task_cache();
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> lk(tc.mu);
+        auto it = tc.cap.find(p);
+        const size_t c = it == tc.cap.end() ? 0 : it->second;
+        if (it != tc.cap.end()) tc.cap.erase(it);
+        if (c == 0 || c > kTaskCacheBytes) {
+            drop.push_back(p);
+        } else {
+            tc.free_.push_back({p, c});
+            tc.held += c;
+            while (tc.free_.size() > kTaskCacheKeep || tc.held > kTaskCacheBytes) {
+                drop.push_back(tc.free_.front().first);
+                tc.held -= tc.free_.front().second;
+                tc.free_.erase(tc.free_.begin());
+            }
+        }
+    }
+    for (void *q : drop) std::free(q);
+}
+
 
 namespace {
 

@@ -8,6 +8,14 @@ namespace mq {
 
 // ---- device task records (plain old data, copied to the GPU as arrays) ----
 
+// Buffers of task lists of at least kTaskCacheMin bytes come from a small
+// process-wide cache of released buffers (task_alloc / task_free): the lists of
+// consecutive plans have similar sizes (hundreds of MB at a chain's centre), and a
+// reused buffer has its pages mapped already instead of faulting fresh ones in.
+constexpr size_t kTaskCacheMin = size_t(1) << 20;
+void *task_alloc(size_t bytes);
+void task_free(void *p, size_t bytes) noexcept;
+
 // Allocator whose resize() leaves the records uninitialised: the builder writes
 // every record of a task list it sizes, often from several threads, and zero-
 // filling lists of 10^6-10^7 records first would be one more pass over them on
@@ -21,6 +29,14 @@ struct NoInitAlloc : std::allocator<T> {
     NoInitAlloc() = default;
     template <class U>
     NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    T *allocate(size_t n) {
+        if (n * sizeof(T) >= kTaskCacheMin) return static_cast<T *>(task_alloc(n * sizeof(T)));
+        return std::allocator<T>::allocate(n);
+    }
+    void deallocate(T *p, size_t n) noexcept {
+        if (n * sizeof(T) >= kTaskCacheMin) task_free(p, n * sizeof(T));
+        else std::allocator<T>::deallocate(p, n);
+    }
     template <class U>
     void construct(U *p) noexcept {
         ::new ((void *)p) U;
