@@ -1459,14 +1459,22 @@ RbtmCore rbtm_core(const Site &s, const BLayout &R, const BLayout *Lsig, HostPla
     out.Mr.assign(s.nl, 0);
     // per MPO row, in parallel a batch of rows at a time (bounded memory): the
     // row's sources, sorted and merged; then strips in row order
-    const int batch = 256;
+    const int batch = 512;
     std::vector<std::vector<RR>> raws(std::min(batch, std::max(s.W.rows, 1)));
     std::vector<double> axfs(raws.size(), 0.0);
+    std::vector<int> order;
     for (int r0 = 0; r0 < s.W.rows; r0 += batch) {
         const int nb = std::min(batch, s.W.rows - r0);
+        // the rows with the most MPO entries first, so that none is left to run alone
+        order.resize(nb);
+        for (int k = 0; k < nb; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return s.W.row_b2[r0 + x].size() > s.W.row_b2[r0 + y].size();
+        });
         parallel_for(
             nb,
-            [&](int k) {
+            [&](int kk) {
+                const int k = order[kk];
                 const int b1 = r0 + k;
                 std::vector<RR> &raw = raws[k];
                 double &axf = axfs[k];
